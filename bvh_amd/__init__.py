@@ -1,0 +1,309 @@
+"""bvh_amd — MI355X-native (gfx950) BVH build and trace engine.
+
+Python mirror of the reference builder interface (include/bvh_builder.h of the
+reference: getName/build/getNodes/getIndices/getLastBuildTimeMS/
+getTimingBreakdown) over the C ABI of include/bvh_hip.h.  The kernels run on
+the GPU through libbvhhip.so; there is no CPU path in this package.
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from . import _native as N
+from ._native import NODE_DTYPE, NativeError, ALGO_LBVH, ALGO_LBVH_PLUS, ALGO_PLOC  # noqa: F401
+
+__all__ = [
+    "Mesh", "Context", "DeviceMesh", "Tree", "LBVHBuilder", "LBVHPlusBuilder", "PLOCBuilder", "NODE_DTYPE",
+    "NativeError", "rays_sphere_camera", "rays_sphere_to_box", "rays_uniform_cube", "rays_camera",
+]
+
+
+class Mesh:
+    """Host triangle soup in SoA form: soa[k] for k = v0x v0y v0z v1x v1y v1z v2x v2y v2z."""
+
+    def __init__(self, soa: np.ndarray):
+        soa = np.ascontiguousarray(soa, dtype=np.float32)
+        if soa.ndim != 2 or soa.shape[0] != 9:
+            raise ValueError("soa must have shape (9, n)")
+        self.soa = soa
+
+    def __len__(self):
+        return self.soa.shape[1]
+
+    @property
+    def n(self):
+        return self.soa.shape[1]
+
+    @staticmethod
+    def _empty(n):
+        return np.zeros((9, n), dtype=np.float32)
+
+    @classmethod
+    def uniform(cls, seed: int, n: int) -> "Mesh":
+        soa = cls._empty(n)
+        N.host().bvh_scene_uniform(seed, n, N.soa_ptrs(soa))
+        return cls(soa)
+
+    @classmethod
+    def blobs(cls, seed: int, n: int) -> "Mesh":
+        soa = cls._empty(n)
+        N.host().bvh_scene_blobs(seed, n, N.soa_ptrs(soa))
+        return cls(soa)
+
+    @classmethod
+    def strands(cls, seed: int, strands: int, steps: int) -> "Mesh":
+        soa = cls._empty(strands * steps)
+        N.host().bvh_scene_strands(seed, strands, steps, N.soa_ptrs(soa))
+        return cls(soa)
+
+    @classmethod
+    def batch_mesh(cls, seed: int, index: int, n: int) -> "Mesh":
+        soa = cls._empty(n)
+        N.host().bvh_scene_batch_mesh(seed, index, n, N.soa_ptrs(soa))
+        return cls(soa)
+
+    def checksum(self) -> int:
+        h = 14695981039346656037
+        for k in range(9):
+            h = N.host().bvh_fnv1a64(self.soa[k].ctypes.data, self.soa[k].nbytes, h)
+        return h
+
+    def bounds(self) -> np.ndarray:
+        v = self.soa.reshape(3, 3, -1)
+        return np.concatenate([v.min(axis=(0, 2)), v.max(axis=(0, 2))]).astype(np.float32)
+
+
+def batch_counts(seed: int, meshes: int, lo: int, hi: int) -> np.ndarray:
+    out = np.zeros(meshes, dtype=np.uint32)
+    N.host().bvh_scene_batch_counts(seed, meshes, lo, hi, out.ctypes.data)
+    return out
+
+
+def rays_camera(eye, look_at, up, fov, width, height) -> np.ndarray:
+    out = np.zeros((width * height, 6), dtype=np.float32)
+    e = np.asarray(eye, np.float32); la = np.asarray(look_at, np.float32); u = np.asarray(up, np.float32)
+    N.host().bvh_rays_camera(e.ctypes.data, la.ctypes.data, u.ctypes.data, fov, width, height, out.ctypes.data)
+    return out
+
+
+def rays_sphere_camera(box, width=1024, height=768, fov=60.0) -> np.ndarray:
+    out = np.zeros((width * height, 6), dtype=np.float32)
+    b = np.asarray(box, np.float32)
+    N.host().bvh_rays_sphere_camera(b.ctypes.data, fov, width, height, out.ctypes.data)
+    return out
+
+
+def rays_sphere_to_box(seed, box, n) -> np.ndarray:
+    out = np.zeros((n, 6), dtype=np.float32)
+    b = np.asarray(box, np.float32)
+    N.host().bvh_rays_sphere_to_box(seed, b.ctypes.data, n, out.ctypes.data)
+    return out
+
+
+def rays_uniform_cube(seed, n) -> np.ndarray:
+    out = np.zeros((n, 6), dtype=np.float32)
+    N.host().bvh_rays_uniform_cube(seed, n, out.ctypes.data)
+    return out
+
+
+class Context:
+    """One HIP stream + pooled workspace on one device (bvh_ctx_t)."""
+
+    def __init__(self, device: int = 0):
+        self._lib = N.hip()
+        h = C.c_void_p()
+        st = self._lib.bvh_ctx_create(device, C.byref(h))
+        if st != N.BVH_OK:
+            raise NativeError(st, "bvh_ctx_create")
+        self.h = h
+
+    def _check(self, st, what=""):
+        if st != N.BVH_OK:
+            raise NativeError(st, f"{what}: {self._lib.bvh_last_error(self.h).decode()}")
+
+    def close(self):
+        if getattr(self, "h", None):
+            self._lib.bvh_ctx_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def upload(self, mesh: Mesh) -> "DeviceMesh":
+        h = C.c_void_p()
+        self._check(self._lib.bvh_mesh_upload(self.h, N.soa_ptrs(mesh.soa), mesh.n, C.byref(h)), "mesh upload")
+        return DeviceMesh(self, h, mesh.n)
+
+    def tree(self) -> "Tree":
+        h = C.c_void_p()
+        self._check(self._lib.bvh_tree_create(self.h, C.byref(h)), "tree create")
+        return Tree(self, h)
+
+    def build(self, dmesh: "DeviceMesh", tree: "Tree", algo=ALGO_LBVH, radius=25, rounds=1, radix_bits=0,
+              keep_keys=False) -> dict:
+        p = N.BuildParams(algo, radius, rounds, radix_bits, N.BUILD_KEEP_KEYS if keep_keys else 0)
+        t = N.StageTimes()
+        self._check(self._lib.bvh_build(self.h, dmesh.h, C.byref(p), tree.h, C.byref(t)), "build")
+        return {"total_ms": float(t.total_ms), "stages": t.as_dict(), "ploc_iterations": int(t.ploc_iterations)}
+
+    def debug_morton(self, dmesh: "DeviceMesh", algo=ALGO_LBVH) -> np.ndarray:
+        out = np.zeros(dmesh.n, dtype=np.uint64)
+        self._check(self._lib.bvh_debug_morton(self.h, dmesh.h, algo, out.ctypes.data), "debug_morton")
+        return out
+
+    def debug_sorted_keys(self, n: int):
+        k = np.zeros(n, dtype=np.uint64)
+        v = np.zeros(n, dtype=np.uint32)
+        self._check(self._lib.bvh_debug_sorted_keys(self.h, k.ctypes.data, v.ctypes.data, n), "sorted keys")
+        return k, v
+
+    def sort_pairs(self, keys: np.ndarray, vals=None, begin_bit=0, end_bit=64, radix_bits=0):
+        """Sorts host arrays on the device (stable LSD over [begin_bit, end_bit))."""
+        keys = np.ascontiguousarray(keys, dtype=np.uint64)
+        n = keys.size
+        dk = C.c_void_p()
+        self._check(self._lib.bvh_device_alloc(self.h, max(8, keys.nbytes), C.byref(dk)), "alloc")
+        dv = C.c_void_p()
+        try:
+            self._check(self._lib.bvh_copy_h2d(self.h, dk, keys.ctypes.data, keys.nbytes), "h2d")
+            if vals is not None:
+                vals = np.ascontiguousarray(vals, dtype=np.uint32)
+                self._check(self._lib.bvh_device_alloc(self.h, max(4, vals.nbytes), C.byref(dv)), "alloc")
+                self._check(self._lib.bvh_copy_h2d(self.h, dv, vals.ctypes.data, vals.nbytes), "h2d")
+            ms = C.c_float(0)
+            self._check(self._lib.bvh_sort_pairs_u64(self.h, dk, dv if vals is not None else None, n, begin_bit,
+                                                      end_bit, radix_bits, C.byref(ms)), "sort")
+            ok = np.empty_like(keys)
+            self._check(self._lib.bvh_copy_d2h(self.h, ok.ctypes.data, dk, keys.nbytes), "d2h")
+            ov = None
+            if vals is not None:
+                ov = np.empty_like(vals)
+                self._check(self._lib.bvh_copy_d2h(self.h, ov.ctypes.data, dv, vals.nbytes), "d2h")
+            return ok, ov, float(ms.value)
+        finally:
+            self._lib.bvh_device_free(self.h, dk)
+            if dv.value:
+                self._lib.bvh_device_free(self.h, dv)
+
+
+class DeviceMesh:
+    def __init__(self, ctx: Context, h, n):
+        self.ctx, self.h, self.n = ctx, h, n
+
+    def free(self):
+        if self.h:
+            self.ctx._lib.bvh_mesh_free(self.ctx.h, self.h)
+            self.h = None
+
+
+class Tree:
+    def __init__(self, ctx: Context, h):
+        self.ctx, self.h = ctx, h
+
+    def info(self):
+        nn, npr = C.c_uint32(), C.c_uint32()
+        self.ctx._lib.bvh_tree_info(self.h, C.byref(nn), C.byref(npr))
+        return nn.value, npr.value
+
+    def download(self):
+        nn, npr = self.info()
+        nodes = np.zeros(nn, dtype=NODE_DTYPE)
+        idx = np.zeros(npr, dtype=np.uint32)
+        self.ctx._check(self.ctx._lib.bvh_tree_download(self.ctx.h, self.h, nodes.ctypes.data, idx.ctypes.data),
+                        "download")
+        return nodes, idx
+
+    def upload_nodes(self, nodes: np.ndarray):
+        nodes = np.ascontiguousarray(nodes, dtype=NODE_DTYPE)
+        self.ctx._check(self.ctx._lib.bvh_tree_from_nodes(self.ctx.h, nodes.ctypes.data, nodes.size, self.h),
+                        "tree_from_nodes")
+
+    def free(self):
+        if self.h:
+            self.ctx._lib.bvh_tree_free(self.ctx.h, self.h)
+            self.h = None
+
+
+class _Builder:
+    """Mirror of the reference BVHBuilder (include/bvh_builder.h:10-37)."""
+    algo = ALGO_LBVH
+
+    def __init__(self, ctx: Context | None = None):
+        self._ctx = ctx
+        self._nodes = np.zeros(0, dtype=NODE_DTYPE)
+        self._indices = np.zeros(0, dtype=np.uint32)
+        self._times = {"total_ms": 0.0, "stages": {}}
+        self._params = {}
+
+    def _context(self):
+        if self._ctx is None:
+            self._ctx = Context(0)
+        return self._ctx
+
+    def build(self, mesh: Mesh) -> None:
+        ctx = self._context()
+        dm = ctx.upload(mesh)
+        tree = ctx.tree()
+        try:
+            self._times = ctx.build(dm, tree, algo=self.algo, **self._params)
+            self._nodes, self._indices = tree.download()
+        finally:
+            tree.free()
+            dm.free()
+
+    def get_nodes(self):
+        return self._nodes
+
+    def get_indices(self):
+        return self._indices
+
+    def get_last_build_time_ms(self) -> float:
+        return self._times["total_ms"]
+
+    def get_timing_breakdown(self) -> str:
+        return "\n".join(f"  {k + ':':<18}{v:.3f} ms" for k, v in self._times["stages"].items())
+
+    def cleanup(self):
+        self._nodes = np.zeros(0, dtype=NODE_DTYPE)
+        self._indices = np.zeros(0, dtype=np.uint32)
+
+
+class LBVHBuilder(_Builder):
+    algo = ALGO_LBVH
+
+    def get_name(self):
+        return "LBVH"
+
+
+class LBVHPlusBuilder(_Builder):
+    algo = ALGO_LBVH_PLUS
+
+    def __init__(self, ctx=None, rounds: int = 1):
+        super().__init__(ctx)
+        self._params = {"rounds": rounds}
+
+    def get_name(self):
+        return "LBVH+"
+
+
+class PLOCBuilder(_Builder):
+    algo = ALGO_PLOC
+
+    def __init__(self, radius: int = 25, ctx=None):
+        super().__init__(ctx)
+        self.radius = radius
+        self._params = {"radius": radius}
+
+    def get_name(self):
+        return f"PLOC CUDA (r={self.radius})"

@@ -1,0 +1,84 @@
+// engine.hpp — internal interface between the HIP kernel files and the C ABI.
+// Host-side launchers only; no device code here.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stddef.h>
+#include <stdint.h>
+
+namespace bvhk {
+
+struct DNode;
+
+// Nine SoA vertex arrays of one mesh, resident in HBM.
+struct MeshView {
+    const float* v[9];
+    uint32_t n;
+};
+
+// ---- LBVH/LBVH+/PLOC front end (k_front.hip) ----------------------------
+// Kernel geometry for the bounds and Morton passes.
+int front_grid(uint32_t n);
+
+// Resets the six order-mapped scene-bound words.
+void launch_init_scene(uint32_t* scene_ord, hipStream_t s);
+// Per-triangle boxes (2 x float4 per triangle) and the scene box reduction.
+void launch_tri_bounds(const MeshView& m, float4* tri_boxes, uint32_t* scene_ord, hipStream_t s);
+
+// Sort plan: digit width, pass count and shifts for a key bit range.
+struct SortPlan {
+    int bits;          // digit width (8, 10 or 11)
+    int passes;        // number of LSD passes
+    int shift[8];      // low bit of each pass
+    uint32_t tiles;    // onesweep tiles
+    uint32_t tile;     // keys per tile
+    int hist_blocks;   // blocks of the histogram producer
+};
+SortPlan make_sort_plan(uint32_t n, int begin_bit, int end_bit, int bits);
+// Bytes of the zero-initialised lookback area a plan needs (status words + tile counters).
+size_t sort_zero_bytes(const SortPlan& p);
+
+// Morton keys + fused all-pass digit histograms + zeroing of `zero_words` words.
+//   m60 = false: keys = morton30 << 32 | i (vals unused)
+//   m60 = true : keys = morton60, vals = i
+void launch_morton(const float4* tri_boxes, uint32_t n, const uint32_t* scene_ord, bool m60, uint64_t* keys,
+                   uint32_t* vals, const SortPlan& plan, uint32_t* hist_part, uint32_t* zero_ptr,
+                   size_t zero_words, hipStream_t s);
+// Debug: Morton codes only (no histograms).
+void launch_morton_codes(const float4* tri_boxes, uint32_t n, const uint32_t* scene_ord, bool m60,
+                         uint64_t* codes, hipStream_t s);
+
+// ---- radix sort (k_sort.hip) ---------------------------------------------
+// Histogram of keys already in memory (used by the standalone sort).
+void launch_sort_histogram(const uint64_t* keys, uint32_t n, const SortPlan& plan, uint32_t* hist_part,
+                           hipStream_t s);
+// Reduces the per-block histograms and scans them into per-pass bin starts.
+void launch_hist_scan(const uint32_t* hist_part, const SortPlan& plan, uint32_t* bin_start, hipStream_t s);
+void launch_fill_zero(uint32_t* p, size_t words, hipStream_t s);
+// All onesweep passes.  Result ends in (*keys_out, *vals_out) which is one of
+// the two buffers passed in.
+void launch_onesweep_pass(const SortPlan& plan, int pass, uint32_t n, const uint64_t* kin, uint64_t* kout,
+                          const uint32_t* vin, uint32_t* vout, const uint32_t* bin_start, uint32_t* lookback,
+                          hipStream_t s);
+void launch_onesweep(const SortPlan& plan, uint32_t n, uint64_t* keys_a, uint64_t* keys_b, uint32_t* vals_a,
+                     uint32_t* vals_b, const uint32_t* bin_start, uint32_t* lookback, uint64_t** keys_out,
+                     uint32_t** vals_out, hipStream_t s);
+
+// ---- hierarchy (k_hier.hip) ------------------------------------------------
+struct HierArgs {
+    const uint64_t* keys;   // sorted keys
+    const uint32_t* vals;   // sorted indices (60-bit keys) or nullptr (packed 30-bit keys)
+    uint32_t n;
+    const float4* tri_boxes;
+    DNode* nodes;           // 2n-1
+    uint32_t* indices;      // n
+    uint32_t* parents;      // optional, 2n-1
+    int2* ranges;           // optional, n-1 (leaf range of each internal node)
+    uint64_t* handoff;      // n words, epoch-tagged
+    uint32_t epoch;
+    uint32_t* err;          // set to nonzero when a sorted key names no primitive
+};
+int hier_chunk();
+void launch_hierarchy(const HierArgs& a, hipStream_t s);
+
+}  // namespace bvhk

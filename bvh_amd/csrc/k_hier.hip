@@ -1,0 +1,194 @@
+// k_hier.hip — fused topology, leaf initialisation and bottom-up refit.
+//
+// Replaces kBuildInternalNodes_nt/_P + kInitLeafNodes_nt/_P + kRefitHierarchy_nt
+// (reference lbvh_builder.cu:247-338, lbvh_plus_builder.cu:292-335,341-366 with
+// doOptimize = false).
+//
+// The reference finds each internal node top-down (Karras 2012: direction,
+// exponential + binary range search, binary split search; O(log n) scattered
+// key reads per node), then refits bottom-up in a second kernel through
+// per-node atomic counters and __threadfence.  Here one pass climbs from the
+// leaves: a node covering sorted leaves [l, r] is the left child of its
+// parent iff delta(r) > delta(l-1) (delta = common-prefix length of adjacent
+// keys, -1 past either end); the parent's split is then r (left child) or
+// l-1 (right child).  The radix tree of unique keys is unique, so this
+// yields exactly the reference's topology, and its node numbering follows
+// from the same rule: an internal node is numbered by its right end when it
+// is a left child and by its left end otherwise (the root is 0), and leaf k
+// is node n-1+k, as in Karras' layout.  The two children of a split meet in a
+// handoff word: the first arrival deposits its far range end and stops, the
+// second merges the boxes (left box first, as the reference merges
+// nodes[left] with nodes[right]) and climbs on.
+//
+// MI355X mapping: a workgroup owns a chunk of 2048 sorted leaves.  Every split
+// whose parent lies entirely inside the chunk (decided from prefix/suffix
+// minima of the chunk's delta values, which both children evaluate
+// identically) hands off through an LDS word with workgroup-scope ordering.
+// Only nodes that straddle chunk boundaries (the top of the tree, roughly
+// 2n/2048 nodes) hand off through an epoch-tagged 64-bit word in HBM with an
+// agent-scope acquire-release exchange, the only place that pays for
+// cross-XCD coherence.  Epoch tags mean the handoff array is never cleared.
+#include "dev_common.hpp"
+#include "engine.hpp"
+
+namespace bvhk {
+
+constexpr int kHierBlock = 256;
+constexpr int kHierItems = 8;
+constexpr int kChunk = kHierBlock * kHierItems;
+
+int hier_chunk() { return kChunk; }
+
+template <bool K60>
+__device__ __forceinline__ int key_delta(const uint64_t* __restrict__ keys, const uint32_t* __restrict__ vals,
+                                         uint32_t j) {
+    uint64_t x = keys[j];
+    uint64_t y = keys[j + 1];
+    if (K60 && x == y) {
+        uint32_t v = vals[j] ^ vals[j + 1];
+        return 96 + (v == 0 ? 32 : __clz((int)v));
+    }
+    uint64_t z = x ^ y;
+    return z == 0 ? 64 : __clzll((long long)z);
+}
+
+__device__ __forceinline__ int wave_incl_min(int v) {
+    uint32_t l = __lane_id();
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        int w = __shfl_up(v, o, 64);
+        if (l >= (uint32_t)o) v = (w < v) ? w : v;
+    }
+    return v;
+}
+
+// Inclusive min-scan of src[0..cnt) into dst (forward, or backward when REV).
+template <bool REV>
+__device__ void block_min_scan(const int8_t* src, int8_t* dst, int cnt, int* s_w) {
+    const int tid = threadIdx.x, wave = tid / kWave, lane = tid % kWave;
+    const int per = (cnt + kHierBlock - 1) / kHierBlock;
+    const int lo = tid * per;
+    const int hi = (lo + per < cnt) ? lo + per : cnt;
+    int m = 127;
+    for (int q = lo; q < hi; ++q) {
+        int v = src[REV ? cnt - 1 - q : q];
+        m = v < m ? v : m;
+    }
+    int incl = wave_incl_min(m);
+    int ex = __shfl_up(incl, 1, 64);
+    if (lane == 0) ex = 127;
+    if (lane == kWave - 1) s_w[wave] = incl;
+    __syncthreads();
+    for (int w = 0; w < wave; ++w) ex = s_w[w] < ex ? s_w[w] : ex;
+    int run = ex;
+    for (int q = lo; q < hi; ++q) {
+        int qq = REV ? cnt - 1 - q : q;
+        int v = src[qq];
+        run = v < run ? v : run;
+        dst[qq] = (int8_t)run;
+    }
+    __syncthreads();
+}
+
+template <bool K60>
+__global__ __launch_bounds__(kHierBlock) void k_hierarchy(HierArgs p) {
+    __shared__ int8_t s_d[kChunk + 1];    // delta of boundaries a-1 .. b-1
+    __shared__ int8_t s_pre[kChunk + 1];  // prefix minima of s_d
+    __shared__ int8_t s_suf[kChunk + 1];  // suffix minima of s_d
+    __shared__ uint32_t s_slot[kChunk];   // LDS handoff words (far end + 1)
+    __shared__ int s_w[kHierBlock / kWave];
+
+    const uint32_t n = p.n;
+    const uint32_t a = blockIdx.x * kChunk;
+    const uint32_t b = (a + kChunk < n) ? a + kChunk : n;
+    const int len = (int)(b - a);
+    const int tid = threadIdx.x;
+
+    for (int q = tid; q <= len; q += kHierBlock) {
+        int64_t j = (int64_t)a - 1 + q;
+        s_d[q] = (j < 0 || j >= (int64_t)n - 1) ? (int8_t)-1 : (int8_t)key_delta<K60>(p.keys, p.vals, (uint32_t)j);
+    }
+    for (int q = tid; q < len; q += kHierBlock) s_slot[q] = 0;
+    __syncthreads();
+    block_min_scan<false>(s_d, s_pre, len + 1, s_w);
+    block_min_scan<true>(s_d, s_suf, len + 1, s_w);
+
+    // delta at boundary j (-1 beyond either end), from LDS inside the chunk.
+    auto dget = [&](int64_t j) -> int {
+        if (j < 0 || j >= (int64_t)n - 1) return -1;
+        if (j >= (int64_t)a - 1 && j <= (int64_t)b - 1) return s_d[j - ((int64_t)a - 1)];
+        return key_delta<K60>(p.keys, p.vals, (uint32_t)j);
+    };
+
+    DNode* __restrict__ nodes = p.nodes;
+    for (int k = 0; k < kHierItems; ++k) {
+        const uint32_t i = a + k * kHierBlock + tid;
+        if (i >= b) continue;
+        uint32_t prim = K60 ? p.vals[i] : (uint32_t)p.keys[i];
+        if (prim >= n) {  // cannot happen for a correct sort; never index out of bounds
+            atomicOr(p.err, 1u);
+            prim = 0;
+        }
+        Box box = load_box(p.tri_boxes + 2 * (size_t)prim);
+        uint32_t my = n - 1 + i;
+        store_node(nodes, my, box, (int32_t)(prim | 0x80000000u), -1);
+        p.indices[i] = prim;
+
+        uint32_t l = i, r = i;
+        int dl = dget((int64_t)l - 1);
+        int dr = dget((int64_t)r);
+        bool local = true;
+        while (!(l == 0 && r == n - 1)) {
+            const bool isLeft = dr > dl;
+            const uint32_t s = isLeft ? r : l - 1;
+            const uint32_t far = isLeft ? l : r;
+            uint32_t other;
+            bool lds = false;
+            if (local && s >= a && s + 2 <= b) {
+                const int qs = (int)(s - a) + 1;
+                const int ds = s_d[qs];
+                lds = (s_pre[qs - 1] < ds) && (s_suf[qs + 1] < ds);
+            }
+            if (lds) {
+                uint32_t old = __hip_atomic_exchange(&s_slot[s - a], far + 1, __ATOMIC_ACQ_REL,
+                                                     __HIP_MEMORY_SCOPE_WORKGROUP);
+                if (old == 0) break;
+                other = old - 1;
+            } else {
+                local = false;
+                uint64_t w = ((uint64_t)p.epoch << 32) | far;
+                uint64_t old = __hip_atomic_exchange(&p.handoff[s], w, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+                if ((uint32_t)(old >> 32) != p.epoch) break;
+                other = (uint32_t)old;
+            }
+            uint32_t sib;
+            if (isLeft) sib = (s + 1 == other) ? (n - 1 + other) : (s + 1);
+            else sib = (other == s) ? (n - 1 + s) : s;
+            const Box sbox = node_box(nodes, sib);
+            const Box pb = isLeft ? merge(box, sbox) : merge(sbox, box);
+            const uint32_t lc = isLeft ? my : sib;
+            const uint32_t rc = isLeft ? sib : my;
+            if (isLeft) r = other;
+            else l = other;
+            dl = dget((int64_t)l - 1);
+            dr = dget((int64_t)r);
+            const uint32_t pidx = (l == 0 && r == n - 1) ? 0u : ((dr > dl) ? r : l);
+            store_node(nodes, pidx, pb, (int32_t)lc, (int32_t)rc);
+            if (p.parents) {
+                p.parents[lc] = pidx;
+                p.parents[rc] = pidx;
+            }
+            if (p.ranges) p.ranges[pidx] = make_int2((int)l, (int)r);
+            my = pidx;
+            box = pb;
+        }
+    }
+}
+
+void launch_hierarchy(const HierArgs& a, hipStream_t s) {
+    uint32_t grid = (a.n + kChunk - 1) / kChunk;
+    if (a.vals) hipLaunchKernelGGL(k_hierarchy<true>, dim3(grid), dim3(kHierBlock), 0, s, a);
+    else hipLaunchKernelGGL(k_hierarchy<false>, dim3(grid), dim3(kHierBlock), 0, s, a);
+}
+
+}  // namespace bvhk

@@ -1,0 +1,308 @@
+// k_sort.hip — stable LSD radix sort of 64-bit keys (with optional 32-bit
+// payload), one kernel per digit pass, decoupled look-back between tiles.
+//
+// Replaces cub::DeviceRadixSort::SortPairs (reference lbvh_builder.cu:485-493)
+// and thrust::sort_by_key (lbvh_plus_builder.cu:465, ploc_builder.cu:459-461).
+//
+// Design for gfx950:
+//   * digit histograms of all passes come precomputed from the Morton pass
+//     (k_front.hip) or from k_sort_histogram; k_hist_scan turns them into
+//     per-pass bin starts;
+//   * each onesweep tile (256 threads x 16 keys) takes a tile id from an
+//     atomic counter (ids follow start order, so every look-back target is
+//     already resident), ranks keys with wave64 ballot matching into
+//     per-wave LDS histograms (stable: items are consumed in input order),
+//     publishes its per-digit counts, resolves its global digit offsets by
+//     decoupled look-back over agent-scope status words (flag | count in one
+//     32-bit word, so no payload fence is needed), reorders the tile in LDS
+//     and writes each digit run contiguously.
+// Key traffic per pass: 8 B read + 8 B write per key (+4 + 4 with payload).
+#include "dev_common.hpp"
+#include "engine.hpp"
+
+namespace bvhk {
+
+constexpr int kSortBlock = 256;
+constexpr int kSortItems = 16;
+constexpr uint32_t kFlagA = 1u << 30;
+constexpr uint32_t kFlagP = 2u << 30;
+constexpr uint32_t kValMask = (1u << 30) - 1;
+
+__global__ void k_fill_zero(uint4* p, size_t n4) {
+    size_t stride = (size_t)gridDim.x * blockDim.x;
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n4; i += stride) p[i] = make_uint4(0, 0, 0, 0);
+}
+
+void launch_fill_zero(uint32_t* p, size_t words, hipStream_t s) {
+    size_t n4 = (words + 3) / 4;
+    size_t g = (n4 + 255) / 256;
+    if (g > 4096) g = 4096;
+    if (g < 1) g = 1;
+    hipLaunchKernelGGL(k_fill_zero, dim3((uint32_t)g), dim3(256), 0, s, reinterpret_cast<uint4*>(p), n4);
+}
+
+struct ShiftArr {
+    int s[8];
+};
+
+template <int BITS>
+__global__ __launch_bounds__(256) void k_sort_histogram(const uint64_t* __restrict__ keys, uint32_t n, int passes,
+                                                         ShiftArr sh, uint32_t* __restrict__ hist_part) {
+    constexpr int BINS = 1 << BITS;
+    extern __shared__ uint32_t s_hist[];
+    for (int i = threadIdx.x; i < passes * BINS; i += 256) s_hist[i] = 0;
+    __syncthreads();
+    const uint32_t stride = gridDim.x * 256;
+    for (uint32_t i = blockIdx.x * 256 + threadIdx.x; i < n; i += stride) {
+        uint64_t k = keys[i];
+        for (int p = 0; p < passes; ++p) atomicAdd(&s_hist[p * BINS + ((uint32_t)(k >> sh.s[p]) & (BINS - 1))], 1u);
+    }
+    __syncthreads();
+    uint32_t* out = hist_part + (size_t)blockIdx.x * passes * BINS;
+    for (int i = threadIdx.x; i < passes * BINS; i += 256) out[i] = s_hist[i];
+}
+
+void launch_sort_histogram(const uint64_t* keys, uint32_t n, const SortPlan& plan, uint32_t* hist_part,
+                           hipStream_t s) {
+    ShiftArr sh;
+    for (int i = 0; i < 8; ++i) sh.s[i] = plan.shift[i];
+    size_t lds = (size_t)plan.passes * (1u << plan.bits) * 4;
+    if (plan.bits == 8)
+        hipLaunchKernelGGL(k_sort_histogram<8>, dim3(plan.hist_blocks), dim3(256), lds, s, keys, n, plan.passes, sh,
+                           hist_part);
+    else
+        hipLaunchKernelGGL(k_sort_histogram<10>, dim3(plan.hist_blocks), dim3(256), lds, s, keys, n, plan.passes, sh,
+                           hist_part);
+}
+
+// Exclusive sum over a 256-thread block; returns the prefix, *total the sum.
+__device__ __forceinline__ uint32_t block_excl_sum256(uint32_t v, uint32_t* s_w, uint32_t* total) {
+    const int wave = threadIdx.x / kWave, lane = threadIdx.x % kWave;
+    uint32_t incl = wave_incl_sum(v);
+    if (lane == kWave - 1) s_w[wave] = incl;
+    __syncthreads();
+    uint32_t before = 0, all = 0;
+#pragma unroll
+    for (int w = 0; w < 256 / kWave; ++w) {
+        uint32_t x = s_w[w];
+        before += (w < wave) ? x : 0u;
+        all += x;
+    }
+    __syncthreads();
+    *total = all;
+    return before + incl - v;
+}
+
+// grid = passes; reduces per-block histograms and scans bins.
+template <int BITS>
+__global__ __launch_bounds__(256) void k_hist_scan(const uint32_t* __restrict__ hist_part, int blocks, int passes,
+                                                    uint32_t* __restrict__ bin_start) {
+    constexpr int BINS = 1 << BITS;
+    constexpr int BPT = BINS / 256;
+    __shared__ uint32_t s_w[4];
+    const int p = blockIdx.x;
+    uint32_t c[BPT];
+    uint32_t sum = 0;
+#pragma unroll
+    for (int j = 0; j < BPT; ++j) {
+        int b = threadIdx.x * BPT + j;
+        uint32_t acc = 0;
+        for (int blk = 0; blk < blocks; ++blk) acc += hist_part[((size_t)blk * passes + p) * BINS + b];
+        c[j] = acc;
+        sum += acc;
+    }
+    uint32_t tot;
+    uint32_t run = block_excl_sum256(sum, s_w, &tot);
+#pragma unroll
+    for (int j = 0; j < BPT; ++j) {
+        bin_start[p * BINS + threadIdx.x * BPT + j] = run;
+        run += c[j];
+    }
+}
+
+void launch_hist_scan(const uint32_t* hist_part, const SortPlan& plan, uint32_t* bin_start, hipStream_t s) {
+    if (plan.bits == 8)
+        hipLaunchKernelGGL(k_hist_scan<8>, dim3(plan.passes), dim3(256), 0, s, hist_part, plan.hist_blocks,
+                           plan.passes, bin_start);
+    else
+        hipLaunchKernelGGL(k_hist_scan<10>, dim3(plan.passes), dim3(256), 0, s, hist_part, plan.hist_blocks,
+                           plan.passes, bin_start);
+}
+
+template <int BITS, bool VALS>
+__global__ __launch_bounds__(kSortBlock) void k_onesweep(const uint64_t* __restrict__ kin, uint64_t* __restrict__ kout,
+                                                        const uint32_t* __restrict__ vin, uint32_t* __restrict__ vout,
+                                                        uint32_t n, int shift, const uint32_t* __restrict__ bin_start,
+                                                        uint32_t* status, uint32_t* tile_ctr) {
+    constexpr int BINS = 1 << BITS;
+    constexpr uint32_t MASK = BINS - 1;
+    constexpr int WAVES = kSortBlock / kWave;
+    constexpr int TILE = kSortBlock * kSortItems;
+    constexpr int BPT = BINS / kSortBlock;
+    __shared__ uint32_t s_whist[WAVES * BINS];
+    __shared__ uint32_t s_local[BINS];
+    __shared__ uint32_t s_global[BINS];
+    __shared__ uint64_t s_keys[TILE];
+    __shared__ uint32_t s_vals[VALS ? TILE : 1];
+    __shared__ uint32_t s_tile;
+    __shared__ uint32_t s_w[WAVES];
+
+    const int tid = threadIdx.x;
+    const int wave = tid / kWave;
+    const int lane = tid % kWave;
+    if (tid == 0) s_tile = atomicAdd(tile_ctr, 1u);
+    for (int i = tid; i < WAVES * BINS; i += kSortBlock) s_whist[i] = 0;
+    __syncthreads();
+    const uint32_t tile = s_tile;
+    if (tile >= gridDim.x) return;  // tile counter not cleared: publish nothing, touch nothing
+    const uint32_t base = tile * TILE;
+    const uint32_t seg = base + wave * (kSortItems * kWave);
+
+    uint64_t key[kSortItems];
+    uint32_t val[kSortItems];
+    uint32_t rk[kSortItems];
+#pragma unroll
+    for (int it = 0; it < kSortItems; ++it) {
+        uint32_t idx = seg + it * kWave + lane;
+        bool valid = idx < n;
+        key[it] = valid ? kin[idx] : ~0ull;
+        if (VALS) val[it] = valid ? vin[idx] : 0u;
+    }
+
+    uint32_t* wh = s_whist + wave * BINS;
+    const uint64_t lt_mask = lanemask_lt();
+#pragma unroll
+    for (int it = 0; it < kSortItems; ++it) {
+        uint32_t idx = seg + it * kWave + lane;
+        bool valid = idx < n;
+        uint32_t d = (uint32_t)(key[it] >> shift) & MASK;
+        uint64_t peers = __ballot(valid);
+#pragma unroll
+        for (int b = 0; b < BITS; ++b) {
+            bool bit = (d >> b) & 1u;
+            uint64_t bb = __ballot(bit);
+            peers &= bit ? bb : ~bb;
+        }
+        uint32_t r = 0;
+        if (valid) {
+            uint64_t lt = peers & lt_mask;
+            uint32_t old = wh[d];
+            r = old + (uint32_t)__popcll(lt);
+            if (lt == 0) wh[d] = old + (uint32_t)__popcll(peers);
+        }
+        rk[it] = r;
+    }
+    __syncthreads();
+
+    uint32_t cnt[BPT];
+    uint32_t tsum = 0;
+#pragma unroll
+    for (int j = 0; j < BPT; ++j) {
+        int b = tid * BPT + j;
+        uint32_t run = 0;
+#pragma unroll
+        for (int w = 0; w < WAVES; ++w) {
+            uint32_t c = s_whist[w * BINS + b];
+            s_whist[w * BINS + b] = run;
+            run += c;
+        }
+        cnt[j] = run;
+        tsum += run;
+        uint32_t word = (tile == 0 ? kFlagP : kFlagA) | run;
+        __hip_atomic_store(status + (size_t)tile * BINS + b, word, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+    uint32_t total;
+    uint32_t lrun = block_excl_sum256(tsum, s_w, &total);
+#pragma unroll
+    for (int j = 0; j < BPT; ++j) {
+        int b = tid * BPT + j;
+        s_local[b] = lrun;
+        uint32_t g = 0;
+        if (tile > 0) {
+            int t = (int)tile - 1;
+            while (true) {
+                uint32_t w = __hip_atomic_load(status + (size_t)t * BINS + b, __ATOMIC_RELAXED,
+                                               __HIP_MEMORY_SCOPE_AGENT);
+                uint32_t flag = w & ~kValMask;
+                if (flag == 0) {
+                    __builtin_amdgcn_s_sleep(1);
+                    continue;
+                }
+                g += w & kValMask;
+                if (flag == kFlagP) break;
+                --t;
+            }
+            __hip_atomic_store(status + (size_t)tile * BINS + b, kFlagP | (g + cnt[j]), __ATOMIC_RELAXED,
+                               __HIP_MEMORY_SCOPE_AGENT);
+        }
+        s_global[b] = bin_start[b] + g - lrun;
+        lrun += cnt[j];
+    }
+    __syncthreads();
+
+#pragma unroll
+    for (int it = 0; it < kSortItems; ++it) {
+        uint32_t idx = seg + it * kWave + lane;
+        if (idx < n) {
+            uint32_t d = (uint32_t)(key[it] >> shift) & MASK;
+            uint32_t pos = s_local[d] + wh[d] + rk[it];
+            s_keys[pos] = key[it];
+            if (VALS) s_vals[pos] = val[it];
+        }
+    }
+    __syncthreads();
+
+    const uint32_t tile_n = (n - base < (uint32_t)TILE) ? (n - base) : (uint32_t)TILE;
+    for (uint32_t j = tid; j < tile_n; j += kSortBlock) {
+        uint64_t k = s_keys[j];
+        uint32_t d = (uint32_t)(k >> shift) & MASK;
+        uint32_t o = s_global[d] + j;
+        if (o < n) {
+            kout[o] = k;
+            if (VALS) vout[o] = s_vals[j];
+        }
+    }
+}
+
+template <int BITS>
+static void onesweep_pass(bool vals, const uint64_t* kin, uint64_t* kout, const uint32_t* vin, uint32_t* vout,
+                          uint32_t n, int shift, uint32_t tiles, const uint32_t* bin_start, uint32_t* status,
+                          uint32_t* tile_ctr, hipStream_t s) {
+    if (vals)
+        hipLaunchKernelGGL((k_onesweep<BITS, true>), dim3(tiles), dim3(kSortBlock), 0, s, kin, kout, vin, vout, n,
+                           shift, bin_start, status, tile_ctr);
+    else
+        hipLaunchKernelGGL((k_onesweep<BITS, false>), dim3(tiles), dim3(kSortBlock), 0, s, kin, kout, vin, vout, n,
+                           shift, bin_start, status, tile_ctr);
+}
+
+void launch_onesweep_pass(const SortPlan& plan, int p, uint32_t n, const uint64_t* kin, uint64_t* kout,
+                          const uint32_t* vin, uint32_t* vout, const uint32_t* bin_start, uint32_t* lookback,
+                          hipStream_t s) {
+    const uint32_t bins = 1u << plan.bits;
+    const bool vals = vin != nullptr;
+    uint32_t* st = lookback + (size_t)p * plan.tiles * bins;
+    uint32_t* ctr = lookback + (size_t)plan.passes * plan.tiles * bins + p;
+    if (plan.bits == 8)
+        onesweep_pass<8>(vals, kin, kout, vin, vout, n, plan.shift[p], plan.tiles, bin_start + p * bins, st, ctr, s);
+    else
+        onesweep_pass<10>(vals, kin, kout, vin, vout, n, plan.shift[p], plan.tiles, bin_start + p * bins, st, ctr, s);
+}
+
+void launch_onesweep(const SortPlan& plan, uint32_t n, uint64_t* keys_a, uint64_t* keys_b, uint32_t* vals_a,
+                     uint32_t* vals_b, const uint32_t* bin_start, uint32_t* lookback, uint64_t** keys_out,
+                     uint32_t** vals_out, hipStream_t s) {
+    uint64_t* kin = keys_a;
+    uint64_t* kout = keys_b;
+    uint32_t* vin = vals_a;
+    uint32_t* vout = vals_b;
+    for (int p = 0; p < plan.passes; ++p) {
+        launch_onesweep_pass(plan, p, n, kin, kout, vin, vout, bin_start, lookback, s);
+        uint64_t* tk = kin; kin = kout; kout = tk;
+        uint32_t* tv = vin; vin = vout; vout = tv;
+    }
+    *keys_out = kin;
+    *vals_out = vin;
+}
+
+}  // namespace bvhk

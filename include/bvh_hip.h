@@ -1,0 +1,131 @@
+/*
+ * bvh_hip.h — C ABI of the MI355X (gfx950) BVH build and trace engine.
+ *
+ * Plain C: opaque handles, plain pointers and sizes, integer status codes.
+ * Nothing in here exits the process; every call returns BVH_OK (0) or an
+ * error code, and bvh_last_error() gives the message of the last failure on a
+ * context.  One context owns one HIP stream and a pooled device workspace;
+ * use one context per host thread.
+ *
+ * What each entry point replaces in the reference (paths relative to the
+ * reference repository):
+ *   bvh_mesh_upload      the nine cudaMalloc + cudaMemcpy H2D uploads at the
+ *                        start of every build (src/cuda/lbvh_builder.cu:398-438,
+ *                        src/cuda/lbvh_plus_builder.cu:424-439,
+ *                        src/cuda/ploc_builder.cu:422-435) and
+ *                        src/render_engine.cu:452-479 (uploadMesh)
+ *   bvh_build            LBVHBuilderCUDA::runCompute  (lbvh_builder.cu:440-527),
+ *                        LBVHPlusBuilderCUDA::runCompute (lbvh_plus_builder.cu:441-508),
+ *                        PLOCBuilderCUDA::build (ploc_builder.cu:403-586)
+ *   bvh_tree_download    LBVHBuilderCUDA::downloadResults (lbvh_builder.cu:529-546),
+ *                        LBVHPlusBuilderCUDA::downloadResults (lbvh_plus_builder.cu:510-527),
+ *                        the D2H copies and host root swap at ploc_builder.cu:554-582
+ *   bvh_tree_from_nodes  the node upload of renderImage (render_engine.cu:742-746)
+ *   bvh_trace            kRenderKernel/traverseBVH (render_engine.cu:289-446)
+ *                        over caller-supplied rays
+ *   bvh_sort_pairs_u64   cub::DeviceRadixSort::SortPairs (lbvh_builder.cu:485-493) and
+ *                        thrust::sort_by_key (lbvh_plus_builder.cu:465, ploc_builder.cu:459-461)
+ */
+#ifndef BVH_HIP_H
+#define BVH_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct bvh_context* bvh_ctx_t;
+typedef struct bvh_mesh_rec* bvh_mesh_t;
+typedef struct bvh_tree_rec* bvh_tree_t;
+
+/* Status codes. */
+#define BVH_OK 0
+#define BVH_E_ARG 1         /* invalid argument (null handle, n == 0, bad radius ...) */
+#define BVH_E_HIP 2         /* a HIP runtime call failed */
+#define BVH_E_NOMEM 3       /* device allocation failed */
+#define BVH_E_NONFINITE 4   /* a vertex or ray coordinate is NaN or infinite */
+#define BVH_E_NOPROGRESS 5  /* PLOC made no merge in an iteration (degenerate input) */
+#define BVH_E_BUDGET 6      /* request exceeds the device-memory budget */
+#define BVH_E_NODEVICE 7    /* no HIP device */
+#define BVH_E_STATE 8       /* call not valid in the current state */
+#define BVH_E_UNSUPPORTED 9 /* configuration not supported by this build */
+
+/* Builders (reference: LBVHBuilderCUDA, LBVHPlusBuilderCUDA, PLOCBuilderCUDA). */
+#define BVH_ALGO_LBVH 0
+#define BVH_ALGO_LBVH_PLUS 1
+#define BVH_ALGO_PLOC 2
+
+/* Build flags. */
+#define BVH_BUILD_KEEP_KEYS 1u /* keep the sorted keys for bvh_debug_sorted_keys */
+
+typedef struct bvh_build_params {
+    int32_t algo;           /* BVH_ALGO_* */
+    int32_t ploc_radius;    /* PLOC window radius, >= 1 (reference default 25) */
+    int32_t treelet_rounds; /* LBVH+ optimisation rounds, >= 0 (reference: 1) */
+    int32_t radix_bits;     /* sort digit width: 0 = tuned default, or 8 / 10 / 11; result-invariant */
+    uint32_t flags;         /* BVH_BUILD_* */
+} bvh_build_params;
+
+#define BVH_MAX_STAGES 16
+typedef struct bvh_stage_times {
+    int32_t count;                    /* number of valid entries in ms/names */
+    float total_ms;                   /* device time of the whole build */
+    float ms[BVH_MAX_STAGES];         /* per-stage device time (hipEvents on the context stream) */
+    char names[BVH_MAX_STAGES][24];   /* stage names, as printed by getTimingBreakdown */
+    int32_t ploc_iterations;          /* PLOC: merge iterations executed (0 otherwise) */
+    int32_t reserved;
+} bvh_stage_times;
+
+/* ---- device and context ------------------------------------------------ */
+int bvh_device_count(int* count);
+int bvh_ctx_create(int device, bvh_ctx_t* out);
+int bvh_ctx_destroy(bvh_ctx_t ctx);
+const char* bvh_last_error(bvh_ctx_t ctx);
+const char* bvh_status_string(int status);
+
+/* ---- raw device memory (used by tests, the renderer and the server) ---- */
+int bvh_device_alloc(bvh_ctx_t ctx, size_t bytes, void** dptr);
+int bvh_device_free(bvh_ctx_t ctx, void* dptr);
+int bvh_copy_h2d(bvh_ctx_t ctx, void* dst, const void* src, size_t bytes);
+int bvh_copy_d2h(bvh_ctx_t ctx, void* dst, const void* src, size_t bytes);
+
+/* ---- meshes ------------------------------------------------------------- */
+/* soa[0..8] = v0x v0y v0z v1x v1y v1z v2x v2y v2z, each n floats (host memory). */
+int bvh_mesh_upload(bvh_ctx_t ctx, const float* const* soa, uint32_t n, bvh_mesh_t* out);
+int bvh_mesh_free(bvh_ctx_t ctx, bvh_mesh_t mesh);
+int bvh_mesh_size(bvh_mesh_t mesh, uint32_t* n);
+
+/* ---- trees -------------------------------------------------------------- */
+int bvh_tree_create(bvh_ctx_t ctx, bvh_tree_t* out);
+int bvh_tree_free(bvh_ctx_t ctx, bvh_tree_t tree);
+/* num_nodes = 2n-1 for every builder; num_prims = n. */
+int bvh_tree_info(bvh_tree_t tree, uint32_t* num_nodes, uint32_t* num_prims);
+/* nodes: num_nodes x 48-byte BVHNode records; indices: num_prims uint32 (either may be NULL). */
+int bvh_tree_download(bvh_ctx_t ctx, bvh_tree_t tree, void* nodes, uint32_t* indices);
+/* Wraps a host node array (e.g. from another builder) into a device tree for tracing. */
+int bvh_tree_from_nodes(bvh_ctx_t ctx, const void* nodes, uint32_t num_nodes, bvh_tree_t tree);
+
+/* ---- build -------------------------------------------------------------- */
+/* Synchronous: returns once the tree is complete; times may be NULL. */
+int bvh_build(bvh_ctx_t ctx, bvh_mesh_t mesh, const bvh_build_params* params, bvh_tree_t tree,
+              bvh_stage_times* times);
+
+/* Debug views for parity tests.  Morton codes of the mesh in input order:
+ * algo LBVH/PLOC give 30-bit codes, LBVH+ 60-bit codes (uint64 each). */
+int bvh_debug_morton(bvh_ctx_t ctx, bvh_mesh_t mesh, int32_t algo, uint64_t* codes);
+/* Sorted keys of the last build made with BVH_BUILD_KEEP_KEYS.  LBVH/PLOC keys
+ * are (morton30 << 32 | index) with vals unused; LBVH+ keys are morton60 with
+ * the index in vals. */
+int bvh_debug_sorted_keys(bvh_ctx_t ctx, uint64_t* keys, uint32_t* vals, uint32_t n);
+
+/* Stable LSD radix sort of (key, val) pairs by key bits [begin_bit, end_bit),
+ * in place on device buffers; d_vals may be NULL. */
+int bvh_sort_pairs_u64(bvh_ctx_t ctx, uint64_t* d_keys, uint32_t* d_vals, uint32_t n, int begin_bit,
+                       int end_bit, int radix_bits, float* ms);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* BVH_HIP_H */

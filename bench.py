@@ -86,6 +86,27 @@ def cpu_baseline(mesh, gpu_nodes, gpu_idx):
     }, parity
 
 
+def calib_copy(bytes_=256 << 20, reps=3):
+    import ctypes as C
+    import bvh_amd as B
+    from bvh_amd import _native as N
+    ctx = B.Context(0)
+    src, dst = C.c_void_p(), C.c_void_p()
+    lib = N.hip()
+    lib.bvh_device_alloc(ctx.h, bytes_, C.byref(src))
+    lib.bvh_device_alloc(ctx.h, bytes_, C.byref(dst))
+    ms = C.c_float(0)
+    for _ in range(reps):
+        st = lib.bvh_debug_stream_copy(ctx.h, dst, src, bytes_, C.byref(ms))
+        if st != 0:
+            raise N.NativeError(st, "stream copy")
+    lib.bvh_device_free(ctx.h, src)
+    lib.bvh_device_free(ctx.h, dst)
+    ctx.close()
+    print(json.dumps({"calib_copy_bytes": bytes_, "ms": ms.value,
+                      "GBps": 2 * bytes_ / (ms.value * 1e-3) / 1e9}), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -93,7 +114,12 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--n", type=int, default=4_000_000)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--calib", action="store_true",
+                    help="only run the 256 MiB stream-copy kernel (reference point for HBM counters)")
     args = ap.parse_args()
+    if args.calib:
+        calib_copy()
+        return
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (this engine shards by mesh/ray "
               f"batches across GPUs, which is not built here).", file=sys.stderr)

@@ -157,6 +157,19 @@ class Context:
         self._check(self._lib.bvh_build(self.h, dmesh.h, C.byref(p), tree.h, C.byref(t)), "build")
         return {"total_ms": float(t.total_ms), "stages": t.as_dict(), "ploc_iterations": int(t.ploc_iterations)}
 
+    def trace(self, tree: "Tree", dmesh: "DeviceMesh", rays: np.ndarray, counters=False):
+        """Closest hit for host rays (n, 6) float32; returns (t, prim[, counters], ms)."""
+        rays = np.ascontiguousarray(rays, dtype=np.float32).reshape(-1, 6)
+        n = rays.shape[0]
+        t = np.zeros(n, np.float32)
+        prim = np.zeros(n, np.int32)
+        cnt = np.zeros((n, 3), np.int32) if counters else None
+        ms = C.c_float(0)
+        self._check(self._lib.bvh_trace_host(self.h, tree.h, dmesh.h, rays.ctypes.data, n, t.ctypes.data,
+                                             prim.ctypes.data, cnt.ctypes.data if counters else None,
+                                             C.byref(ms)), "trace")
+        return (t, prim, cnt, float(ms.value)) if counters else (t, prim, float(ms.value))
+
     def debug_morton(self, dmesh: "DeviceMesh", algo=ALGO_LBVH) -> np.ndarray:
         out = np.zeros(dmesh.n, dtype=np.uint64)
         self._check(self._lib.bvh_debug_morton(self.h, dmesh.h, algo, out.ctypes.data), "debug_morton")

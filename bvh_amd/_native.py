@@ -85,6 +85,9 @@ _HIP_SIGS = {
     "bvh_debug_morton": (_i32, [_vp, _vp, _i32, _vp]),
     "bvh_debug_sorted_keys": (_i32, [_vp, _vp, _vp, _u32]),
     "bvh_sort_pairs_u64": (_i32, [_vp, _vp, _vp, _u32, C.c_int, C.c_int, C.c_int, C.POINTER(_f)]),
+    "bvh_debug_stream_copy": (_i32, [_vp, _vp, _vp, _sz, C.POINTER(_f)]),
+    "bvh_trace": (_i32, [_vp, _vp, _vp, _vp, _u32, _vp, _vp, _vp, C.POINTER(_f)]),
+    "bvh_trace_host": (_i32, [_vp, _vp, _vp, _vp, _u32, _vp, _vp, _vp, C.POINTER(_f)]),
 }
 
 _HOST_SIGS = {

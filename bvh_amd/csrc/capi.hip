@@ -7,6 +7,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cmath>
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
@@ -33,6 +34,9 @@ struct bvh_context {
     // workspace
     DevBuf tri_boxes, keys_a, keys_b, vals_a, vals_b, hist_part, bin_start, lookback, scene, handoff, parents,
         ranges;
+    DevBuf ploc_box[2], ploc_id[2], ploc_nn, ploc_state, ploc_status;
+    DevBuf counters;  // LBVH+ arrival words
+    DevBuf trace_ovf, trace_list, trace_io;  // traversal queue and host-call staging
     // last build's sorted keys (for bvh_debug_sorted_keys)
     uint64_t* last_keys = nullptr;
     uint32_t* last_vals = nullptr;
@@ -41,6 +45,7 @@ struct bvh_context {
 
 struct bvh_mesh_rec {
     float* base = nullptr;  // 9 x n floats, one allocation
+    float4* tri = nullptr;  // 3 x float4 per triangle (traversal copy)
     uint32_t n = 0;
     size_t stride = 0;      // floats between arrays
     MeshView view() const {
@@ -123,10 +128,11 @@ struct StageClock {
         HIP_TRY(ctx, hipEventRecord(ctx->ev[0], ctx->stream));
         return BVH_OK;
     }
+    // Past the last slot, the final stage absorbs the remaining work (the
+    // total stays exact).
     int mark(const char* name) {
-        if (k >= BVH_MAX_STAGES) return BVH_OK;
-        names[k] = name;
-        ++k;
+        if (k < BVH_MAX_STAGES) ++k;
+        names[k - 1] = name;
         HIP_TRY(ctx, hipEventRecord(ctx->ev[k], ctx->stream));
         return BVH_OK;
     }
@@ -257,6 +263,142 @@ int build_lbvh(bvh_ctx_t ctx, bvh_mesh_t mesh, const bvh_build_params* prm, bvh_
     return check_input_flag(ctx);
 }
 
+int build_lbvh_plus(bvh_ctx_t ctx, bvh_mesh_t mesh, const bvh_build_params* prm, bvh_tree_t tree,
+                    bvh_stage_times* times) {
+    const MeshView m = mesh->view();
+    const uint32_t n = m.n;
+    const int rounds = prm->treelet_rounds;
+    if (rounds < 0 || rounds > 64) return set_err(ctx, BVH_E_ARG, "treelet_rounds %d out of range [0, 64]", rounds);
+    int st;
+    if ((st = ensure_tree(ctx, tree, n))) return st;
+    if ((st = ensure(ctx, ctx->handoff, (size_t)n * 8, true))) return st;
+    if ((st = ensure(ctx, ctx->parents, (size_t)(2 * (size_t)n) * 4))) return st;
+    if ((st = ensure(ctx, ctx->ranges, (size_t)n * 8))) return st;
+    if ((st = ensure(ctx, ctx->counters, (size_t)n * 4, true))) return st;
+    StageClock clk{ctx};
+    uint64_t* keys = nullptr;
+    uint32_t* vals = nullptr;
+    if ((st = build_front(ctx, m, true, prm->radix_bits, clk, &keys, &vals))) return st;
+    HierArgs ha;
+    ha.keys = keys;
+    ha.vals = vals;
+    ha.n = n;
+    ha.tri_boxes = P<float4>(ctx->tri_boxes);
+    ha.nodes = P<DNode>(tree->nodes);
+    ha.indices = P<uint32_t>(tree->indices);
+    ha.parents = P<uint32_t>(ctx->parents);
+    ha.ranges = P<int2>(ctx->ranges);
+    ha.handoff = P<uint64_t>(ctx->handoff);
+    ha.epoch = ++ctx->epoch;
+    ha.err = P<uint32_t>(ctx->scene) + 7;
+    launch_hierarchy(ha, ctx->stream);
+    if ((st = check_launch(ctx, "k_hierarchy"))) return st;
+    if ((st = clk.mark("Topology+Refit"))) return st;
+    static const char* round_names[8] = {"Treelet Round 1", "Treelet Round 2", "Treelet Round 3",
+                                         "Treelet Round 4", "Treelet Round 5", "Treelet Round 6",
+                                         "Treelet Round 7", "Treelet Rounds 8+"};
+    for (int r = 0; r < rounds && n > 1; ++r) {
+        TreeletArgs ta;
+        ta.n = n;
+        ta.nodes = P<DNode>(tree->nodes);
+        ta.parents = P<uint32_t>(ctx->parents);
+        ta.ranges = P<int2>(ctx->ranges);
+        ta.counters = P<uint32_t>(ctx->counters);
+        ta.epoch = ++ctx->epoch;
+        launch_treelet(ta, ctx->stream);
+        if ((st = check_launch(ctx, "k_treelet"))) return st;
+        if (r < 7 || r == rounds - 1)
+            if ((st = clk.mark(round_names[r < 7 ? r : 7]))) return st;
+    }
+    if ((st = clk.finish(times))) return st;
+    ctx->last_keys = keys;
+    ctx->last_vals = vals;
+    ctx->last_n = n;
+    return check_input_flag(ctx);
+}
+
+int build_ploc(bvh_ctx_t ctx, bvh_mesh_t mesh, const bvh_build_params* prm, bvh_tree_t tree, bvh_stage_times* times) {
+    const MeshView m = mesh->view();
+    const uint32_t n = m.n;
+    const int radius = prm->ploc_radius;
+    if (radius < 1 || radius > 1024) return set_err(ctx, BVH_E_ARG, "PLOC radius %d out of range [1, 1024]", radius);
+    int st;
+    if ((st = ensure_tree(ctx, tree, n))) return st;
+    StageClock clk{ctx};
+    uint64_t* keys = nullptr;
+    uint32_t* vals = nullptr;
+    if ((st = build_front(ctx, m, false, prm->radix_bits, clk, &keys, &vals))) return st;
+    const size_t nn = n;
+    if ((st = ensure(ctx, ctx->ploc_box[0], nn * 32))) return st;
+    if ((st = ensure(ctx, ctx->ploc_box[1], nn * 32))) return st;
+    if ((st = ensure(ctx, ctx->ploc_id[0], nn * 4))) return st;
+    if ((st = ensure(ctx, ctx->ploc_id[1], nn * 4))) return st;
+    if ((st = ensure(ctx, ctx->ploc_nn, nn * 4))) return st;
+    if ((st = ensure(ctx, ctx->ploc_state, (size_t)(kPlocMaxIter + 2) * sizeof(PlocState)))) return st;
+    if ((st = ensure(ctx, ctx->ploc_status, (nn / 256 + 2) * 8 + 64))) return st;
+    PlocArgs a;
+    a.n = n;
+    a.radius = radius;
+    for (int b = 0; b < 2; ++b) {
+        a.box[b] = P<float4>(ctx->ploc_box[b]);
+        a.id[b] = P<uint32_t>(ctx->ploc_id[b]);
+    }
+    a.nn = P<int32_t>(ctx->ploc_nn);
+    a.nodes = P<DNode>(tree->nodes);
+    a.state = P<PlocState>(ctx->ploc_state);
+    a.status = P<uint64_t>(ctx->ploc_status);
+    a.tile_ctr = reinterpret_cast<uint32_t*>(a.status + (nn / 256 + 2));
+    a.final_slot = kPlocMaxIter + 1;
+    launch_ploc_init(a, keys, P<float4>(ctx->tri_boxes), P<uint32_t>(tree->indices), ctx->stream);
+    if ((st = check_launch(ctx, "k_ploc_init"))) return st;
+    if ((st = clk.mark("PLOC Init"))) return st;
+    PlocState hs;
+    std::memset(&hs, 0, sizeof(hs));
+    hs.m = n;
+    int k = 0;
+    if (n > 1) {
+        // Global iterations in batches; the device skips iterations once the
+        // cluster count is small, so a batch may overshoot harmlessly.
+        uint32_t m_bound = n;
+        while (m_bound > (uint32_t)kPlocSmall) {
+            double est = std::log((double)m_bound / kPlocSmall) / std::log(1.0 / 0.72);
+            int batch = (int)std::ceil(est);
+            batch = batch < 2 ? 2 : (batch > 48 ? 48 : batch);
+            // State slots form a ring (a batch never spans more than 48 of them).
+            for (int b = 0; b < batch; ++b, k = (k + 1) % kPlocMaxIter) {
+                launch_ploc_nn(a, k, m_bound, ctx->stream);
+                launch_ploc_merge(a, k, m_bound, ctx->stream);
+            }
+            if ((st = check_launch(ctx, "k_ploc_nn/merge"))) return st;
+            HIP_TRY(ctx, hipMemcpyAsync(&hs, a.state + k, sizeof(hs), hipMemcpyDeviceToHost, ctx->stream));
+            HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+            if (hs.flags) break;
+            m_bound = hs.m;
+        }
+        if ((st = clk.mark("PLOC Global Iterations"))) return st;
+        launch_ploc_tail(a, k, ctx->stream);
+        if ((st = check_launch(ctx, "k_ploc_tail"))) return st;
+        if ((st = clk.mark("PLOC LDS Tail"))) return st;
+    }
+    if ((st = clk.finish(times))) return st;
+    if (n > 1) {
+        HIP_TRY(ctx, hipMemcpyAsync(&hs, a.state + a.final_slot, sizeof(hs), hipMemcpyDeviceToHost, ctx->stream));
+        HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+    }
+    if (times) times->ploc_iterations = (int32_t)hs.iters;
+    ctx->last_keys = keys;
+    ctx->last_vals = nullptr;
+    ctx->last_n = n;
+    if ((st = check_input_flag(ctx))) return st;
+    if (hs.flags & 2u) return set_err(ctx, BVH_E_STATE, "internal error: sorted keys name no primitive");
+    if (hs.flags & 4u) return set_err(ctx, BVH_E_STATE, "internal error: PLOC tail received too many clusters");
+    if (hs.flags & 1u)
+        return set_err(ctx, BVH_E_NOPROGRESS, "PLOC iteration %u made no merge (%u clusters left)", hs.iters, hs.m);
+    if (n > 1 && (hs.m != 1 || hs.node_base != 2 * n - 1))
+        return set_err(ctx, BVH_E_STATE, "PLOC ended with %u clusters and %u nodes", hs.m, hs.node_base);
+    return BVH_OK;
+}
+
 }  // namespace
 
 extern "C" {
@@ -315,6 +457,10 @@ int bvh_ctx_destroy(bvh_ctx_t ctx) {
     DevBuf* bufs[] = {&ctx->tri_boxes, &ctx->keys_a,    &ctx->keys_b, &ctx->vals_a,  &ctx->vals_b,  &ctx->hist_part,
                       &ctx->bin_start, &ctx->lookback,  &ctx->scene,  &ctx->handoff, &ctx->parents, &ctx->ranges};
     for (DevBuf* b : bufs) free_buf(*b);
+    DevBuf* pbufs[] = {&ctx->ploc_box[0], &ctx->ploc_box[1], &ctx->ploc_id[0], &ctx->ploc_id[1],
+                       &ctx->ploc_nn,     &ctx->ploc_state,  &ctx->ploc_status, &ctx->counters,
+                       &ctx->trace_ovf,   &ctx->trace_list,  &ctx->trace_io};
+    for (DevBuf* b : pbufs) free_buf(*b);
     for (auto& e : ctx->ev)
         if (e) (void)hipEventDestroy(e);
     (void)hipStreamDestroy(ctx->stream);
@@ -375,8 +521,16 @@ int bvh_mesh_upload(bvh_ctx_t ctx, const float* const* soa, uint32_t n, bvh_mesh
             return set_err(ctx, BVH_E_HIP, "mesh upload failed: %s", hipGetErrorString(e));
         }
     }
+    e = hipMalloc(&m->tri, (size_t)n * 48);
+    if (e != hipSuccess) {
+        (void)hipFree(m->base);
+        delete m;
+        return set_err(ctx, BVH_E_NOMEM, "mesh allocation failed: %s", hipGetErrorString(e));
+    }
+    launch_tri_aos(m->view(), m->tri, ctx->stream);
     if (hipStreamSynchronize(ctx->stream) != hipSuccess) {
         (void)hipFree(m->base);
+        (void)hipFree(m->tri);
         delete m;
         return set_err(ctx, BVH_E_HIP, "mesh upload sync failed");
     }
@@ -388,6 +542,7 @@ int bvh_mesh_free(bvh_ctx_t ctx, bvh_mesh_t mesh) {
     if (!ctx || !mesh) return BVH_E_ARG;
     HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
     if (mesh->base) HIP_TRY(ctx, hipFree(mesh->base));
+    if (mesh->tri) HIP_TRY(ctx, hipFree(mesh->tri));
     delete mesh;
     return BVH_OK;
 }
@@ -454,6 +609,8 @@ int bvh_build(bvh_ctx_t ctx, bvh_mesh_t mesh, const bvh_build_params* params, bv
     HIP_TRY(ctx, hipSetDevice(ctx->device));
     switch (params->algo) {
         case BVH_ALGO_LBVH: return build_lbvh(ctx, mesh, params, tree, times);
+        case BVH_ALGO_PLOC: return build_ploc(ctx, mesh, params, tree, times);
+        case BVH_ALGO_LBVH_PLUS: return build_lbvh_plus(ctx, mesh, params, tree, times);
         default: return set_err(ctx, BVH_E_UNSUPPORTED, "algorithm %d not available", params->algo);
     }
 }
@@ -483,6 +640,75 @@ int bvh_debug_sorted_keys(bvh_ctx_t ctx, uint64_t* keys, uint32_t* vals, uint32_
     if (vals && ctx->last_vals)
         HIP_TRY(ctx, hipMemcpyAsync(vals, ctx->last_vals, (size_t)n * 4, hipMemcpyDeviceToHost, ctx->stream));
     HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+    return BVH_OK;
+}
+
+int bvh_trace(bvh_ctx_t ctx, bvh_tree_t tree, bvh_mesh_t mesh, const float* d_rays, uint32_t nrays, float* d_t,
+              int32_t* d_prim, int32_t* d_counters, float* ms) {
+    if (!ctx || !tree || !mesh || (nrays && (!d_rays || !d_t || !d_prim))) return BVH_E_ARG;
+    if (tree->num_nodes == 0) return set_err(ctx, BVH_E_STATE, "tree is empty");
+    if (nrays == 0) return BVH_OK;
+    HIP_TRY(ctx, hipSetDevice(ctx->device));
+    int st;
+    if ((st = ensure(ctx, ctx->trace_ovf, 64))) return st;
+    if ((st = ensure(ctx, ctx->trace_list, (size_t)nrays * 4))) return st;
+    TraceArgs a;
+    a.nodes = P<DNode>(tree->nodes);
+    a.num_nodes = tree->num_nodes;
+    a.num_prims = mesh->n;
+    a.tri = mesh->tri;
+    a.rays = d_rays;
+    a.nrays = nrays;
+    a.t_out = d_t;
+    a.prim_out = d_prim;
+    a.counters = d_counters;
+    a.overflow_count = P<uint32_t>(ctx->trace_ovf);
+    a.overflow_list = P<uint32_t>(ctx->trace_list);
+    HIP_TRY(ctx, hipEventRecord(ctx->ev[0], ctx->stream));
+    launch_fill_zero(a.overflow_count, 4, ctx->stream);
+    launch_trace(a, ctx->stream);
+    launch_trace_deep(a, ctx->stream);
+    if ((st = check_launch(ctx, "k_trace"))) return st;
+    HIP_TRY(ctx, hipEventRecord(ctx->ev[1], ctx->stream));
+    uint32_t ovf[2] = {0, 0};
+    HIP_TRY(ctx, hipMemcpyAsync(ovf, a.overflow_count, 8, hipMemcpyDeviceToHost, ctx->stream));
+    HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+    if (ms) HIP_TRY(ctx, hipEventElapsedTime(ms, ctx->ev[0], ctx->ev[1]));
+    if (ovf[1]) return set_err(ctx, BVH_E_STATE, "traversal needed more than 256 stack entries (or the nodes are not a tree)");
+    return BVH_OK;
+}
+
+int bvh_trace_host(bvh_ctx_t ctx, bvh_tree_t tree, bvh_mesh_t mesh, const float* rays, uint32_t nrays, float* t,
+                   int32_t* prim, int32_t* counters, float* ms) {
+    if (!ctx || !tree || !mesh || (nrays && (!rays || !t || !prim))) return BVH_E_ARG;
+    if (nrays == 0) return BVH_OK;
+    const size_t rb = (size_t)nrays * 24, tb = (size_t)nrays * 4, cb = (size_t)nrays * 12;
+    int st;
+    if ((st = ensure(ctx, ctx->trace_io, rb + 2 * tb + cb + 256))) return st;
+    char* base = P<char>(ctx->trace_io);
+    float* d_rays = reinterpret_cast<float*>(base);
+    float* d_t = reinterpret_cast<float*>(base + ((rb + 63) & ~(size_t)63));
+    int32_t* d_prim = reinterpret_cast<int32_t*>(reinterpret_cast<char*>(d_t) + ((tb + 63) & ~(size_t)63));
+    int32_t* d_cnt = counters ? reinterpret_cast<int32_t*>(reinterpret_cast<char*>(d_prim) + ((tb + 63) & ~(size_t)63))
+                              : nullptr;
+    HIP_TRY(ctx, hipMemcpyAsync(d_rays, rays, rb, hipMemcpyHostToDevice, ctx->stream));
+    if ((st = bvh_trace(ctx, tree, mesh, d_rays, nrays, d_t, d_prim, d_cnt, ms))) return st;
+    HIP_TRY(ctx, hipMemcpyAsync(t, d_t, tb, hipMemcpyDeviceToHost, ctx->stream));
+    HIP_TRY(ctx, hipMemcpyAsync(prim, d_prim, tb, hipMemcpyDeviceToHost, ctx->stream));
+    if (counters) HIP_TRY(ctx, hipMemcpyAsync(counters, d_cnt, cb, hipMemcpyDeviceToHost, ctx->stream));
+    HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+    return BVH_OK;
+}
+
+int bvh_debug_stream_copy(bvh_ctx_t ctx, void* d_dst, const void* d_src, size_t bytes, float* ms) {
+    if (!ctx || !d_dst || !d_src || (bytes % 16) != 0) return BVH_E_ARG;
+    HIP_TRY(ctx, hipEventRecord(ctx->ev[0], ctx->stream));
+    launch_stream_copy(d_src, d_dst, bytes, ctx->stream);
+    int st;
+    if ((st = check_launch(ctx, "k_stream_copy"))) return st;
+    HIP_TRY(ctx, hipEventRecord(ctx->ev[1], ctx->stream));
+    HIP_TRY(ctx, hipEventSynchronize(ctx->ev[1]));
+    if (ms) HIP_TRY(ctx, hipEventElapsedTime(ms, ctx->ev[0], ctx->ev[1]));
     return BVH_OK;
 }
 

@@ -30,6 +30,7 @@ struct SortPlan {
     int bits;          // digit width (8, 10 or 11)
     int passes;        // number of LSD passes
     int shift[8];      // low bit of each pass
+    uint32_t mask[8];  // digit mask of each pass (the last pass may be narrower)
     uint32_t tiles;    // onesweep tiles
     uint32_t tile;     // keys per tile
     int hist_blocks;   // blocks of the histogram producer
@@ -55,6 +56,7 @@ void launch_sort_histogram(const uint64_t* keys, uint32_t n, const SortPlan& pla
 // Reduces the per-block histograms and scans them into per-pass bin starts.
 void launch_hist_scan(const uint32_t* hist_part, const SortPlan& plan, uint32_t* bin_start, hipStream_t s);
 void launch_fill_zero(uint32_t* p, size_t words, hipStream_t s);
+void launch_stream_copy(const void* src, void* dst, size_t bytes, hipStream_t s);
 // All onesweep passes.  Result ends in (*keys_out, *vals_out) which is one of
 // the two buffers passed in.
 void launch_onesweep_pass(const SortPlan& plan, int pass, uint32_t n, const uint64_t* kin, uint64_t* kout,
@@ -80,5 +82,70 @@ struct HierArgs {
 };
 int hier_chunk();
 void launch_hierarchy(const HierArgs& a, hipStream_t s);
+
+// ---- traversal (k_trace.hip) --------------------------------------------------
+struct TraceArgs {
+    const DNode* nodes;
+    uint32_t num_nodes;
+    uint32_t num_prims;
+    const float4* tri;          // 3 x float4 per triangle
+    const float* rays;          // 6 floats per ray: origin, direction
+    uint32_t nrays;
+    float* t_out;
+    int32_t* prim_out;
+    int32_t* counters;          // optional: 3 per ray
+    uint32_t* overflow_count;   // [0] queued rays, [1] error flag (cleared by the caller)
+    uint32_t* overflow_list;    // nrays entries
+};
+void launch_tri_aos(const MeshView& m, float4* tri, hipStream_t s);
+void launch_trace(const TraceArgs& a, hipStream_t s);
+void launch_trace_deep(const TraceArgs& a, hipStream_t s);
+
+// ---- LBVH+ treelets (k_treelet.hip) -------------------------------------------
+struct TreeletArgs {
+    uint32_t n;
+    DNode* nodes;        // 2n-1
+    uint32_t* parents;   // 2n-1
+    int2* ranges;        // n-1: min/max sorted leaf position under each internal node
+    uint32_t* counters;  // n-1 arrival words (2*epoch+1 tags, never cleared)
+    uint32_t epoch;
+};
+void launch_treelet(const TreeletArgs& a, hipStream_t s);
+
+// ---- PLOC (k_ploc.hip) -------------------------------------------------------
+// Clusters left when one workgroup takes over all remaining iterations in LDS.
+constexpr int kPlocSmall = 2048;
+constexpr int kPlocMaxIter = 4096;
+
+// Device-side loop state of one PLOC iteration (32 B).
+struct PlocState {
+    uint32_t m;          // live clusters entering the iteration
+    uint32_t node_base;  // id of the next internal node (before the root swap)
+    uint32_t parity;     // which cluster buffer holds the live clusters
+    uint32_t iters;      // merge iterations executed so far
+    uint32_t flags;      // 1 = no progress, 2 = invalid key, 4 = tail overflow
+    uint32_t pad[3];
+};
+
+struct PlocArgs {
+    uint32_t n;
+    int radius;
+    float4* box[2];     // cluster boxes (2 x float4 each), double-buffered
+    uint32_t* id[2];    // cluster node ids, double-buffered
+    int32_t* nn;        // nearest neighbour per cluster
+    DNode* nodes;       // 2n-1
+    PlocState* state;   // kPlocMaxIter + 2 slots
+    uint64_t* status;   // look-back words, one per merge tile
+    uint32_t* tile_ctr; // merge tile counter
+    int final_slot;     // state slot the tail kernel writes
+};
+int ploc_small();
+int ploc_nn_block();
+int ploc_merge_tile();
+void launch_ploc_init(const PlocArgs& a, const uint64_t* keys, const float4* tri_boxes, uint32_t* indices,
+                      hipStream_t s);
+void launch_ploc_nn(const PlocArgs& a, int k, uint32_t m_bound, hipStream_t s);
+void launch_ploc_merge(const PlocArgs& a, int k, uint32_t m_bound, hipStream_t s);
+void launch_ploc_tail(const PlocArgs& a, int k, hipStream_t s);
 
 }  // namespace bvhk

@@ -111,7 +111,12 @@ SortPlan make_sort_plan(uint32_t n, int begin_bit, int end_bit, int bits) {
     p.bits = bits;
     int span = end_bit - begin_bit;
     p.passes = (span + bits - 1) / bits;
-    for (int i = 0; i < 8; ++i) p.shift[i] = begin_bit + i * bits;
+    for (int i = 0; i < 8; ++i) {
+        p.shift[i] = begin_bit + i * bits;
+        int w = end_bit - p.shift[i];
+        w = w < 0 ? 0 : (w > bits ? bits : w);
+        p.mask[i] = (1u << w) - 1u;
+    }
     p.tile = 256 * 16;
     p.tiles = (n + p.tile - 1) / p.tile;
     if (p.tiles < 1) p.tiles = 1;
@@ -127,6 +132,7 @@ size_t sort_zero_bytes(const SortPlan& p) {
 
 struct Shifts {
     int s[8];
+    uint32_t m[8];
 };
 
 // Morton keys + per-block digit histograms for all passes + zeroing.
@@ -163,7 +169,7 @@ __global__ __launch_bounds__(kFrontBlock) void k_morton(const float4* __restrict
         }
         keys[i] = key;
         for (int p = 0; p < passes; ++p) {
-            uint32_t d = (uint32_t)(key >> sh.s[p]) & (BINS - 1);
+            uint32_t d = (uint32_t)(key >> sh.s[p]) & sh.m[p];
             atomicAdd(&s_hist[p * BINS + d], 1u);
         }
     }
@@ -177,7 +183,10 @@ static void morton_launch(const float4* tri_boxes, uint32_t n, const uint32_t* s
                           uint32_t* vals, const SortPlan& plan, uint32_t* hist_part, uint32_t* zero_ptr,
                           size_t zero_words, hipStream_t s) {
     Shifts sh;
-    for (int i = 0; i < 8; ++i) sh.s[i] = plan.shift[i];
+    for (int i = 0; i < 8; ++i) {
+        sh.s[i] = plan.shift[i];
+        sh.m[i] = plan.mask[i];
+    }
     size_t lds = (size_t)plan.passes * (1u << BITS) * 4;
     hipLaunchKernelGGL((k_morton<BITS, M60>), dim3(plan.hist_blocks), dim3(kFrontBlock), lds, s, tri_boxes, n,
                        scene_ord, keys, vals, plan.passes, sh, hist_part, reinterpret_cast<uint4*>(zero_ptr),

@@ -33,6 +33,18 @@ __global__ void k_fill_zero(uint4* p, size_t n4) {
     for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n4; i += stride) p[i] = make_uint4(0, 0, 0, 0);
 }
 
+// Plain streaming copy (16-B vectors); the reference point for HBM counters.
+__global__ void k_stream_copy(const uint4* __restrict__ src, uint4* __restrict__ dst, size_t n4) {
+    size_t stride = (size_t)gridDim.x * blockDim.x;
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n4; i += stride) dst[i] = src[i];
+}
+
+void launch_stream_copy(const void* src, void* dst, size_t bytes, hipStream_t s) {
+    size_t n4 = bytes / 16;
+    hipLaunchKernelGGL(k_stream_copy, dim3(2048), dim3(256), 0, s, reinterpret_cast<const uint4*>(src),
+                       reinterpret_cast<uint4*>(dst), n4);
+}
+
 void launch_fill_zero(uint32_t* p, size_t words, hipStream_t s) {
     size_t n4 = (words + 3) / 4;
     size_t g = (n4 + 255) / 256;
@@ -43,6 +55,7 @@ void launch_fill_zero(uint32_t* p, size_t words, hipStream_t s) {
 
 struct ShiftArr {
     int s[8];
+    uint32_t m[8];
 };
 
 template <int BITS>
@@ -55,7 +68,7 @@ __global__ __launch_bounds__(256) void k_sort_histogram(const uint64_t* __restri
     const uint32_t stride = gridDim.x * 256;
     for (uint32_t i = blockIdx.x * 256 + threadIdx.x; i < n; i += stride) {
         uint64_t k = keys[i];
-        for (int p = 0; p < passes; ++p) atomicAdd(&s_hist[p * BINS + ((uint32_t)(k >> sh.s[p]) & (BINS - 1))], 1u);
+        for (int p = 0; p < passes; ++p) atomicAdd(&s_hist[p * BINS + ((uint32_t)(k >> sh.s[p]) & sh.m[p])], 1u);
     }
     __syncthreads();
     uint32_t* out = hist_part + (size_t)blockIdx.x * passes * BINS;
@@ -65,7 +78,10 @@ __global__ __launch_bounds__(256) void k_sort_histogram(const uint64_t* __restri
 void launch_sort_histogram(const uint64_t* keys, uint32_t n, const SortPlan& plan, uint32_t* hist_part,
                            hipStream_t s) {
     ShiftArr sh;
-    for (int i = 0; i < 8; ++i) sh.s[i] = plan.shift[i];
+    for (int i = 0; i < 8; ++i) {
+        sh.s[i] = plan.shift[i];
+        sh.m[i] = plan.mask[i];
+    }
     size_t lds = (size_t)plan.passes * (1u << plan.bits) * 4;
     if (plan.bits == 8)
         hipLaunchKernelGGL(k_sort_histogram<8>, dim3(plan.hist_blocks), dim3(256), lds, s, keys, n, plan.passes, sh,
@@ -132,10 +148,10 @@ void launch_hist_scan(const uint32_t* hist_part, const SortPlan& plan, uint32_t*
 template <int BITS, bool VALS>
 __global__ __launch_bounds__(kSortBlock) void k_onesweep(const uint64_t* __restrict__ kin, uint64_t* __restrict__ kout,
                                                         const uint32_t* __restrict__ vin, uint32_t* __restrict__ vout,
-                                                        uint32_t n, int shift, const uint32_t* __restrict__ bin_start,
-                                                        uint32_t* status, uint32_t* tile_ctr) {
+                                                        uint32_t n, int shift, uint32_t MASK,
+                                                        const uint32_t* __restrict__ bin_start, uint32_t* status,
+                                                        uint32_t* tile_ctr) {
     constexpr int BINS = 1 << BITS;
-    constexpr uint32_t MASK = BINS - 1;
     constexpr int WAVES = kSortBlock / kWave;
     constexpr int TILE = kSortBlock * kSortItems;
     constexpr int BPT = BINS / kSortBlock;
@@ -266,14 +282,14 @@ __global__ __launch_bounds__(kSortBlock) void k_onesweep(const uint64_t* __restr
 
 template <int BITS>
 static void onesweep_pass(bool vals, const uint64_t* kin, uint64_t* kout, const uint32_t* vin, uint32_t* vout,
-                          uint32_t n, int shift, uint32_t tiles, const uint32_t* bin_start, uint32_t* status,
+                          uint32_t n, int shift, uint32_t mask, uint32_t tiles, const uint32_t* bin_start, uint32_t* status,
                           uint32_t* tile_ctr, hipStream_t s) {
     if (vals)
         hipLaunchKernelGGL((k_onesweep<BITS, true>), dim3(tiles), dim3(kSortBlock), 0, s, kin, kout, vin, vout, n,
-                           shift, bin_start, status, tile_ctr);
+                           shift, mask, bin_start, status, tile_ctr);
     else
         hipLaunchKernelGGL((k_onesweep<BITS, false>), dim3(tiles), dim3(kSortBlock), 0, s, kin, kout, vin, vout, n,
-                           shift, bin_start, status, tile_ctr);
+                           shift, mask, bin_start, status, tile_ctr);
 }
 
 void launch_onesweep_pass(const SortPlan& plan, int p, uint32_t n, const uint64_t* kin, uint64_t* kout,
@@ -284,9 +300,9 @@ void launch_onesweep_pass(const SortPlan& plan, int p, uint32_t n, const uint64_
     uint32_t* st = lookback + (size_t)p * plan.tiles * bins;
     uint32_t* ctr = lookback + (size_t)plan.passes * plan.tiles * bins + p;
     if (plan.bits == 8)
-        onesweep_pass<8>(vals, kin, kout, vin, vout, n, plan.shift[p], plan.tiles, bin_start + p * bins, st, ctr, s);
+        onesweep_pass<8>(vals, kin, kout, vin, vout, n, plan.shift[p], plan.mask[p], plan.tiles, bin_start + p * bins, st, ctr, s);
     else
-        onesweep_pass<10>(vals, kin, kout, vin, vout, n, plan.shift[p], plan.tiles, bin_start + p * bins, st, ctr, s);
+        onesweep_pass<10>(vals, kin, kout, vin, vout, n, plan.shift[p], plan.mask[p], plan.tiles, bin_start + p * bins, st, ctr, s);
 }
 
 void launch_onesweep(const SortPlan& plan, uint32_t n, uint64_t* keys_a, uint64_t* keys_b, uint32_t* vals_a,

@@ -112,6 +112,18 @@ int bvh_tree_from_nodes(bvh_ctx_t ctx, const void* nodes, uint32_t num_nodes, bv
 int bvh_build(bvh_ctx_t ctx, bvh_mesh_t mesh, const bvh_build_params* params, bvh_tree_t tree,
               bvh_stage_times* times);
 
+/* ---- traversal ----------------------------------------------------------- */
+/* Closest hit of nrays rays (6 floats each: origin xyz, direction xyz) against
+ * `tree` over `mesh`, all buffers in device memory.  t_out[i] = hit distance
+ * (1e30 when missed), prim_out[i] = original triangle index or -1;
+ * counters (optional, 3 int32 per ray) = nodes visited, AABB tests, triangle
+ * tests, as counted by the reference renderer.  ms (optional) = device time. */
+int bvh_trace(bvh_ctx_t ctx, bvh_tree_t tree, bvh_mesh_t mesh, const float* d_rays, uint32_t nrays, float* d_t,
+              int32_t* d_prim, int32_t* d_counters, float* ms);
+/* Same with host buffers (staged through the context's workspace). */
+int bvh_trace_host(bvh_ctx_t ctx, bvh_tree_t tree, bvh_mesh_t mesh, const float* rays, uint32_t nrays, float* t,
+                   int32_t* prim, int32_t* counters, float* ms);
+
 /* Debug views for parity tests.  Morton codes of the mesh in input order:
  * algo LBVH/PLOC give 30-bit codes, LBVH+ 60-bit codes (uint64 each). */
 int bvh_debug_morton(bvh_ctx_t ctx, bvh_mesh_t mesh, int32_t algo, uint64_t* codes);
@@ -124,6 +136,11 @@ int bvh_debug_sorted_keys(bvh_ctx_t ctx, uint64_t* keys, uint32_t* vals, uint32_
  * in place on device buffers; d_vals may be NULL. */
 int bvh_sort_pairs_u64(bvh_ctx_t ctx, uint64_t* d_keys, uint32_t* d_vals, uint32_t n, int begin_bit,
                        int end_bit, int radix_bits, float* ms);
+
+/* Plain 16-byte-vector copy kernel of `bytes` (multiple of 16) between device
+ * buffers: the calibration point for HBM traffic counters.  No reference
+ * counterpart (measurement support). */
+int bvh_debug_stream_copy(bvh_ctx_t ctx, void* d_dst, const void* d_src, size_t bytes, float* ms);
 
 #ifdef __cplusplus
 }

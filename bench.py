@@ -42,28 +42,53 @@ def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
-def gpu_lbvh(mesh, steps, warmup):
-    import bvh_amd as B
-    ctx = B.Context(0)
-    dm = ctx.upload(mesh)
-    tree = ctx.tree()
+def timed_builds(ctx, dm, tree, steps, warmup, **kw):
+    """Back-to-back synchronous builds of a resident mesh: wall time, mean device time, mean per-stage times."""
     for _ in range(warmup):
-        ctx.build(dm, tree, algo=B.ALGO_LBVH)
+        ctx.build(dm, tree, **kw)
     stage_sum = {}
     dev_total = 0.0
+    iters = 0
     t0 = time.perf_counter()
     for _ in range(steps):
-        t = ctx.build(dm, tree, algo=B.ALGO_LBVH)
+        t = ctx.build(dm, tree, **kw)
         dev_total += t["total_ms"]
+        iters = t["ploc_iterations"]
         for k, v in t["stages"].items():
             stage_sum[k] = stage_sum.get(k, 0.0) + v
     wall = time.perf_counter() - t0
-    nodes, idx = tree.download()
-    tree.free()
-    dm.free()
-    ctx.close()
-    stages = {k: v / steps for k, v in stage_sum.items()}
-    return wall, dev_total / steps, stages, nodes, idx
+    return wall, dev_total / steps, {k: v / steps for k, v in stage_sum.items()}, iters
+
+
+def extras(ctx, dm, tree, mesh, steps):
+    """Config-1 companions of the headline: LBVH+ (3 rounds), PLOC (r=25), and
+    1024x768 camera rays traced through the LBVH and PLOC trees."""
+    import bvh_amd as B
+    out = {}
+    for name, kw in (("lbvh_plus_3_rounds", {"algo": B.ALGO_LBVH_PLUS, "rounds": 3}),
+                     ("ploc_r25", {"algo": B.ALGO_PLOC, "radius": 25})):
+        log(f"[bench] {name}")
+        wall, dev, st, iters = timed_builds(ctx, dm, tree, max(3, steps // 4), 1, **kw)
+        n_steps = max(3, steps // 4)
+        out[name] = {"ms_per_build": round(wall / n_steps * 1e3, 3), "device_ms": round(dev, 3),
+                     "Mtri_s": round(mesh.n / (wall / n_steps) / 1e6, 1),
+                     "stages_ms": {k: round(v, 4) for k, v in st.items()}}
+        if iters:
+            out[name]["iterations"] = iters
+    rays = B.rays_sphere_camera(mesh.bounds(), 1024, 768)
+    for name, kw in (("trace_lbvh", {"algo": B.ALGO_LBVH}), ("trace_ploc_r25", {"algo": B.ALGO_PLOC, "radius": 25})):
+        log(f"[bench] {name}")
+        ctx.build(dm, tree, **kw)
+        t, prim, cnt, _ = ctx.trace(tree, dm, rays, counters=True)
+        ms = []
+        for _ in range(5):
+            ms.append(ctx.trace(tree, dm, rays)[2])
+        med = float(np.median(ms))
+        out[name] = {"rays": int(rays.shape[0]), "device_ms": round(med, 4),
+                     "Mrays_s": round(rays.shape[0] / med / 1e3, 1), "hit_fraction": round(float((prim >= 0).mean()), 4),
+                     "avg_nodes_visited": round(float(cnt[:, 0].mean()), 2),
+                     "avg_tri_tests": round(float(cnt[:, 2].mean()), 2)}
+    return out
 
 
 def cpu_baseline(mesh, gpu_nodes, gpu_idx):
@@ -114,6 +139,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--n", type=int, default=4_000_000)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--lbvh-only", action="store_true", help="skip the LBVH+/PLOC/trace companions")
     ap.add_argument("--calib", action="store_true",
                     help="only run the 256 MiB stream-copy kernel (reference point for HBM counters)")
     args = ap.parse_args()
@@ -129,7 +155,15 @@ def main():
     log(f"[bench] generating {args.n:,} blob triangles (seed 7)")
     mesh = B.Mesh.blobs(7, args.n)
     log(f"[bench] LBVH: {args.warmup} warmup + {args.steps} timed builds")
-    wall, dev_ms, stages, nodes, idx = gpu_lbvh(mesh, args.steps, args.warmup)
+    ctx = B.Context(0)
+    dm = ctx.upload(mesh)
+    tree = ctx.tree()
+    wall, dev_ms, stages, _ = timed_builds(ctx, dm, tree, args.steps, args.warmup, algo=B.ALGO_LBVH)
+    nodes, idx = tree.download()
+    extra = extras(ctx, dm, tree, mesh, args.steps) if not args.lbvh_only else {}
+    tree.free()
+    dm.free()
+    ctx.close()
     ms_per_step = wall / args.steps * 1e3
     value = mesh.n / (ms_per_step / 1e3) / 1e6
 
@@ -142,7 +176,8 @@ def main():
         per_stage[name] = {"ms": round(ms, 4), "bytes_per_unit": ub,
                            "GBps": round(gbs, 1) if gbs else None,
                            "frac_hbm": round(gbs / HBM_PEAK_GBS, 4) if gbs else None}
-        kname = "k_onesweep" if name.startswith("Sort Pass") else name
+        kname = {"Bounds/Centroids": "k_tri_bounds", "Morton Codes": "k_morton+k_hist_reduce+k_hist_scan",
+                 "Topology+Refit": "k_hierarchy"}.get(name, "k_onesweep" if name.startswith("Sort Pass") else name)
         k = kernels.setdefault(kname, {"ms": 0.0, "launches": 0, "unit": ub})
         k["ms"] += ms
         k["launches"] += 1
@@ -172,6 +207,7 @@ def main():
         "roofline": roofline,
         "cpu_baseline": cpu,
         "parity_vs_oracle": parity,
+        "config1_companions": extra,
     }
     print(json.dumps(out), flush=True)
 

@@ -54,7 +54,8 @@ void launch_morton_codes(const float4* tri_boxes, uint32_t n, const uint32_t* sc
 void launch_sort_histogram(const uint64_t* keys, uint32_t n, const SortPlan& plan, uint32_t* hist_part,
                            hipStream_t s);
 // Reduces the per-block histograms and scans them into per-pass bin starts.
-void launch_hist_scan(const uint32_t* hist_part, const SortPlan& plan, uint32_t* bin_start, hipStream_t s);
+// (Row 0 of hist_part is overwritten with the column totals.)
+void launch_hist_scan(uint32_t* hist_part, const SortPlan& plan, uint32_t* bin_start, hipStream_t s);
 void launch_fill_zero(uint32_t* p, size_t words, hipStream_t s);
 void launch_stream_copy(const void* src, void* dst, size_t bytes, hipStream_t s);
 // All onesweep passes.  Result ends in (*keys_out, *vals_out) which is one of

@@ -120,7 +120,9 @@ SortPlan make_sort_plan(uint32_t n, int begin_bit, int end_bit, int bits) {
     p.tile = 256 * 16;
     p.tiles = (n + p.tile - 1) / p.tile;
     if (p.tiles < 1) p.tiles = 1;
-    p.hist_blocks = front_grid(n);
+    // Histogram producers: about 4K keys per block, at most 1024 blocks (4 per CU).
+    uint32_t hb = (n + 4095) / 4096;
+    p.hist_blocks = (int)(hb < 1 ? 1 : (hb > 1024 ? 1024 : hb));
     return p;
 }
 

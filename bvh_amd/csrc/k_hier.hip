@@ -90,6 +90,37 @@ __device__ void block_min_scan(const int8_t* src, int8_t* dst, int cnt, int* s_w
     __syncthreads();
 }
 
+// Node record written with agent-scope atomic stores (coherent across XCDs
+// without flushing the L2), and the matching loads.
+__device__ __forceinline__ void store_node_coherent(DNode* nodes, uint32_t i, const Box& b, int32_t l, int32_t r) {
+    float* f = reinterpret_cast<float*>(nodes + i);
+    int32_t* q = reinterpret_cast<int32_t*>(nodes + i);
+    __hip_atomic_store(f + 0, b.mnx, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __hip_atomic_store(f + 1, b.mny, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __hip_atomic_store(f + 2, b.mnz, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __hip_atomic_store(f + 3, 0.0f, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __hip_atomic_store(f + 4, b.mxx, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __hip_atomic_store(f + 5, b.mxy, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __hip_atomic_store(f + 6, b.mxz, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __hip_atomic_store(f + 7, 0.0f, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __hip_atomic_store(q + 8, l, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __hip_atomic_store(q + 9, r, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    q[10] = 0;
+    q[11] = 0;
+}
+
+__device__ __forceinline__ Box load_box_coherent(const DNode* nodes, uint32_t i) {
+    const float* f = reinterpret_cast<const float*>(nodes + i);
+    Box b;
+    b.mnx = __hip_atomic_load(f + 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    b.mny = __hip_atomic_load(f + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    b.mnz = __hip_atomic_load(f + 2, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    b.mxx = __hip_atomic_load(f + 4, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    b.mxy = __hip_atomic_load(f + 5, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    b.mxz = __hip_atomic_load(f + 6, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    return b;
+}
+
 template <bool K60>
 __global__ __launch_bounds__(kHierBlock) void k_hierarchy(HierArgs p) {
     __shared__ int8_t s_d[kChunk + 1];    // delta of boundaries a-1 .. b-1
@@ -129,51 +160,65 @@ __global__ __launch_bounds__(kHierBlock) void k_hierarchy(HierArgs p) {
             atomicOr(p.err, 1u);
             prim = 0;
         }
-        Box box = load_box(p.tri_boxes + 2 * (size_t)prim);
-        uint32_t my = n - 1 + i;
-        store_node(nodes, my, box, (int32_t)(prim | 0x80000000u), -1);
         p.indices[i] = prim;
-
+        // The current node: its range, box, children and index.  Its record is
+        // written once its handoff kind is known.
         uint32_t l = i, r = i;
+        uint32_t my = n - 1 + i;
+        Box box = load_box(p.tri_boxes + 2 * (size_t)prim);
+        int32_t lc = (int32_t)(prim | 0x80000000u), rc = -1;
         int dl = dget((int64_t)l - 1);
         int dr = dget((int64_t)r);
         bool local = true;
-        while (!(l == 0 && r == n - 1)) {
+        while (true) {
+            const bool root = (l == 0 && r == n - 1);
             const bool isLeft = dr > dl;
             const uint32_t s = isLeft ? r : l - 1;
             const uint32_t far = isLeft ? l : r;
-            uint32_t other;
             bool lds = false;
-            if (local && s >= a && s + 2 <= b) {
+            if (!root && local && s >= a && s + 2 <= b) {
                 const int qs = (int)(s - a) + 1;
                 const int ds = s_d[qs];
                 lds = (s_pre[qs - 1] < ds) && (s_suf[qs + 1] < ds);
             }
+            if (root || lds) {
+                store_node(nodes, my, box, lc, rc);
+            } else {
+                // Its partner may run on another XCD: agent-coherent stores,
+                // complete before the handoff word is swapped.
+                store_node_coherent(nodes, my, box, lc, rc);
+                __builtin_amdgcn_s_waitcnt(0);
+            }
+            if (root) break;
+            uint32_t other;
+            Box sbox;
+            uint32_t sib;
             if (lds) {
                 uint32_t old = __hip_atomic_exchange(&s_slot[s - a], far + 1, __ATOMIC_ACQ_REL,
                                                      __HIP_MEMORY_SCOPE_WORKGROUP);
                 if (old == 0) break;
                 other = old - 1;
+                if (isLeft) sib = (s + 1 == other) ? (n - 1 + other) : (s + 1);
+                else sib = (other == s) ? (n - 1 + s) : s;
+                sbox = node_box(nodes, sib);
             } else {
                 local = false;
                 uint64_t w = ((uint64_t)p.epoch << 32) | far;
-                uint64_t old = __hip_atomic_exchange(&p.handoff[s], w, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+                uint64_t old = __hip_atomic_exchange(&p.handoff[s], w, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
                 if ((uint32_t)(old >> 32) != p.epoch) break;
                 other = (uint32_t)old;
+                if (isLeft) sib = (s + 1 == other) ? (n - 1 + other) : (s + 1);
+                else sib = (other == s) ? (n - 1 + s) : s;
+                sbox = load_box_coherent(nodes, sib);
             }
-            uint32_t sib;
-            if (isLeft) sib = (s + 1 == other) ? (n - 1 + other) : (s + 1);
-            else sib = (other == s) ? (n - 1 + s) : s;
-            const Box sbox = node_box(nodes, sib);
             const Box pb = isLeft ? merge(box, sbox) : merge(sbox, box);
-            const uint32_t lc = isLeft ? my : sib;
-            const uint32_t rc = isLeft ? sib : my;
+            lc = (int32_t)(isLeft ? my : sib);
+            rc = (int32_t)(isLeft ? sib : my);
             if (isLeft) r = other;
             else l = other;
             dl = dget((int64_t)l - 1);
             dr = dget((int64_t)r);
             const uint32_t pidx = (l == 0 && r == n - 1) ? 0u : ((dr > dl) ? r : l);
-            store_node(nodes, pidx, pb, (int32_t)lc, (int32_t)rc);
             if (p.parents) {
                 p.parents[lc] = pidx;
                 p.parents[rc] = pidx;

@@ -109,10 +109,33 @@ __device__ __forceinline__ uint32_t block_excl_sum256(uint32_t v, uint32_t* s_w,
     return before + incl - v;
 }
 
-// grid = passes; reduces per-block histograms and scans bins.
+// Column sums of the per-block histograms: each block owns 64 (pass, bin)
+// columns and its 4 waves split the producer blocks; partial sums meet in LDS.
+// The totals overwrite row 0 of hist_part.
+__global__ __launch_bounds__(256) void k_hist_reduce(uint32_t* __restrict__ hist_part, int blocks, int cols) {
+    __shared__ uint32_t s_part[4][64];
+    const int wave = threadIdx.x / kWave, lane = threadIdx.x % kWave;
+    const int col = blockIdx.x * 64 + lane;
+    uint32_t acc = 0;
+    if (col < cols) {
+        int blk = wave;
+        for (; blk + 12 < blocks; blk += 16) {
+            uint32_t a0 = hist_part[(size_t)blk * cols + col];
+            uint32_t a1 = hist_part[(size_t)(blk + 4) * cols + col];
+            uint32_t a2 = hist_part[(size_t)(blk + 8) * cols + col];
+            uint32_t a3 = hist_part[(size_t)(blk + 12) * cols + col];
+            acc += (a0 + a1) + (a2 + a3);
+        }
+        for (; blk < blocks; blk += 4) acc += hist_part[(size_t)blk * cols + col];
+    }
+    s_part[wave][lane] = acc;
+    __syncthreads();
+    if (wave == 0 && col < cols) hist_part[col] = s_part[0][lane] + s_part[1][lane] + s_part[2][lane] + s_part[3][lane];
+}
+
+// grid = passes; exclusive scan of the per-pass totals (row 0 of hist_part).
 template <int BITS>
-__global__ __launch_bounds__(256) void k_hist_scan(const uint32_t* __restrict__ hist_part, int blocks, int passes,
-                                                    uint32_t* __restrict__ bin_start) {
+__global__ __launch_bounds__(256) void k_hist_scan(const uint32_t* __restrict__ totals, uint32_t* __restrict__ bin_start) {
     constexpr int BINS = 1 << BITS;
     constexpr int BPT = BINS / 256;
     __shared__ uint32_t s_w[4];
@@ -121,11 +144,8 @@ __global__ __launch_bounds__(256) void k_hist_scan(const uint32_t* __restrict__ 
     uint32_t sum = 0;
 #pragma unroll
     for (int j = 0; j < BPT; ++j) {
-        int b = threadIdx.x * BPT + j;
-        uint32_t acc = 0;
-        for (int blk = 0; blk < blocks; ++blk) acc += hist_part[((size_t)blk * passes + p) * BINS + b];
-        c[j] = acc;
-        sum += acc;
+        c[j] = totals[p * BINS + threadIdx.x * BPT + j];
+        sum += c[j];
     }
     uint32_t tot;
     uint32_t run = block_excl_sum256(sum, s_w, &tot);
@@ -136,13 +156,13 @@ __global__ __launch_bounds__(256) void k_hist_scan(const uint32_t* __restrict__ 
     }
 }
 
-void launch_hist_scan(const uint32_t* hist_part, const SortPlan& plan, uint32_t* bin_start, hipStream_t s) {
+void launch_hist_scan(uint32_t* hist_part, const SortPlan& plan, uint32_t* bin_start, hipStream_t s) {
+    const int cols = plan.passes * (1 << plan.bits);
+    hipLaunchKernelGGL(k_hist_reduce, dim3((cols + 63) / 64), dim3(256), 0, s, hist_part, plan.hist_blocks, cols);
     if (plan.bits == 8)
-        hipLaunchKernelGGL(k_hist_scan<8>, dim3(plan.passes), dim3(256), 0, s, hist_part, plan.hist_blocks,
-                           plan.passes, bin_start);
+        hipLaunchKernelGGL(k_hist_scan<8>, dim3(plan.passes), dim3(256), 0, s, hist_part, bin_start);
     else
-        hipLaunchKernelGGL(k_hist_scan<10>, dim3(plan.passes), dim3(256), 0, s, hist_part, plan.hist_blocks,
-                           plan.passes, bin_start);
+        hipLaunchKernelGGL(k_hist_scan<10>, dim3(plan.passes), dim3(256), 0, s, hist_part, bin_start);
 }
 
 template <int BITS, bool VALS>
@@ -229,29 +249,47 @@ __global__ __launch_bounds__(kSortBlock) void k_onesweep(const uint64_t* __restr
     }
     uint32_t total;
     uint32_t lrun = block_excl_sum256(tsum, s_w, &total);
+    // Decoupled look-back: the thread's BPT bins walk back together, so their
+    // status loads overlap.
+    uint32_t g[BPT];
+    int tj[BPT];
+    uint32_t pending = 0;
+#pragma unroll
+    for (int j = 0; j < BPT; ++j) {
+        g[j] = 0;
+        tj[j] = (int)tile - 1;
+        if (tile > 0) pending |= 1u << j;
+    }
+    while (pending) {
+        uint32_t w[BPT];
+#pragma unroll
+        for (int j = 0; j < BPT; ++j)
+            w[j] = ((pending >> j) & 1u) ? __hip_atomic_load(status + (size_t)tj[j] * BINS + tid * BPT + j,
+                                                              __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)
+                                          : 0u;
+        bool stalled = false;
+#pragma unroll
+        for (int j = 0; j < BPT; ++j) {
+            if (!((pending >> j) & 1u)) continue;
+            const uint32_t flag = w[j] & ~kValMask;
+            if (flag == 0) {
+                stalled = true;
+                continue;
+            }
+            g[j] += w[j] & kValMask;
+            if (flag == kFlagP) pending &= ~(1u << j);
+            else --tj[j];
+        }
+        if (stalled) __builtin_amdgcn_s_sleep(1);
+    }
 #pragma unroll
     for (int j = 0; j < BPT; ++j) {
         int b = tid * BPT + j;
-        s_local[b] = lrun;
-        uint32_t g = 0;
-        if (tile > 0) {
-            int t = (int)tile - 1;
-            while (true) {
-                uint32_t w = __hip_atomic_load(status + (size_t)t * BINS + b, __ATOMIC_RELAXED,
-                                               __HIP_MEMORY_SCOPE_AGENT);
-                uint32_t flag = w & ~kValMask;
-                if (flag == 0) {
-                    __builtin_amdgcn_s_sleep(1);
-                    continue;
-                }
-                g += w & kValMask;
-                if (flag == kFlagP) break;
-                --t;
-            }
-            __hip_atomic_store(status + (size_t)tile * BINS + b, kFlagP | (g + cnt[j]), __ATOMIC_RELAXED,
+        if (tile > 0)
+            __hip_atomic_store(status + (size_t)tile * BINS + b, kFlagP | (g[j] + cnt[j]), __ATOMIC_RELAXED,
                                __HIP_MEMORY_SCOPE_AGENT);
-        }
-        s_global[b] = bin_start[b] + g - lrun;
+        s_local[b] = lrun;
+        s_global[b] = bin_start[b] + g[j] - lrun;
         lrun += cnt[j];
     }
     __syncthreads();

@@ -16,7 +16,7 @@ from ._native import NODE_DTYPE, NativeError, ALGO_LBVH, ALGO_LBVH_PLUS, ALGO_PL
 
 __all__ = [
     "Mesh", "Context", "DeviceMesh", "Tree", "LBVHBuilder", "LBVHPlusBuilder", "PLOCBuilder", "NODE_DTYPE",
-    "NativeError", "rays_sphere_camera", "rays_sphere_to_box", "rays_uniform_cube", "rays_camera",
+    "NativeError", "rays_sphere_camera", "rays_sphere_to_box", "rays_uniform_cube", "rays_camera", "Batch",
 ]
 
 
@@ -157,6 +157,28 @@ class Context:
         self._check(self._lib.bvh_build(self.h, dmesh.h, C.byref(p), tree.h, C.byref(t)), "build")
         return {"total_ms": float(t.total_ms), "stages": t.as_dict(), "ploc_iterations": int(t.ploc_iterations)}
 
+    def batch(self, dmeshes, trees, algo=ALGO_LBVH, radius=25, radix_bits=0, tune=False) -> "Batch":
+        """Captures one launch sequence that builds every mesh into its tree (bvh_batch_create)."""
+        k = len(dmeshes)
+        if len(trees) != k:
+            raise ValueError("one tree per mesh")
+        hm = (C.c_void_p * k)(*[m.h.value if hasattr(m.h, "value") else m.h for m in dmeshes])
+        ht = (C.c_void_p * k)(*[t.h.value if hasattr(t.h, "value") else t.h for t in trees])
+        p = N.BuildParams(algo, radius, 0, radix_bits, N.BUILD_TUNE if tune else 0)
+        h = C.c_void_p()
+        self._check(self._lib.bvh_batch_create(self.h, hm, ht, k, C.byref(p), C.byref(h)), "batch create")
+        return Batch(self, h, list(trees))
+
+    def build_batched(self, dmeshes, trees, algo=ALGO_LBVH, radius=25, radix_bits=0) -> dict:
+        """Eager segmented build of several meshes in one launch sequence (bvh_build_batched)."""
+        k = len(dmeshes)
+        hm = (C.c_void_p * k)(*[m.h.value if hasattr(m.h, "value") else m.h for m in dmeshes])
+        ht = (C.c_void_p * k)(*[t.h.value if hasattr(t.h, "value") else t.h for t in trees])
+        p = N.BuildParams(algo, radius, 0, radix_bits, 0)
+        t = N.StageTimes()
+        self._check(self._lib.bvh_build_batched(self.h, hm, ht, k, C.byref(p), C.byref(t)), "build_batched")
+        return {"total_ms": float(t.total_ms), "stages": t.as_dict(), "ploc_iterations": int(t.ploc_iterations)}
+
     def trace(self, tree: "Tree", dmesh: "DeviceMesh", rays: np.ndarray, counters=False):
         """Closest hit for host rays (n, 6) float32; returns (t, prim[, counters], ms)."""
         rays = np.ascontiguousarray(rays, dtype=np.float32).reshape(-1, 6)
@@ -208,6 +230,28 @@ class Context:
             self._lib.bvh_device_free(self.h, dk)
             if dv.value:
                 self._lib.bvh_device_free(self.h, dv)
+
+
+class Batch:
+    """k meshes built into k trees by one captured hipGraph (bvh_batch_t)."""
+
+    def __init__(self, ctx: Context, h, trees):
+        self.ctx, self.h, self.trees = ctx, h, trees
+
+    def run(self) -> float:
+        ms = C.c_float(0)
+        self.ctx._check(self.ctx._lib.bvh_batch_run(self.h, C.byref(ms)), "batch run")
+        return float(ms.value)
+
+    def info(self) -> dict:
+        i = N.BatchInfo()
+        self.ctx._check(self.ctx._lib.bvh_batch_get_info(self.h, C.byref(i)), "batch info")
+        return i.as_dict()
+
+    def free(self):
+        if self.h:
+            self.ctx._lib.bvh_batch_free(self.h)
+            self.h = None
 
 
 class DeviceMesh:

@@ -21,6 +21,7 @@ STATUS = {
 }
 ALGO_LBVH, ALGO_LBVH_PLUS, ALGO_PLOC = 0, 1, 2
 BUILD_KEEP_KEYS = 1
+BUILD_TUNE = 2
 
 # Byte image of BVHNode (include/bvh_node.h): 48 bytes.
 NODE_DTYPE = np.dtype([
@@ -50,6 +51,22 @@ class StageTimes(C.Structure):
 
     def as_dict(self):
         return {self.names[i].value.decode(): float(self.ms[i]) for i in range(self.count)}
+
+
+class BatchInfo(C.Structure):
+    _fields_ = [("meshes", C.c_uint32), ("triangles", C.c_uint32), ("max_triangles", C.c_uint32),
+                ("algo", C.c_int32), ("radix_bits", C.c_int32), ("sort_passes", C.c_int32), ("key_bits", C.c_int32),
+                ("ploc_captured_iterations", C.c_int32), ("ploc_iterations", C.c_int32),
+                ("ploc_nn_grid", C.c_int32), ("ploc_merge_grid", C.c_int32), ("graph_nodes", C.c_int32),
+                ("runs", C.c_int32), ("eager_continuations", C.c_int32), ("tune_count", C.c_int32),
+                ("tune_ms", C.c_float * 8), ("tune_radix_bits", C.c_int32 * 8), ("tune_nn_grid", C.c_int32 * 8)]
+
+    def as_dict(self):
+        d = {name: getattr(self, name) for name, _ in self._fields_
+             if name not in ("tune_ms", "tune_radix_bits", "tune_nn_grid")}
+        d["tune"] = [{"ms": float(self.tune_ms[i]), "radix_bits": int(self.tune_radix_bits[i]),
+                      "nn_grid": int(self.tune_nn_grid[i])} for i in range(min(self.tune_count, 8))]
+        return d
 
 
 _hip = None
@@ -86,6 +103,12 @@ _HIP_SIGS = {
     "bvh_debug_sorted_keys": (_i32, [_vp, _vp, _vp, _u32]),
     "bvh_sort_pairs_u64": (_i32, [_vp, _vp, _vp, _u32, C.c_int, C.c_int, C.c_int, C.POINTER(_f)]),
     "bvh_debug_stream_copy": (_i32, [_vp, _vp, _vp, _sz, C.POINTER(_f)]),
+    "bvh_build_batched": (_i32, [_vp, C.POINTER(_vp), C.POINTER(_vp), _u32, C.POINTER(BuildParams),
+                                 C.POINTER(StageTimes)]),
+    "bvh_batch_create": (_i32, [_vp, C.POINTER(_vp), C.POINTER(_vp), _u32, C.POINTER(BuildParams), _pp]),
+    "bvh_batch_run": (_i32, [_vp, C.POINTER(_f)]),
+    "bvh_batch_get_info": (_i32, [_vp, C.POINTER(BatchInfo)]),
+    "bvh_batch_free": (_i32, [_vp]),
     "bvh_trace": (_i32, [_vp, _vp, _vp, _vp, _u32, _vp, _vp, _vp, C.POINTER(_f)]),
     "bvh_trace_host": (_i32, [_vp, _vp, _vp, _vp, _u32, _vp, _vp, _vp, C.POINTER(_f)]),
 }

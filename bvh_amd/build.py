@@ -29,7 +29,7 @@ HIP_FLAGS = [
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
     "-Wno-unused-command-line-argument", f"-I{os.path.join(ROOT, 'include')}",
 ]
-HOST_FLAGS = ["-O2", "-fPIC", "-std=c++17", "-ffp-contract=off", f"-I{os.path.join(ROOT, 'include')}",
+HOST_FLAGS = ["-O2", "-fPIC", "-pthread", "-std=c++17", "-ffp-contract=off", f"-I{os.path.join(ROOT, 'include')}",
               f"-I{HERE}"]
 
 
@@ -67,7 +67,7 @@ def build_all(verbose: bool = False) -> None:
     if host_objs:
         hlib = os.path.join(HERE, "libbvhhost.so")
         _run([CXX, "-shared", "-fPIC", *host_objs, "-o", hlib + ".tmp", f"-L{HERE}", "-lbvhhip",
-              "-Wl,-rpath,$ORIGIN", "-Wl,--no-undefined", "-L/opt/rocm/lib", "-lamdhip64"])
+              "-Wl,-rpath,$ORIGIN", "-Wl,--no-undefined", "-L/opt/rocm/lib", "-lamdhip64", "-pthread"])
         os.replace(hlib + ".tmp", hlib)
     if verbose:
         print("built", lib)

@@ -2,8 +2,13 @@
 //
 // A context owns one HIP stream (default priority), a set of timing events and
 // a grow-only device workspace reused across builds, so steady-state builds
-// allocate nothing.  Errors never exit: they set the context's message and
-// return a status code.
+// allocate nothing.  A batch owns a workspace of its own plus a captured
+// hipGraph of its whole launch sequence.  Errors never exit: they set the
+// context's message and return a status code.
+//
+// Every build is expressed as a job over segments (meshes): a standalone build
+// is a job of one segment, a batch is a job of k.  Both run the same kernels,
+// so each batched tree is bit-identical to its standalone build.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -25,17 +30,43 @@ struct DevBuf {
     size_t cap = 0;
 };
 
+// Device workspace of one build pipeline.
+struct Workspace {
+    DevBuf tri_boxes, keys_a, keys_b, vals_a, vals_b, hist_part, bin_start, lookback, scene, handoff, epoch;
+    DevBuf parents, ranges, counters;  // LBVH+
+    DevBuf ploc_box[2], ploc_id[2], ploc_sid[2], ploc_seg[2], ploc_nn, ploc_state, ploc_status, ploc_out;
+    template <class F>
+    void each(F f) {
+        DevBuf* all[] = {&tri_boxes,   &keys_a,      &keys_b,      &vals_a,      &vals_b,     &hist_part,
+                         &bin_start,   &lookback,    &scene,       &handoff,     &epoch,      &parents,
+                         &ranges,      &counters,    &ploc_box[0], &ploc_box[1], &ploc_id[0], &ploc_id[1],
+                         &ploc_sid[0], &ploc_sid[1], &ploc_seg[0], &ploc_seg[1], &ploc_nn,    &ploc_state,
+                         &ploc_status, &ploc_out};
+        for (DevBuf* b : all) f(*b);
+    }
+};
+
+// Segment table and block tables of a segmented job.
+struct SegTables {
+    DevBuf segs, fblocks, hblocks;
+    std::vector<SegDesc> host_segs;
+    void release() {
+        for (DevBuf* b : {&segs, &fblocks, &hblocks}) {
+            if (b->p) (void)hipFree(b->p);
+            b->p = nullptr;
+            b->cap = 0;
+        }
+    }
+};
+
 struct bvh_context {
     int device = 0;
     hipStream_t stream = nullptr;
     hipEvent_t ev[BVH_MAX_STAGES + 1] = {};
     std::string err;
-    uint32_t epoch = 0;
-    // workspace
-    DevBuf tri_boxes, keys_a, keys_b, vals_a, vals_b, hist_part, bin_start, lookback, scene, handoff, parents,
-        ranges;
-    DevBuf ploc_box[2], ploc_id[2], ploc_nn, ploc_state, ploc_status;
-    DevBuf counters;  // LBVH+ arrival words
+    uint32_t treelet_epoch = 0;  // LBVH+ arrival-word tags
+    Workspace ws;
+    SegTables tabs;  // eager segmented builds
     DevBuf trace_ovf, trace_list, trace_io;  // traversal queue and host-call staging
     // last build's sorted keys (for bvh_debug_sorted_keys)
     uint64_t* last_keys = nullptr;
@@ -62,6 +93,36 @@ struct bvh_tree_rec {
     uint32_t num_prims = 0;
 };
 
+// A build job: the segments and how kernel blocks map onto them.
+struct Job {
+    SegBlocks fsb;        // front blocks (kFrontChunk triangles each)
+    SegBlocks hsb;        // hierarchy blocks (hier_chunk() leaves each)
+    uint32_t k = 1;       // segments
+    uint32_t total = 0;   // triangles over all segments
+    uint32_t maxn = 0;    // largest segment
+    KeyFormat kf{32, 62};
+    bool m60 = false;
+    int begin_bit = 32, end_bit = 62;
+};
+
+struct bvh_batch_rec {
+    bvh_ctx_t ctx = nullptr;
+    int algo = BVH_ALGO_LBVH;
+    int radius = 25;
+    uint32_t k = 0;
+    std::vector<bvh_mesh_t> meshes;
+    std::vector<bvh_tree_t> trees;
+    SegTables tabs;
+    Workspace ws;
+    Job job;
+    int radix_bits = 10;
+    int ploc_iters = 0;   // global PLOC iterations captured in the graph
+    uint32_t nn_grid = 0, merge_grid = 0;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    bvh_batch_info info;
+};
+
 namespace {
 
 int set_err(bvh_ctx_t ctx, int code, const char* fmt, ...) {
@@ -82,6 +143,14 @@ int set_err(bvh_ctx_t ctx, int code, const char* fmt, ...) {
         if (e_ != hipSuccess) return set_err(ctx, BVH_E_HIP, "%s failed: %s", #call, hipGetErrorString(e_)); \
     } while (0)
 
+#define TRY(call)                  \
+    do {                           \
+        int st_ = (call);          \
+        if (st_) return st_;       \
+    } while (0)
+
+// Grow-only allocation; a new buffer is zeroed when `zero` is set (never
+// called while a stream is being captured).
 int ensure(bvh_ctx_t ctx, DevBuf& b, size_t bytes, bool zero = false) {
     if (bytes == 0) bytes = 16;
     if (b.cap >= bytes) return BVH_OK;
@@ -92,6 +161,7 @@ int ensure(bvh_ctx_t ctx, DevBuf& b, size_t bytes, bool zero = false) {
         b.cap = 0;
     }
     size_t want = bytes + bytes / 8;
+    want = (want + 255) & ~(size_t)255;
     hipError_t e = hipMalloc(&b.p, want);
     if (e != hipSuccess) {
         b.p = nullptr;
@@ -154,66 +224,110 @@ struct StageClock {
     }
 };
 
-int pick_bits(int requested, int span) {
+// Stage mark on an optional clock (captured sequences carry none).
+int mark(StageClock* clk, const char* name) { return clk ? clk->mark(name) : BVH_OK; }
+
+int pick_bits(int requested) {
     if (requested == 8 || requested == 10) return requested;
-    // Tuned default: 10-bit digits (3 passes over 30 bits, 6 over 60).
-    (void)span;
-    return 10;
+    return 10;  // tuned default: 3 passes over 30 bits, 6 over 60
 }
 
 int ensure_tree(bvh_ctx_t ctx, bvh_tree_t tree, uint32_t n) {
-    int st;
-    if ((st = ensure(ctx, tree->nodes, (size_t)(2 * (size_t)n - 1) * sizeof(DNode)))) return st;
-    if ((st = ensure(ctx, tree->indices, (size_t)n * 4))) return st;
+    TRY(ensure(ctx, tree->nodes, (size_t)(2 * (size_t)n - 1) * sizeof(DNode)));
+    TRY(ensure(ctx, tree->indices, (size_t)n * 4));
     tree->num_prims = n;
     tree->num_nodes = 2 * n - 1;
     return BVH_OK;
 }
 
-// Front end + sort shared by all builders.  m60 selects the LBVH+ key format.
-int build_front(bvh_ctx_t ctx, const MeshView& m, bool m60, int radix_bits, StageClock& clk, uint64_t** keys_out,
-                uint32_t** vals_out) {
-    const uint32_t n = m.n;
-    int st;
-    const int begin_bit = m60 ? 0 : 32;
-    const int end_bit = m60 ? 60 : 62;
-    SortPlan plan = make_sort_plan(n, begin_bit, end_bit, pick_bits(radix_bits, end_bit - begin_bit));
-    const size_t bins = 1u << plan.bits;
-    const size_t zero_bytes = (sort_zero_bytes(plan) + 15) & ~(size_t)15;
-    if ((st = ensure(ctx, ctx->tri_boxes, (size_t)n * 32))) return st;
-    if ((st = ensure(ctx, ctx->keys_a, (size_t)n * 8))) return st;
-    if ((st = ensure(ctx, ctx->keys_b, (size_t)n * 8))) return st;
-    if (m60) {
-        if ((st = ensure(ctx, ctx->vals_a, (size_t)n * 4))) return st;
-        if ((st = ensure(ctx, ctx->vals_b, (size_t)n * 4))) return st;
-    }
-    if ((st = ensure(ctx, ctx->hist_part, (size_t)plan.hist_blocks * plan.passes * bins * 4))) return st;
-    if ((st = ensure(ctx, ctx->bin_start, (size_t)plan.passes * bins * 4))) return st;
-    if ((st = ensure(ctx, ctx->lookback, zero_bytes))) return st;
-    if ((st = ensure(ctx, ctx->scene, 256))) return st;
+SegDesc seg_of(const bvh_mesh_rec* mesh, bvh_tree_rec* tree, uint32_t start) {
+    SegDesc d;
+    for (int k = 0; k < 9; ++k) d.v[k] = mesh->base + k * mesh->stride;
+    d.n = mesh->n;
+    d.start = start;
+    d.nodes = tree ? P<DNode>(tree->nodes) : nullptr;
+    d.indices = tree ? P<uint32_t>(tree->indices) : nullptr;
+    return d;
+}
 
-    if ((st = clk.begin())) return st;
-    launch_init_scene(P<uint32_t>(ctx->scene), ctx->stream);
-    launch_tri_bounds(m, P<float4>(ctx->tri_boxes), P<uint32_t>(ctx->scene), ctx->stream);
-    if ((st = check_launch(ctx, "k_tri_bounds"))) return st;
-    if ((st = clk.mark("Bounds/Centroids"))) return st;
-    launch_morton(P<float4>(ctx->tri_boxes), n, P<uint32_t>(ctx->scene), m60, P<uint64_t>(ctx->keys_a),
-                  m60 ? P<uint32_t>(ctx->vals_a) : nullptr, plan, P<uint32_t>(ctx->hist_part),
-                  P<uint32_t>(ctx->lookback), zero_bytes / 4, ctx->stream);
-    launch_hist_scan(P<uint32_t>(ctx->hist_part), plan, P<uint32_t>(ctx->bin_start), ctx->stream);
-    if ((st = check_launch(ctx, "k_morton"))) return st;
-    if ((st = clk.mark("Morton Codes"))) return st;
+uint32_t cdiv(uint32_t a, uint32_t b) { return (a + b - 1) / b; }
+
+int bit_width(uint32_t v) {  // bits needed for values 0..v
+    int b = 0;
+    while (b < 32 && (v >> b) != 0) ++b;
+    return b;
+}
+
+// Standalone build: one segment, keys morton30 << 32 | index (or morton60).
+Job single_job(const bvh_mesh_rec* mesh, bvh_tree_rec* tree, bool m60) {
+    Job j;
+    j.fsb.segs = nullptr;
+    j.fsb.blocks = nullptr;
+    j.fsb.single = seg_of(mesh, tree, 0);
+    j.fsb.nblocks = cdiv(mesh->n, kFrontChunk);
+    j.hsb = j.fsb;
+    j.hsb.nblocks = cdiv(mesh->n, (uint32_t)hier_chunk());
+    j.k = 1;
+    j.total = mesh->n;
+    j.maxn = mesh->n;
+    j.m60 = m60;
+    j.kf = KeyFormat{32, 62};
+    j.begin_bit = m60 ? 0 : 32;
+    j.end_bit = m60 ? 60 : 62;
+    return j;
+}
+
+SortPlan plan_for(const Job& j, int radix_bits) {
+    return make_sort_plan(j.total, j.begin_bit, j.end_bit, pick_bits(radix_bits), (int)j.fsb.nblocks);
+}
+
+size_t zero_bytes_of(const SortPlan& plan) { return (sort_zero_bytes(plan) + 15) & ~(size_t)15; }
+
+// Allocations of the front end and the hierarchy (outside any capture).
+int reserve_front(bvh_ctx_t ctx, Workspace& w, const Job& j, const SortPlan& plan) {
+    const size_t n = j.total;
+    const size_t bins = 1u << plan.bits;
+    TRY(ensure(ctx, w.tri_boxes, n * 32));
+    TRY(ensure(ctx, w.keys_a, n * 8));
+    TRY(ensure(ctx, w.keys_b, n * 8));
+    if (j.m60) {
+        TRY(ensure(ctx, w.vals_a, n * 4));
+        TRY(ensure(ctx, w.vals_b, n * 4));
+    }
+    TRY(ensure(ctx, w.hist_part, (size_t)plan.hist_blocks * plan.passes * bins * 4));
+    TRY(ensure(ctx, w.bin_start, (size_t)plan.passes * bins * 4));
+    TRY(ensure(ctx, w.lookback, zero_bytes_of(plan)));
+    TRY(ensure(ctx, w.scene, (size_t)j.k * 32));
+    TRY(ensure(ctx, w.handoff, n * 8, true));
+    TRY(ensure(ctx, w.epoch, 16, true));
+    return BVH_OK;
+}
+
+// Bounds, Morton keys + histograms, and the LSD passes.  Result in *keys/*vals.
+int enqueue_front(bvh_ctx_t ctx, Workspace& w, const Job& j, const SortPlan& plan, StageClock* clk,
+                  uint64_t** keys_out, uint32_t** vals_out) {
+    hipStream_t s = ctx->stream;
+    launch_init_build(P<uint32_t>(w.scene), j.k, P<uint32_t>(w.epoch), s);
+    launch_tri_bounds(j.fsb, P<float4>(w.tri_boxes), P<uint32_t>(w.scene), s);
+    TRY(check_launch(ctx, "k_tri_bounds"));
+    TRY(mark(clk, "Bounds/Centroids"));
+    launch_morton(j.fsb, P<float4>(w.tri_boxes), P<uint32_t>(w.scene), j.m60, j.kf, P<uint64_t>(w.keys_a),
+                  j.m60 ? P<uint32_t>(w.vals_a) : nullptr, plan, P<uint32_t>(w.hist_part), P<uint32_t>(w.lookback),
+                  zero_bytes_of(plan) / 4, s);
+    launch_hist_scan(P<uint32_t>(w.hist_part), plan, P<uint32_t>(w.bin_start), s);
+    TRY(check_launch(ctx, "k_morton"));
+    TRY(mark(clk, "Morton Codes"));
     static const char* pass_names[8] = {"Sort Pass 1", "Sort Pass 2", "Sort Pass 3", "Sort Pass 4",
                                         "Sort Pass 5", "Sort Pass 6", "Sort Pass 7", "Sort Pass 8"};
-    uint64_t* kin = P<uint64_t>(ctx->keys_a);
-    uint64_t* kout = P<uint64_t>(ctx->keys_b);
-    uint32_t* vin = m60 ? P<uint32_t>(ctx->vals_a) : nullptr;
-    uint32_t* vout = m60 ? P<uint32_t>(ctx->vals_b) : nullptr;
+    uint64_t* kin = P<uint64_t>(w.keys_a);
+    uint64_t* kout = P<uint64_t>(w.keys_b);
+    uint32_t* vin = j.m60 ? P<uint32_t>(w.vals_a) : nullptr;
+    uint32_t* vout = j.m60 ? P<uint32_t>(w.vals_b) : nullptr;
     for (int p = 0; p < plan.passes; ++p) {
-        launch_onesweep_pass(plan, p, n, kin, kout, vin, vout, P<uint32_t>(ctx->bin_start),
-                             P<uint32_t>(ctx->lookback), ctx->stream);
-        if ((st = check_launch(ctx, "k_onesweep"))) return st;
-        if ((st = clk.mark(pass_names[p]))) return st;
+        launch_onesweep_pass(plan, p, j.total, kin, kout, vin, vout, P<uint32_t>(w.bin_start),
+                             P<uint32_t>(w.lookback), s);
+        TRY(check_launch(ctx, "k_onesweep"));
+        TRY(mark(clk, pass_names[p]));
         std::swap(kin, kout);
         std::swap(vin, vout);
     }
@@ -222,78 +336,78 @@ int build_front(bvh_ctx_t ctx, const MeshView& m, bool m60, int radix_bits, Stag
     return BVH_OK;
 }
 
-int check_input_flag(bvh_ctx_t ctx) {
-    uint32_t flag[2] = {0, 0};
-    HIP_TRY(ctx, hipMemcpyAsync(flag, P<uint32_t>(ctx->scene) + 6, 8, hipMemcpyDeviceToHost, ctx->stream));
+HierArgs hier_args(Workspace& w, const Job& j, const uint64_t* keys, const uint32_t* vals, bool links) {
+    HierArgs ha;
+    ha.sb = j.hsb;
+    ha.keys = keys;
+    ha.vals = vals;
+    ha.kf = j.kf;
+    ha.tri_boxes = P<float4>(w.tri_boxes);
+    ha.parents = links ? P<uint32_t>(w.parents) : nullptr;
+    ha.ranges = links ? P<int2>(w.ranges) : nullptr;
+    ha.handoff = P<uint64_t>(w.handoff);
+    ha.epoch = P<uint32_t>(w.epoch);
+    ha.err = P<uint32_t>(w.scene) + 7;
+    return ha;
+}
+
+// Non-finite input and internal-error words of every segment.
+int check_input_flags(bvh_ctx_t ctx, Workspace& w, uint32_t k) {
+    std::vector<uint32_t> sc((size_t)k * 8);
+    HIP_TRY(ctx, hipMemcpyAsync(sc.data(), w.scene.p, sc.size() * 4, hipMemcpyDeviceToHost, ctx->stream));
     HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
-    if (flag[0]) return set_err(ctx, BVH_E_NONFINITE, "mesh contains NaN or infinite vertex coordinates");
-    if (flag[1]) return set_err(ctx, BVH_E_STATE, "internal error: sorted keys name no primitive");
+    for (uint32_t s = 0; s < k; ++s) {
+        if (sc[8 * s + 6]) {
+            if (k == 1) return set_err(ctx, BVH_E_NONFINITE, "mesh contains NaN or infinite vertex coordinates");
+            return set_err(ctx, BVH_E_NONFINITE, "mesh %u of the batch contains NaN or infinite vertex coordinates", s);
+        }
+    }
+    if (sc[7]) return set_err(ctx, BVH_E_STATE, "internal error: sorted keys name no primitive");
     return BVH_OK;
 }
 
 int build_lbvh(bvh_ctx_t ctx, bvh_mesh_t mesh, const bvh_build_params* prm, bvh_tree_t tree, bvh_stage_times* times) {
-    const MeshView m = mesh->view();
-    const uint32_t n = m.n;
-    int st;
-    if ((st = ensure_tree(ctx, tree, n))) return st;
-    if ((st = ensure(ctx, ctx->handoff, (size_t)n * 8, true))) return st;
+    TRY(ensure_tree(ctx, tree, mesh->n));
+    Job j = single_job(mesh, tree, false);
+    SortPlan plan = plan_for(j, prm->radix_bits);
+    Workspace& w = ctx->ws;
+    TRY(reserve_front(ctx, w, j, plan));
     StageClock clk{ctx};
+    TRY(clk.begin());
     uint64_t* keys = nullptr;
     uint32_t* vals = nullptr;
-    if ((st = build_front(ctx, m, false, prm->radix_bits, clk, &keys, &vals))) return st;
-    HierArgs ha;
-    ha.keys = keys;
-    ha.vals = nullptr;
-    ha.n = n;
-    ha.tri_boxes = P<float4>(ctx->tri_boxes);
-    ha.nodes = P<DNode>(tree->nodes);
-    ha.indices = P<uint32_t>(tree->indices);
-    ha.parents = nullptr;
-    ha.ranges = nullptr;
-    ha.handoff = P<uint64_t>(ctx->handoff);
-    ha.epoch = ++ctx->epoch;
-    ha.err = P<uint32_t>(ctx->scene) + 7;
-    launch_hierarchy(ha, ctx->stream);
-    if ((st = check_launch(ctx, "k_hierarchy"))) return st;
-    if ((st = clk.mark("Topology+Refit"))) return st;
-    if ((st = clk.finish(times))) return st;
+    TRY(enqueue_front(ctx, w, j, plan, &clk, &keys, &vals));
+    launch_hierarchy(hier_args(w, j, keys, nullptr, false), ctx->stream);
+    TRY(check_launch(ctx, "k_hierarchy"));
+    TRY(clk.mark("Topology+Refit"));
+    TRY(clk.finish(times));
     ctx->last_keys = keys;
     ctx->last_vals = nullptr;
-    ctx->last_n = n;
-    return check_input_flag(ctx);
+    ctx->last_n = mesh->n;
+    return check_input_flags(ctx, w, 1);
 }
 
 int build_lbvh_plus(bvh_ctx_t ctx, bvh_mesh_t mesh, const bvh_build_params* prm, bvh_tree_t tree,
                     bvh_stage_times* times) {
-    const MeshView m = mesh->view();
-    const uint32_t n = m.n;
+    const uint32_t n = mesh->n;
     const int rounds = prm->treelet_rounds;
     if (rounds < 0 || rounds > 64) return set_err(ctx, BVH_E_ARG, "treelet_rounds %d out of range [0, 64]", rounds);
-    int st;
-    if ((st = ensure_tree(ctx, tree, n))) return st;
-    if ((st = ensure(ctx, ctx->handoff, (size_t)n * 8, true))) return st;
-    if ((st = ensure(ctx, ctx->parents, (size_t)(2 * (size_t)n) * 4))) return st;
-    if ((st = ensure(ctx, ctx->ranges, (size_t)n * 8))) return st;
-    if ((st = ensure(ctx, ctx->counters, (size_t)n * 4, true))) return st;
+    TRY(ensure_tree(ctx, tree, n));
+    Job j = single_job(mesh, tree, true);
+    SortPlan plan = plan_for(j, prm->radix_bits);
+    Workspace& w = ctx->ws;
+    TRY(reserve_front(ctx, w, j, plan));
+    TRY(ensure(ctx, w.parents, (size_t)(2 * (size_t)n) * 4));
+    TRY(ensure(ctx, w.ranges, (size_t)n * 8));
+    TRY(ensure(ctx, w.counters, (size_t)n * 4, true));
     StageClock clk{ctx};
+    TRY(clk.begin());
     uint64_t* keys = nullptr;
     uint32_t* vals = nullptr;
-    if ((st = build_front(ctx, m, true, prm->radix_bits, clk, &keys, &vals))) return st;
-    HierArgs ha;
-    ha.keys = keys;
-    ha.vals = vals;
-    ha.n = n;
-    ha.tri_boxes = P<float4>(ctx->tri_boxes);
-    ha.nodes = P<DNode>(tree->nodes);
-    ha.indices = P<uint32_t>(tree->indices);
-    ha.parents = P<uint32_t>(ctx->parents);
-    ha.ranges = P<int2>(ctx->ranges);
-    ha.handoff = P<uint64_t>(ctx->handoff);
-    ha.epoch = ++ctx->epoch;
-    ha.err = P<uint32_t>(ctx->scene) + 7;
-    launch_hierarchy(ha, ctx->stream);
-    if ((st = check_launch(ctx, "k_hierarchy"))) return st;
-    if ((st = clk.mark("Topology+Refit"))) return st;
+    TRY(enqueue_front(ctx, w, j, plan, &clk, &keys, &vals));
+    launch_hierarchy(hier_args(w, j, keys, vals, true), ctx->stream);
+    TRY(check_launch(ctx, "k_hierarchy"));
+    TRY(clk.mark("Topology+Refit"));
     static const char* round_names[8] = {"Treelet Round 1", "Treelet Round 2", "Treelet Round 3",
                                          "Treelet Round 4", "Treelet Round 5", "Treelet Round 6",
                                          "Treelet Round 7", "Treelet Rounds 8+"};
@@ -301,102 +415,457 @@ int build_lbvh_plus(bvh_ctx_t ctx, bvh_mesh_t mesh, const bvh_build_params* prm,
         TreeletArgs ta;
         ta.n = n;
         ta.nodes = P<DNode>(tree->nodes);
-        ta.parents = P<uint32_t>(ctx->parents);
-        ta.ranges = P<int2>(ctx->ranges);
-        ta.counters = P<uint32_t>(ctx->counters);
-        ta.epoch = ++ctx->epoch;
+        ta.parents = P<uint32_t>(w.parents);
+        ta.ranges = P<int2>(w.ranges);
+        ta.counters = P<uint32_t>(w.counters);
+        ta.epoch = ++ctx->treelet_epoch;
         launch_treelet(ta, ctx->stream);
-        if ((st = check_launch(ctx, "k_treelet"))) return st;
-        if (r < 7 || r == rounds - 1)
-            if ((st = clk.mark(round_names[r < 7 ? r : 7]))) return st;
+        TRY(check_launch(ctx, "k_treelet"));
+        if (r < 7 || r == rounds - 1) TRY(clk.mark(round_names[r < 7 ? r : 7]));
     }
-    if ((st = clk.finish(times))) return st;
+    TRY(clk.finish(times));
     ctx->last_keys = keys;
     ctx->last_vals = vals;
     ctx->last_n = n;
-    return check_input_flag(ctx);
+    return check_input_flags(ctx, w, 1);
+}
+
+// ---- PLOC ---------------------------------------------------------------------
+
+int reserve_ploc(bvh_ctx_t ctx, Workspace& w, const Job& j) {
+    const size_t n = j.total;
+    for (int b = 0; b < 2; ++b) {
+        TRY(ensure(ctx, w.ploc_box[b], n * 32));
+        TRY(ensure(ctx, w.ploc_id[b], n * 4));
+        TRY(ensure(ctx, w.ploc_seg[b], (size_t)j.k * 16));
+        if (j.k > 1) TRY(ensure(ctx, w.ploc_sid[b], n * 4));
+    }
+    TRY(ensure(ctx, w.ploc_nn, n * 4));
+    TRY(ensure(ctx, w.ploc_state, (size_t)kPlocMaxIter * sizeof(PlocState)));
+    TRY(ensure(ctx, w.ploc_status, (n / (size_t)ploc_merge_tile() + 2) * 8 + 64));
+    TRY(ensure(ctx, w.ploc_out, (size_t)j.k * 16));
+    return BVH_OK;
+}
+
+PlocArgs ploc_args(Workspace& w, const Job& j, int radius) {
+    PlocArgs a;
+    a.sb = j.fsb;
+    a.nseg = j.k;
+    a.radius = radius;
+    a.m0 = j.total;
+    a.maxm0 = j.maxn;
+    for (int b = 0; b < 2; ++b) {
+        a.box[b] = P<float4>(w.ploc_box[b]);
+        a.id[b] = P<uint32_t>(w.ploc_id[b]);
+        a.sid[b] = j.k > 1 ? P<uint32_t>(w.ploc_sid[b]) : nullptr;
+        a.seg[b] = P<uint4>(w.ploc_seg[b]);
+    }
+    a.nn = P<int32_t>(w.ploc_nn);
+    a.state = P<PlocState>(w.ploc_state);
+    a.status = P<uint64_t>(w.ploc_status);
+    a.tile_ctr = reinterpret_cast<uint32_t*>(a.status + (j.total / (uint32_t)ploc_merge_tile() + 2));
+    a.out = P<uint4>(w.ploc_out);
+    return a;
+}
+
+// Most global PLOC iterations captured into a batch graph (more run eagerly).
+constexpr int kPlocCaptureMax = 512;
+
+// Global iterations a mesh of m clusters needs before it is small (measured
+// decay of the cluster count is 0.72-0.87 per iteration).
+int ploc_iter_estimate(uint32_t m, double ratio) {
+    if (m <= (uint32_t)kPlocSmall) return 0;
+    return (int)std::ceil(std::log((double)m / kPlocSmall) / std::log(1.0 / ratio));
+}
+
+// Host-driven global iterations from ring slot *slot until every mesh is
+// small (or a flag is raised): batches of launches, one 32-byte state read
+// per batch.  m_bound bounds the clusters entering slot *slot.
+int ploc_drive(bvh_ctx_t ctx, const PlocArgs& a, int* slot, uint32_t m_bound, uint32_t maxm, uint32_t nn_grid,
+               uint32_t merge_grid, PlocState* last) {
+    int k = *slot;
+    PlocState hs;
+    std::memset(&hs, 0, sizeof(hs));
+    hs.m = m_bound;
+    hs.maxm = maxm;
+    while (hs.maxm > (uint32_t)kPlocSmall && hs.flags == 0) {
+        int batch = ploc_iter_estimate(hs.maxm, 0.80);
+        batch = batch < 2 ? 2 : (batch > 48 ? 48 : batch);
+        for (int b = 0; b < batch; ++b, k = (k + 1) % kPlocMaxIter) {
+            launch_ploc_nn(a, k, std::min(hs.m, nn_grid * (uint32_t)ploc_nn_block()), ctx->stream);
+            launch_ploc_merge(a, k, std::min(hs.m, merge_grid * (uint32_t)ploc_merge_tile()), ctx->stream);
+        }
+        TRY(check_launch(ctx, "k_ploc_nn/merge"));
+        HIP_TRY(ctx, hipMemcpyAsync(&hs, a.state + k, sizeof(hs), hipMemcpyDeviceToHost, ctx->stream));
+        HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+    }
+    *slot = k;
+    if (last) *last = hs;
+    return BVH_OK;
+}
+
+// Checks the per-mesh records the tail wrote.  *unfinished is set when the
+// tail ran before every mesh was small.
+int ploc_check(bvh_ctx_t ctx, Workspace& w, const Job& j, const std::vector<uint32_t>& ns, bool* unfinished,
+               uint32_t* max_iters) {
+    std::vector<uint4> out(j.k);
+    HIP_TRY(ctx, hipMemcpyAsync(out.data(), w.ploc_out.p, (size_t)j.k * 16, hipMemcpyDeviceToHost, ctx->stream));
+    HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+    *unfinished = false;
+    uint32_t mi = 0;
+    for (uint32_t s = 0; s < j.k; ++s) {
+        const uint4 o = out[s];
+        if (o.w & kPlocUnfinished) {
+            *unfinished = true;
+            continue;
+        }
+        if (o.w & kPlocNoProgress)
+            return set_err(ctx, BVH_E_NOPROGRESS, "PLOC iteration %u of mesh %u made no merge (%u clusters left)", o.z,
+                           s, o.x);
+        const uint32_t n = ns[s];
+        if (o.x != 1 || o.y != 2 * n - 1)
+            return set_err(ctx, BVH_E_STATE, "PLOC of mesh %u ended with %u clusters and %u nodes", s, o.x, o.y);
+        mi = std::max(mi, o.z);
+    }
+    if (max_iters) *max_iters = mi;
+    return BVH_OK;
 }
 
 int build_ploc(bvh_ctx_t ctx, bvh_mesh_t mesh, const bvh_build_params* prm, bvh_tree_t tree, bvh_stage_times* times) {
-    const MeshView m = mesh->view();
-    const uint32_t n = m.n;
+    const uint32_t n = mesh->n;
     const int radius = prm->ploc_radius;
     if (radius < 1 || radius > 1024) return set_err(ctx, BVH_E_ARG, "PLOC radius %d out of range [1, 1024]", radius);
-    int st;
-    if ((st = ensure_tree(ctx, tree, n))) return st;
+    TRY(ensure_tree(ctx, tree, n));
+    Job j = single_job(mesh, tree, false);
+    SortPlan plan = plan_for(j, prm->radix_bits);
+    Workspace& w = ctx->ws;
+    TRY(reserve_front(ctx, w, j, plan));
+    TRY(reserve_ploc(ctx, w, j));
     StageClock clk{ctx};
+    TRY(clk.begin());
     uint64_t* keys = nullptr;
     uint32_t* vals = nullptr;
-    if ((st = build_front(ctx, m, false, prm->radix_bits, clk, &keys, &vals))) return st;
-    const size_t nn = n;
-    if ((st = ensure(ctx, ctx->ploc_box[0], nn * 32))) return st;
-    if ((st = ensure(ctx, ctx->ploc_box[1], nn * 32))) return st;
-    if ((st = ensure(ctx, ctx->ploc_id[0], nn * 4))) return st;
-    if ((st = ensure(ctx, ctx->ploc_id[1], nn * 4))) return st;
-    if ((st = ensure(ctx, ctx->ploc_nn, nn * 4))) return st;
-    if ((st = ensure(ctx, ctx->ploc_state, (size_t)(kPlocMaxIter + 2) * sizeof(PlocState)))) return st;
-    if ((st = ensure(ctx, ctx->ploc_status, (nn / 256 + 2) * 8 + 64))) return st;
-    PlocArgs a;
-    a.n = n;
-    a.radius = radius;
-    for (int b = 0; b < 2; ++b) {
-        a.box[b] = P<float4>(ctx->ploc_box[b]);
-        a.id[b] = P<uint32_t>(ctx->ploc_id[b]);
-    }
-    a.nn = P<int32_t>(ctx->ploc_nn);
-    a.nodes = P<DNode>(tree->nodes);
-    a.state = P<PlocState>(ctx->ploc_state);
-    a.status = P<uint64_t>(ctx->ploc_status);
-    a.tile_ctr = reinterpret_cast<uint32_t*>(a.status + (nn / 256 + 2));
-    a.final_slot = kPlocMaxIter + 1;
-    launch_ploc_init(a, keys, P<float4>(ctx->tri_boxes), P<uint32_t>(tree->indices), ctx->stream);
-    if ((st = check_launch(ctx, "k_ploc_init"))) return st;
-    if ((st = clk.mark("PLOC Init"))) return st;
-    PlocState hs;
-    std::memset(&hs, 0, sizeof(hs));
-    hs.m = n;
-    int k = 0;
-    if (n > 1) {
-        // Global iterations in batches; the device skips iterations once the
-        // cluster count is small, so a batch may overshoot harmlessly.
-        uint32_t m_bound = n;
-        while (m_bound > (uint32_t)kPlocSmall) {
-            double est = std::log((double)m_bound / kPlocSmall) / std::log(1.0 / 0.72);
-            int batch = (int)std::ceil(est);
-            batch = batch < 2 ? 2 : (batch > 48 ? 48 : batch);
-            // State slots form a ring (a batch never spans more than 48 of them).
-            for (int b = 0; b < batch; ++b, k = (k + 1) % kPlocMaxIter) {
-                launch_ploc_nn(a, k, m_bound, ctx->stream);
-                launch_ploc_merge(a, k, m_bound, ctx->stream);
-            }
-            if ((st = check_launch(ctx, "k_ploc_nn/merge"))) return st;
-            HIP_TRY(ctx, hipMemcpyAsync(&hs, a.state + k, sizeof(hs), hipMemcpyDeviceToHost, ctx->stream));
-            HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
-            if (hs.flags) break;
-            m_bound = hs.m;
-        }
-        if ((st = clk.mark("PLOC Global Iterations"))) return st;
-        launch_ploc_tail(a, k, ctx->stream);
-        if ((st = check_launch(ctx, "k_ploc_tail"))) return st;
-        if ((st = clk.mark("PLOC LDS Tail"))) return st;
-    }
-    if ((st = clk.finish(times))) return st;
-    if (n > 1) {
-        HIP_TRY(ctx, hipMemcpyAsync(&hs, a.state + a.final_slot, sizeof(hs), hipMemcpyDeviceToHost, ctx->stream));
-        HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
-    }
-    if (times) times->ploc_iterations = (int32_t)hs.iters;
+    TRY(enqueue_front(ctx, w, j, plan, &clk, &keys, &vals));
+    const PlocArgs a = ploc_args(w, j, radius);
+    launch_ploc_init(a, keys, j.kf, P<float4>(w.tri_boxes), P<uint32_t>(w.scene) + 7, ctx->stream);
+    TRY(check_launch(ctx, "k_ploc_init"));
+    TRY(clk.mark("PLOC Init"));
+    int slot = 0;
+    TRY(ploc_drive(ctx, a, &slot, n, n, 2048, 1024, nullptr));
+    TRY(clk.mark("PLOC Global Iterations"));
+    launch_ploc_tail(a, slot, ctx->stream);
+    TRY(check_launch(ctx, "k_ploc_tail"));
+    TRY(clk.mark("PLOC LDS Tail"));
+    TRY(clk.finish(times));
     ctx->last_keys = keys;
     ctx->last_vals = nullptr;
     ctx->last_n = n;
-    if ((st = check_input_flag(ctx))) return st;
-    if (hs.flags & 2u) return set_err(ctx, BVH_E_STATE, "internal error: sorted keys name no primitive");
-    if (hs.flags & 4u) return set_err(ctx, BVH_E_STATE, "internal error: PLOC tail received too many clusters");
-    if (hs.flags & 1u)
-        return set_err(ctx, BVH_E_NOPROGRESS, "PLOC iteration %u made no merge (%u clusters left)", hs.iters, hs.m);
-    if (n > 1 && (hs.m != 1 || hs.node_base != 2 * n - 1))
-        return set_err(ctx, BVH_E_STATE, "PLOC ended with %u clusters and %u nodes", hs.m, hs.node_base);
+    TRY(check_input_flags(ctx, w, 1));
+    bool unfinished = false;
+    uint32_t iters = 0;
+    TRY(ploc_check(ctx, w, j, std::vector<uint32_t>{n}, &unfinished, &iters));
+    if (unfinished) return set_err(ctx, BVH_E_STATE, "internal error: PLOC tail ran before the mesh was small");
+    if (times) times->ploc_iterations = (int32_t)iters;
     return BVH_OK;
+}
+
+// ---- segmented jobs and batches ----------------------------------------------------
+
+// Refreshes a segment table (tree storage may have moved) and uploads it.
+int seg_tables(bvh_ctx_t ctx, SegTables& t, const bvh_mesh_t* meshes, const bvh_tree_t* trees, uint32_t k,
+               bool force) {
+    std::vector<SegDesc> segs(k);
+    uint32_t start = 0;
+    for (uint32_t s = 0; s < k; ++s) {
+        TRY(ensure_tree(ctx, trees[s], meshes[s]->n));
+        segs[s] = seg_of(meshes[s], trees[s], start);
+        start += meshes[s]->n;
+    }
+    bool same = !force && t.host_segs.size() == segs.size() &&
+                std::memcmp(t.host_segs.data(), segs.data(), segs.size() * sizeof(SegDesc)) == 0;
+    if (same) return BVH_OK;
+    t.host_segs = segs;
+    TRY(ensure(ctx, t.segs, segs.size() * sizeof(SegDesc)));
+    HIP_TRY(ctx, hipMemcpyAsync(t.segs.p, segs.data(), segs.size() * sizeof(SegDesc), hipMemcpyHostToDevice,
+                                ctx->stream));
+    HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+    return BVH_OK;
+}
+
+// Job of k meshes: block tables, segment table and key format.
+int plan_job(bvh_ctx_t ctx, SegTables& t, const bvh_mesh_t* meshes, const bvh_tree_t* trees, uint32_t k, Job& j) {
+    uint64_t total = 0;
+    for (uint32_t s = 0; s < k; ++s) {
+        if (!meshes[s] || !trees[s]) return set_err(ctx, BVH_E_ARG, "mesh or tree %u is null", s);
+        total += meshes[s]->n;
+    }
+    if (total >= (1ull << 31))
+        return set_err(ctx, BVH_E_UNSUPPORTED, "batch of %llu triangles (limit 2^31 - 1)", (unsigned long long)total);
+    std::vector<uint2> fb, hb;
+    const uint32_t hc = (uint32_t)hier_chunk();
+    j.total = 0;
+    j.maxn = 0;
+    for (uint32_t s = 0; s < k; ++s) {
+        const uint32_t n = meshes[s]->n;
+        for (uint32_t c = 0; c < cdiv(n, kFrontChunk); ++c) fb.push_back(make_uint2(s, c));
+        for (uint32_t c = 0; c < cdiv(n, hc); ++c) hb.push_back(make_uint2(s, c));
+        j.total += n;
+        j.maxn = std::max(j.maxn, n);
+    }
+    const int idx_bits = std::max(1, bit_width(j.maxn - 1));
+    const int seg_bits = bit_width(k - 1);
+    if (idx_bits + 30 + seg_bits > 64)
+        return set_err(ctx, BVH_E_UNSUPPORTED, "batch keys need %d bits (> 64): too many or too large meshes",
+                       idx_bits + 30 + seg_bits);
+    TRY(ensure(ctx, t.fblocks, fb.size() * sizeof(uint2)));
+    TRY(ensure(ctx, t.hblocks, hb.size() * sizeof(uint2)));
+    HIP_TRY(ctx, hipMemcpyAsync(t.fblocks.p, fb.data(), fb.size() * sizeof(uint2), hipMemcpyHostToDevice, ctx->stream));
+    HIP_TRY(ctx, hipMemcpyAsync(t.hblocks.p, hb.data(), hb.size() * sizeof(uint2), hipMemcpyHostToDevice, ctx->stream));
+    HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+    TRY(seg_tables(ctx, t, meshes, trees, k, true));
+    j.k = k;
+    j.m60 = false;
+    j.fsb.segs = P<SegDesc>(t.segs);
+    j.fsb.blocks = P<uint2>(t.fblocks);
+    std::memset(&j.fsb.single, 0, sizeof(j.fsb.single));
+    j.fsb.nblocks = (uint32_t)fb.size();
+    j.hsb = j.fsb;
+    j.hsb.blocks = P<uint2>(t.hblocks);
+    j.hsb.nblocks = (uint32_t)hb.size();
+    j.kf = KeyFormat{(uint32_t)idx_bits, (uint32_t)(idx_bits + 30)};
+    j.begin_bit = idx_bits;
+    j.end_bit = idx_bits + 30 + seg_bits;
+    return BVH_OK;
+}
+
+std::vector<uint32_t> mesh_sizes(const bvh_mesh_t* meshes, uint32_t k) {
+    std::vector<uint32_t> ns(k);
+    for (uint32_t s = 0; s < k; ++s) ns[s] = meshes[s]->n;
+    return ns;
+}
+
+// Eager segmented build on the context stream (same kernels as a batch graph).
+int build_batched_eager(bvh_ctx_t ctx, const bvh_mesh_t* meshes, const bvh_tree_t* trees, uint32_t k,
+                        const bvh_build_params* prm, bvh_stage_times* times) {
+    Job j;
+    TRY(plan_job(ctx, ctx->tabs, meshes, trees, k, j));
+    SortPlan plan = plan_for(j, prm->radix_bits);
+    if (plan.passes > 8) return set_err(ctx, BVH_E_UNSUPPORTED, "batch keys need more than 8 sort passes");
+    Workspace& w = ctx->ws;
+    TRY(reserve_front(ctx, w, j, plan));
+    if (prm->algo == BVH_ALGO_PLOC) TRY(reserve_ploc(ctx, w, j));
+    StageClock clk{ctx};
+    TRY(clk.begin());
+    uint64_t* keys = nullptr;
+    uint32_t* vals = nullptr;
+    TRY(enqueue_front(ctx, w, j, plan, &clk, &keys, &vals));
+    ctx->last_keys = nullptr;
+    if (prm->algo == BVH_ALGO_LBVH) {
+        launch_hierarchy(hier_args(w, j, keys, nullptr, false), ctx->stream);
+        TRY(check_launch(ctx, "k_hierarchy"));
+        TRY(clk.mark("Topology+Refit"));
+        TRY(clk.finish(times));
+        return check_input_flags(ctx, w, k);
+    }
+    const PlocArgs a = ploc_args(w, j, prm->ploc_radius);
+    launch_ploc_init(a, keys, j.kf, P<float4>(w.tri_boxes), P<uint32_t>(w.scene) + 7, ctx->stream);
+    TRY(check_launch(ctx, "k_ploc_init"));
+    TRY(clk.mark("PLOC Init"));
+    int slot = 0;
+    TRY(ploc_drive(ctx, a, &slot, j.total, j.maxn, 2048, 1024, nullptr));
+    TRY(clk.mark("PLOC Global Iterations"));
+    launch_ploc_tail(a, slot, ctx->stream);
+    TRY(check_launch(ctx, "k_ploc_tail"));
+    TRY(clk.mark("PLOC LDS Tail"));
+    TRY(clk.finish(times));
+    TRY(check_input_flags(ctx, w, k));
+    bool unfinished = false;
+    uint32_t iters = 0;
+    TRY(ploc_check(ctx, w, j, mesh_sizes(meshes, k), &unfinished, &iters));
+    if (unfinished) return set_err(ctx, BVH_E_STATE, "internal error: PLOC tail ran before every mesh was small");
+    if (times) times->ploc_iterations = (int32_t)iters;
+    return BVH_OK;
+}
+
+int batch_tables(bvh_batch_t b, bool force) {
+    return seg_tables(b->ctx, b->tabs, b->meshes.data(), b->trees.data(), b->k, force);
+}
+
+int batch_plan(bvh_batch_t b) { return plan_job(b->ctx, b->tabs, b->meshes.data(), b->trees.data(), b->k, b->job); }
+
+void batch_destroy_graph(bvh_batch_t b) {
+    if (b->exec) (void)hipGraphExecDestroy(b->exec);
+    if (b->graph) (void)hipGraphDestroy(b->graph);
+    b->exec = nullptr;
+    b->graph = nullptr;
+}
+
+// Enqueues the whole batch (captured into the graph).
+int batch_enqueue(bvh_batch_t b, const SortPlan& plan) {
+    bvh_ctx_t ctx = b->ctx;
+    Workspace& w = b->ws;
+    const Job& j = b->job;
+    uint64_t* keys = nullptr;
+    uint32_t* vals = nullptr;
+    TRY(enqueue_front(ctx, w, j, plan, nullptr, &keys, &vals));
+    if (b->algo == BVH_ALGO_LBVH) {
+        launch_hierarchy(hier_args(w, j, keys, nullptr, false), ctx->stream);
+        return check_launch(ctx, "k_hierarchy");
+    }
+    const PlocArgs a = ploc_args(w, j, b->radius);
+    launch_ploc_init(a, keys, j.kf, P<float4>(w.tri_boxes), P<uint32_t>(w.scene) + 7, ctx->stream);
+    for (int k = 0; k < b->ploc_iters; ++k) {
+        launch_ploc_nn(a, k % kPlocMaxIter, std::min(j.total, b->nn_grid * (uint32_t)ploc_nn_block()), ctx->stream);
+        launch_ploc_merge(a, k % kPlocMaxIter, std::min(j.total, b->merge_grid * (uint32_t)ploc_merge_tile()),
+                          ctx->stream);
+    }
+    launch_ploc_tail(a, b->ploc_iters % kPlocMaxIter, ctx->stream);
+    return check_launch(ctx, "k_ploc");
+}
+
+int batch_capture(bvh_batch_t b) {
+    bvh_ctx_t ctx = b->ctx;
+    batch_destroy_graph(b);
+    SortPlan plan = make_sort_plan(b->job.total, b->job.begin_bit, b->job.end_bit, b->radix_bits,
+                                   (int)b->job.fsb.nblocks);
+    if (plan.passes > 8) return set_err(ctx, BVH_E_UNSUPPORTED, "batch keys need more than 8 sort passes");
+    TRY(reserve_front(ctx, b->ws, b->job, plan));
+    if (b->algo == BVH_ALGO_PLOC) TRY(reserve_ploc(ctx, b->ws, b->job));
+    HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+    HIP_TRY(ctx, hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+    int st = batch_enqueue(b, plan);
+    hipGraph_t g = nullptr;
+    hipError_t e = hipStreamEndCapture(ctx->stream, &g);
+    if (st) {
+        if (g) (void)hipGraphDestroy(g);
+        return st;
+    }
+    if (e != hipSuccess) return set_err(ctx, BVH_E_HIP, "hipStreamEndCapture failed: %s", hipGetErrorString(e));
+    b->graph = g;
+    HIP_TRY(ctx, hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0));
+    size_t nodes = 0;
+    HIP_TRY(ctx, hipGraphGetNodes(b->graph, nullptr, &nodes));
+    b->info.graph_nodes = (int32_t)nodes;
+    b->info.radix_bits = plan.bits;
+    b->info.sort_passes = plan.passes;
+    b->info.ploc_captured_iterations = b->algo == BVH_ALGO_PLOC ? b->ploc_iters : 0;
+    b->info.ploc_nn_grid = (int32_t)b->nn_grid;
+    b->info.ploc_merge_grid = (int32_t)b->merge_grid;
+    return BVH_OK;
+}
+
+int batch_launch(bvh_batch_t b, float* ms) {
+    bvh_ctx_t ctx = b->ctx;
+    HIP_TRY(ctx, hipSetDevice(ctx->device));
+    TRY(batch_tables(b, false));
+    HIP_TRY(ctx, hipEventRecord(ctx->ev[0], ctx->stream));
+    HIP_TRY(ctx, hipGraphLaunch(b->exec, ctx->stream));
+    HIP_TRY(ctx, hipEventRecord(ctx->ev[1], ctx->stream));
+    HIP_TRY(ctx, hipEventSynchronize(ctx->ev[1]));
+    float t = 0.0f;
+    HIP_TRY(ctx, hipEventElapsedTime(&t, ctx->ev[0], ctx->ev[1]));
+    TRY(check_input_flags(ctx, b->ws, b->k));
+    b->info.runs++;
+    if (b->algo == BVH_ALGO_PLOC) {
+        std::vector<uint32_t> ns(b->k);
+        for (uint32_t s = 0; s < b->k; ++s) ns[s] = b->meshes[s]->n;
+        const PlocArgs a = ploc_args(b->ws, b->job, b->radius);
+        bool unfinished = false;
+        TRY(ploc_check(ctx, b->ws, b->job, ns, &unfinished, nullptr));
+        PlocState hs;
+        HIP_TRY(ctx, hipMemcpyAsync(&hs, a.state + b->ploc_iters % kPlocMaxIter, sizeof(hs), hipMemcpyDeviceToHost,
+                                    ctx->stream));
+        HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+        if (unfinished) {
+            // The graph ran out of captured iterations: continue eagerly.
+            b->info.eager_continuations++;
+            HIP_TRY(ctx, hipEventRecord(ctx->ev[0], ctx->stream));
+            int slot = b->ploc_iters % kPlocMaxIter;
+            TRY(ploc_drive(ctx, a, &slot, hs.m, hs.maxm, b->nn_grid, b->merge_grid, &hs));
+            launch_ploc_tail(a, slot, ctx->stream);
+            TRY(check_launch(ctx, "k_ploc_tail"));
+            HIP_TRY(ctx, hipEventRecord(ctx->ev[1], ctx->stream));
+            HIP_TRY(ctx, hipEventSynchronize(ctx->ev[1]));
+            float t2 = 0.0f;
+            HIP_TRY(ctx, hipEventElapsedTime(&t2, ctx->ev[0], ctx->ev[1]));
+            t += t2;
+            TRY(ploc_check(ctx, b->ws, b->job, ns, &unfinished, nullptr));
+            if (unfinished) return set_err(ctx, BVH_E_STATE, "internal error: PLOC batch did not finish");
+        }
+        b->info.ploc_iterations = (int32_t)hs.iters;
+    }
+    if (ms) *ms = t;
+    return BVH_OK;
+}
+
+// FNV-1a over every tree of the batch (tuning must not change any bit).
+int batch_digest(bvh_batch_t b, uint64_t* h) {
+    bvh_ctx_t ctx = b->ctx;
+    uint64_t x = 1469598103934665603ull;
+    std::vector<unsigned char> buf;
+    for (uint32_t s = 0; s < b->k; ++s) {
+        bvh_tree_t t = b->trees[s];
+        const size_t nb = (size_t)t->num_nodes * sizeof(DNode), ib = (size_t)t->num_prims * 4;
+        buf.resize(nb + ib);
+        HIP_TRY(ctx, hipMemcpyAsync(buf.data(), t->nodes.p, nb, hipMemcpyDeviceToHost, ctx->stream));
+        HIP_TRY(ctx, hipMemcpyAsync(buf.data() + nb, t->indices.p, ib, hipMemcpyDeviceToHost, ctx->stream));
+        HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+        for (unsigned char c : buf) {
+            x ^= c;
+            x *= 1099511628211ull;
+        }
+    }
+    *h = x;
+    return BVH_OK;
+}
+
+// Times candidate configurations (digit width x persistent grid of the
+// neighbour search) and keeps the fastest; every candidate must produce the
+// same trees bit for bit.
+int batch_tune(bvh_batch_t b) {
+    struct Cand {
+        int bits;
+        uint32_t nn_grid;
+    };
+    std::vector<Cand> cands;
+    for (int bits : {8, 10}) {
+        if (b->algo == BVH_ALGO_PLOC)
+            for (uint32_t g : {1024u, 2048u}) cands.push_back({bits, g});
+        else
+            cands.push_back({bits, b->nn_grid});
+    }
+    float best = 1e30f;
+    Cand pick = cands[0];
+    uint64_t ref = 0;
+    int count = 0;
+    for (const Cand& c : cands) {
+        b->radix_bits = c.bits;
+        b->nn_grid = c.nn_grid;
+        TRY(batch_capture(b));
+        float ms[3];
+        for (float& m : ms) TRY(batch_launch(b, &m));
+        std::sort(ms, ms + 3);
+        uint64_t h = 0;
+        TRY(batch_digest(b, &h));
+        if (count == 0) ref = h;
+        else if (h != ref)
+            return set_err(b->ctx, BVH_E_STATE, "internal error: tuned configurations disagree (digit width %d)",
+                           c.bits);
+        if (count < 8) {
+            b->info.tune_ms[count] = ms[1];
+            b->info.tune_radix_bits[count] = c.bits;
+            b->info.tune_nn_grid[count] = (int32_t)c.nn_grid;
+        }
+        ++count;
+        if (ms[1] < best) {
+            best = ms[1];
+            pick = c;
+        }
+    }
+    b->info.tune_count = count;
+    b->radix_bits = pick.bits;
+    b->nn_grid = pick.nn_grid;
+    return batch_capture(b);
 }
 
 }  // namespace
@@ -454,13 +923,11 @@ int bvh_ctx_destroy(bvh_ctx_t ctx) {
     if (!ctx) return BVH_E_ARG;
     (void)hipSetDevice(ctx->device);
     (void)hipStreamSynchronize(ctx->stream);
-    DevBuf* bufs[] = {&ctx->tri_boxes, &ctx->keys_a,    &ctx->keys_b, &ctx->vals_a,  &ctx->vals_b,  &ctx->hist_part,
-                      &ctx->bin_start, &ctx->lookback,  &ctx->scene,  &ctx->handoff, &ctx->parents, &ctx->ranges};
-    for (DevBuf* b : bufs) free_buf(*b);
-    DevBuf* pbufs[] = {&ctx->ploc_box[0], &ctx->ploc_box[1], &ctx->ploc_id[0], &ctx->ploc_id[1],
-                       &ctx->ploc_nn,     &ctx->ploc_state,  &ctx->ploc_status, &ctx->counters,
-                       &ctx->trace_ovf,   &ctx->trace_list,  &ctx->trace_io};
-    for (DevBuf* b : pbufs) free_buf(*b);
+    ctx->ws.each([](DevBuf& b) { free_buf(b); });
+    ctx->tabs.release();
+    free_buf(ctx->trace_ovf);
+    free_buf(ctx->trace_list);
+    free_buf(ctx->trace_io);
     for (auto& e : ctx->ev)
         if (e) (void)hipEventDestroy(e);
     (void)hipStreamDestroy(ctx->stream);
@@ -469,6 +936,16 @@ int bvh_ctx_destroy(bvh_ctx_t ctx) {
 }
 
 const char* bvh_last_error(bvh_ctx_t ctx) { return ctx ? ctx->err.c_str() : "null context"; }
+
+int bvh_device_memory(bvh_ctx_t ctx, uint64_t* free_bytes, uint64_t* total_bytes) {
+    if (!ctx || !free_bytes || !total_bytes) return BVH_E_ARG;
+    HIP_TRY(ctx, hipSetDevice(ctx->device));
+    size_t f = 0, t = 0;
+    HIP_TRY(ctx, hipMemGetInfo(&f, &t));
+    *free_bytes = f;
+    *total_bytes = t;
+    return BVH_OK;
+}
 
 int bvh_device_alloc(bvh_ctx_t ctx, size_t bytes, void** dptr) {
     if (!ctx || !dptr) return BVH_E_ARG;
@@ -617,20 +1094,22 @@ int bvh_build(bvh_ctx_t ctx, bvh_mesh_t mesh, const bvh_build_params* params, bv
 
 int bvh_debug_morton(bvh_ctx_t ctx, bvh_mesh_t mesh, int32_t algo, uint64_t* codes) {
     if (!ctx || !mesh || !codes) return BVH_E_ARG;
-    const MeshView m = mesh->view();
-    int st;
-    if ((st = ensure(ctx, ctx->tri_boxes, (size_t)m.n * 32))) return st;
-    if ((st = ensure(ctx, ctx->keys_a, (size_t)m.n * 8))) return st;
-    if ((st = ensure(ctx, ctx->scene, 256))) return st;
-    launch_init_scene(P<uint32_t>(ctx->scene), ctx->stream);
-    launch_tri_bounds(m, P<float4>(ctx->tri_boxes), P<uint32_t>(ctx->scene), ctx->stream);
-    launch_morton_codes(P<float4>(ctx->tri_boxes), m.n, P<uint32_t>(ctx->scene), algo == BVH_ALGO_LBVH_PLUS,
-                        P<uint64_t>(ctx->keys_a), ctx->stream);
-    if ((st = check_launch(ctx, "k_morton_codes"))) return st;
-    HIP_TRY(ctx, hipMemcpyAsync(codes, ctx->keys_a.p, (size_t)m.n * 8, hipMemcpyDeviceToHost, ctx->stream));
+    HIP_TRY(ctx, hipSetDevice(ctx->device));
+    const Job j = single_job(mesh, nullptr, algo == BVH_ALGO_LBVH_PLUS);
+    Workspace& w = ctx->ws;
+    TRY(ensure(ctx, w.tri_boxes, (size_t)j.total * 32));
+    TRY(ensure(ctx, w.keys_a, (size_t)j.total * 8));
+    TRY(ensure(ctx, w.scene, 32));
+    TRY(ensure(ctx, w.epoch, 16, true));
+    launch_init_build(P<uint32_t>(w.scene), 1, P<uint32_t>(w.epoch), ctx->stream);
+    launch_tri_bounds(j.fsb, P<float4>(w.tri_boxes), P<uint32_t>(w.scene), ctx->stream);
+    launch_morton_codes(j.fsb, P<float4>(w.tri_boxes), P<uint32_t>(w.scene), j.m60, P<uint64_t>(w.keys_a),
+                        ctx->stream);
+    TRY(check_launch(ctx, "k_morton_codes"));
+    HIP_TRY(ctx, hipMemcpyAsync(codes, w.keys_a.p, (size_t)j.total * 8, hipMemcpyDeviceToHost, ctx->stream));
     HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
     ctx->last_keys = nullptr;
-    return BVH_OK;
+    return check_input_flags(ctx, w, 1);
 }
 
 int bvh_debug_sorted_keys(bvh_ctx_t ctx, uint64_t* keys, uint32_t* vals, uint32_t n) {
@@ -640,6 +1119,94 @@ int bvh_debug_sorted_keys(bvh_ctx_t ctx, uint64_t* keys, uint32_t* vals, uint32_
     if (vals && ctx->last_vals)
         HIP_TRY(ctx, hipMemcpyAsync(vals, ctx->last_vals, (size_t)n * 4, hipMemcpyDeviceToHost, ctx->stream));
     HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+    return BVH_OK;
+}
+
+int bvh_batch_create(bvh_ctx_t ctx, const bvh_mesh_t* meshes, const bvh_tree_t* trees, uint32_t k,
+                     const bvh_build_params* params, bvh_batch_t* out) {
+    if (!ctx || !meshes || !trees || !params || !out || k == 0)
+        return ctx ? set_err(ctx, BVH_E_ARG, "null argument or empty batch") : BVH_E_ARG;
+    *out = nullptr;
+    if (params->algo != BVH_ALGO_LBVH && params->algo != BVH_ALGO_PLOC)
+        return set_err(ctx, BVH_E_UNSUPPORTED, "batches build LBVH or PLOC (algorithm %d requested)", params->algo);
+    if (params->algo == BVH_ALGO_PLOC && (params->ploc_radius < 1 || params->ploc_radius > 1024))
+        return set_err(ctx, BVH_E_ARG, "PLOC radius %d out of range [1, 1024]", params->ploc_radius);
+    if (params->radix_bits != 0 && params->radix_bits != 8 && params->radix_bits != 10)
+        return set_err(ctx, BVH_E_UNSUPPORTED, "radix_bits must be 0, 8 or 10");
+    for (uint32_t s = 0; s < k; ++s)
+        if (!meshes[s] || !trees[s]) return set_err(ctx, BVH_E_ARG, "mesh or tree %u is null", s);
+    HIP_TRY(ctx, hipSetDevice(ctx->device));
+    bvh_batch_rec* b = new bvh_batch_rec();
+    std::memset(&b->info, 0, sizeof(b->info));
+    b->ctx = ctx;
+    b->algo = params->algo;
+    b->radius = params->ploc_radius;
+    b->k = k;
+    b->meshes.assign(meshes, meshes + k);
+    b->trees.assign(trees, trees + k);
+    b->radix_bits = pick_bits(params->radix_bits);
+    b->nn_grid = 2048;
+    b->merge_grid = 1024;
+    auto fail = [&](int st) {
+        bvh_batch_free(b);
+        return st;
+    };
+    int st = batch_plan(b);
+    if (st) return fail(st);
+    b->ploc_iters = b->algo == BVH_ALGO_PLOC ? std::min(ploc_iter_estimate(b->job.maxn, 0.85) + 2, kPlocCaptureMax) : 0;
+    if ((st = batch_capture(b))) return fail(st);
+    if (b->algo == BVH_ALGO_PLOC) {
+        // One run finds the exact number of global iterations these meshes need.
+        if ((st = batch_launch(b, nullptr))) return fail(st);
+        b->ploc_iters = std::min(b->info.ploc_iterations, kPlocCaptureMax);
+        if ((st = batch_capture(b))) return fail(st);
+    }
+    if ((params->flags & BVH_BUILD_TUNE) && (st = batch_tune(b))) return fail(st);
+    b->info.runs = 0;  // counters describe the caller's runs
+    b->info.eager_continuations = 0;
+    b->info.meshes = k;
+    b->info.triangles = b->job.total;
+    b->info.max_triangles = b->job.maxn;
+    b->info.algo = b->algo;
+    b->info.key_bits = b->job.end_bit - b->job.begin_bit;
+    *out = b;
+    return BVH_OK;
+}
+
+int bvh_build_batched(bvh_ctx_t ctx, const bvh_mesh_t* meshes, const bvh_tree_t* trees, uint32_t k,
+                      const bvh_build_params* params, bvh_stage_times* times) {
+    if (!ctx || !meshes || !trees || !params || k == 0)
+        return ctx ? set_err(ctx, BVH_E_ARG, "null argument or empty batch") : BVH_E_ARG;
+    if (params->algo != BVH_ALGO_LBVH && params->algo != BVH_ALGO_PLOC)
+        return set_err(ctx, BVH_E_UNSUPPORTED, "batches build LBVH or PLOC (algorithm %d requested)", params->algo);
+    if (params->algo == BVH_ALGO_PLOC && (params->ploc_radius < 1 || params->ploc_radius > 1024))
+        return set_err(ctx, BVH_E_ARG, "PLOC radius %d out of range [1, 1024]", params->ploc_radius);
+    if (params->radix_bits != 0 && params->radix_bits != 8 && params->radix_bits != 10)
+        return set_err(ctx, BVH_E_UNSUPPORTED, "radix_bits must be 0, 8 or 10");
+    HIP_TRY(ctx, hipSetDevice(ctx->device));
+    return build_batched_eager(ctx, meshes, trees, k, params, times);
+}
+
+int bvh_batch_run(bvh_batch_t batch, float* ms) {
+    if (!batch || !batch->exec) return BVH_E_ARG;
+    return batch_launch(batch, ms);
+}
+
+int bvh_batch_get_info(bvh_batch_t batch, bvh_batch_info* info) {
+    if (!batch || !info) return BVH_E_ARG;
+    *info = batch->info;
+    return BVH_OK;
+}
+
+int bvh_batch_free(bvh_batch_t batch) {
+    if (!batch) return BVH_E_ARG;
+    bvh_ctx_t ctx = batch->ctx;
+    (void)hipSetDevice(ctx->device);
+    (void)hipStreamSynchronize(ctx->stream);
+    batch_destroy_graph(batch);
+    batch->ws.each([](DevBuf& b) { free_buf(b); });
+    batch->tabs.release();
+    delete batch;
     return BVH_OK;
 }
 
@@ -720,26 +1287,26 @@ int bvh_sort_pairs_u64(bvh_ctx_t ctx, uint64_t* d_keys, uint32_t* d_vals, uint32
     if (begin_bit < 0 || end_bit > 64 || begin_bit >= end_bit) return set_err(ctx, BVH_E_ARG, "bad bit range");
     if (radix_bits != 0 && radix_bits != 8 && radix_bits != 10)
         return set_err(ctx, BVH_E_UNSUPPORTED, "radix_bits must be 0, 8 or 10");
-    const int bits = pick_bits(radix_bits, end_bit - begin_bit);
-    SortPlan plan = make_sort_plan(n, begin_bit, end_bit, bits);
+    SortPlan plan = make_sort_plan(n, begin_bit, end_bit, pick_bits(radix_bits));
     if (plan.passes > 8) return set_err(ctx, BVH_E_ARG, "too many passes");
     const size_t bins = 1u << plan.bits;
-    const size_t zero_bytes = (sort_zero_bytes(plan) + 15) & ~(size_t)15;
-    int st;
-    if ((st = ensure(ctx, ctx->keys_b, (size_t)n * 8))) return st;
-    if (d_vals && (st = ensure(ctx, ctx->vals_b, (size_t)n * 4))) return st;
-    if ((st = ensure(ctx, ctx->hist_part, (size_t)plan.hist_blocks * plan.passes * bins * 4))) return st;
-    if ((st = ensure(ctx, ctx->bin_start, (size_t)plan.passes * bins * 4))) return st;
-    if ((st = ensure(ctx, ctx->lookback, zero_bytes))) return st;
+    const size_t zero_bytes = zero_bytes_of(plan);
+    Workspace& w = ctx->ws;
+    HIP_TRY(ctx, hipSetDevice(ctx->device));
+    TRY(ensure(ctx, w.keys_b, (size_t)n * 8));
+    if (d_vals) TRY(ensure(ctx, w.vals_b, (size_t)n * 4));
+    TRY(ensure(ctx, w.hist_part, (size_t)plan.hist_blocks * plan.passes * bins * 4));
+    TRY(ensure(ctx, w.bin_start, (size_t)plan.passes * bins * 4));
+    TRY(ensure(ctx, w.lookback, zero_bytes));
     HIP_TRY(ctx, hipEventRecord(ctx->ev[0], ctx->stream));
-    launch_sort_histogram(d_keys, n, plan, P<uint32_t>(ctx->hist_part), ctx->stream);
-    launch_hist_scan(P<uint32_t>(ctx->hist_part), plan, P<uint32_t>(ctx->bin_start), ctx->stream);
-    launch_fill_zero(P<uint32_t>(ctx->lookback), zero_bytes / 4, ctx->stream);
+    launch_sort_histogram(d_keys, n, plan, P<uint32_t>(w.hist_part), ctx->stream);
+    launch_hist_scan(P<uint32_t>(w.hist_part), plan, P<uint32_t>(w.bin_start), ctx->stream);
+    launch_fill_zero(P<uint32_t>(w.lookback), zero_bytes / 4, ctx->stream);
     uint64_t* kout = nullptr;
     uint32_t* vout = nullptr;
-    launch_onesweep(plan, n, d_keys, P<uint64_t>(ctx->keys_b), d_vals, d_vals ? P<uint32_t>(ctx->vals_b) : nullptr,
-                    P<uint32_t>(ctx->bin_start), P<uint32_t>(ctx->lookback), &kout, &vout, ctx->stream);
-    if ((st = check_launch(ctx, "k_onesweep"))) return st;
+    launch_onesweep(plan, n, d_keys, P<uint64_t>(w.keys_b), d_vals, d_vals ? P<uint32_t>(w.vals_b) : nullptr,
+                    P<uint32_t>(w.bin_start), P<uint32_t>(w.lookback), &kout, &vout, ctx->stream);
+    TRY(check_launch(ctx, "k_onesweep"));
     HIP_TRY(ctx, hipEventRecord(ctx->ev[1], ctx->stream));
     if (kout != d_keys) {
         HIP_TRY(ctx, hipMemcpyAsync(d_keys, kout, (size_t)n * 8, hipMemcpyDeviceToDevice, ctx->stream));

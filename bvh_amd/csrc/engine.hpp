@@ -16,18 +16,44 @@ struct MeshView {
     uint32_t n;
 };
 
-// ---- LBVH/LBVH+/PLOC front end (k_front.hip) ----------------------------
-// Kernel geometry for the bounds and Morton passes.
-int front_grid(uint32_t n);
+// ---- segments ----------------------------------------------------------------
+// One mesh of a build.  A standalone build is a batch of one mesh.
+struct SegDesc {
+    const float* v[9];   // vertex SoA arrays
+    uint32_t n;          // triangles
+    uint32_t start;      // first slot of this mesh in the batch-wide per-triangle arrays
+    DNode* nodes;        // output: 2n-1 nodes
+    uint32_t* indices;   // output: n primitive indices
+};
 
-// Resets the six order-mapped scene-bound words.
-void launch_init_scene(uint32_t* scene_ord, hipStream_t s);
-// Per-triangle boxes (2 x float4 per triangle) and the scene box reduction.
-void launch_tri_bounds(const MeshView& m, float4* tri_boxes, uint32_t* scene_ord, hipStream_t s);
+// How a kernel's blocks map onto segments: block b covers chunk blocks[b].y of
+// segment blocks[b].x; with blocks == nullptr, one segment (`single`) and
+// chunk = blockIdx.x.
+struct SegBlocks {
+    const SegDesc* segs;
+    const uint2* blocks;
+    SegDesc single;
+    uint32_t nblocks;
+};
+
+// Packed 64-bit sort key of LBVH/PLOC: seg << seg_shift | morton30 << idx_bits | index.
+// A single mesh uses idx_bits = 32 (key = morton30 << 32 | index).
+struct KeyFormat {
+    uint32_t idx_bits;
+    uint32_t seg_shift;
+};
+
+// ---- LBVH/LBVH+/PLOC front end (k_front.hip) ----------------------------
+constexpr uint32_t kFrontChunk = 4096;   // triangles per block of the front kernels
+
+// Resets the per-segment scene words (8 per segment) and advances *epoch.
+void launch_init_build(uint32_t* seg_scene, uint32_t k, uint32_t* epoch, hipStream_t s);
+// Per-triangle boxes (2 x float4 per triangle) and each segment's scene box.
+void launch_tri_bounds(const SegBlocks& sb, float4* tri_boxes, uint32_t* seg_scene, hipStream_t s);
 
 // Sort plan: digit width, pass count and shifts for a key bit range.
 struct SortPlan {
-    int bits;          // digit width (8, 10 or 11)
+    int bits;          // digit width (8 or 10)
     int passes;        // number of LSD passes
     int shift[8];      // low bit of each pass
     uint32_t mask[8];  // digit mask of each pass (the last pass may be narrower)
@@ -35,18 +61,20 @@ struct SortPlan {
     uint32_t tile;     // keys per tile
     int hist_blocks;   // blocks of the histogram producer
 };
-SortPlan make_sort_plan(uint32_t n, int begin_bit, int end_bit, int bits);
+// hist_blocks <= 0 picks a default producer size.
+SortPlan make_sort_plan(uint32_t n, int begin_bit, int end_bit, int bits, int hist_blocks = 0);
 // Bytes of the zero-initialised lookback area a plan needs (status words + tile counters).
 size_t sort_zero_bytes(const SortPlan& p);
 
-// Morton keys + fused all-pass digit histograms + zeroing of `zero_words` words.
-//   m60 = false: keys = morton30 << 32 | i (vals unused)
-//   m60 = true : keys = morton60, vals = i
-void launch_morton(const float4* tri_boxes, uint32_t n, const uint32_t* scene_ord, bool m60, uint64_t* keys,
-                   uint32_t* vals, const SortPlan& plan, uint32_t* hist_part, uint32_t* zero_ptr,
-                   size_t zero_words, hipStream_t s);
-// Debug: Morton codes only (no histograms).
-void launch_morton_codes(const float4* tri_boxes, uint32_t n, const uint32_t* scene_ord, bool m60,
+// Morton keys + fused all-pass digit histograms (one row per block of sb) +
+// zeroing of `zero_words` words.
+//   m60 = false: keys = packed per KeyFormat (vals unused)
+//   m60 = true : keys = morton60, vals = index (single segment only)
+void launch_morton(const SegBlocks& sb, const float4* tri_boxes, const uint32_t* seg_scene, bool m60,
+                   const KeyFormat& kf, uint64_t* keys, uint32_t* vals, const SortPlan& plan, uint32_t* hist_part,
+                   uint32_t* zero_ptr, size_t zero_words, hipStream_t s);
+// Debug: Morton codes only (no histograms), in input order.
+void launch_morton_codes(const SegBlocks& sb, const float4* tri_boxes, const uint32_t* seg_scene, bool m60,
                          uint64_t* codes, hipStream_t s);
 
 // ---- radix sort (k_sort.hip) ---------------------------------------------
@@ -69,16 +97,15 @@ void launch_onesweep(const SortPlan& plan, uint32_t n, uint64_t* keys_a, uint64_
 
 // ---- hierarchy (k_hier.hip) ------------------------------------------------
 struct HierArgs {
-    const uint64_t* keys;   // sorted keys
-    const uint32_t* vals;   // sorted indices (60-bit keys) or nullptr (packed 30-bit keys)
-    uint32_t n;
+    SegBlocks sb;           // blocks over chunks of hier_chunk() leaves
+    const uint64_t* keys;   // sorted keys (batch-wide)
+    const uint32_t* vals;   // sorted indices (60-bit keys) or nullptr (packed keys)
+    KeyFormat kf;
     const float4* tri_boxes;
-    DNode* nodes;           // 2n-1
-    uint32_t* indices;      // n
-    uint32_t* parents;      // optional, 2n-1
-    int2* ranges;           // optional, n-1 (leaf range of each internal node)
-    uint64_t* handoff;      // n words, epoch-tagged
-    uint32_t epoch;
+    uint32_t* parents;      // optional, 2n-1 (single segment)
+    int2* ranges;           // optional, n-1 (single segment)
+    uint64_t* handoff;      // batch-wide, epoch-tagged
+    const uint32_t* epoch;  // device word advanced by launch_init_build
     uint32_t* err;          // set to nonzero when a sorted key names no primitive
 };
 int hier_chunk();
@@ -114,39 +141,46 @@ struct TreeletArgs {
 void launch_treelet(const TreeletArgs& a, hipStream_t s);
 
 // ---- PLOC (k_ploc.hip) -------------------------------------------------------
-// Clusters left when one workgroup takes over all remaining iterations in LDS.
+// Clusters per mesh at or below which one workgroup finishes the mesh in LDS.
 constexpr int kPlocSmall = 2048;
 constexpr int kPlocMaxIter = 4096;
+constexpr uint32_t kPlocNoProgress = 1u;   // an iteration of a mesh with >= 2 clusters merged nothing
+constexpr uint32_t kPlocUnfinished = 8u;   // the tail ran before every mesh was small
 
 // Device-side loop state of one PLOC iteration (32 B).
 struct PlocState {
-    uint32_t m;          // live clusters entering the iteration
-    uint32_t node_base;  // id of the next internal node (before the root swap)
+    uint32_t m;          // clusters (all meshes) entering the iteration
+    uint32_t maxm;       // largest per-mesh cluster count entering the iteration
     uint32_t parity;     // which cluster buffer holds the live clusters
-    uint32_t iters;      // merge iterations executed so far
-    uint32_t flags;      // 1 = no progress, 2 = invalid key, 4 = tail overflow
+    uint32_t iters;      // global iterations executed so far
+    uint32_t flags;      // kPlocNoProgress
     uint32_t pad[3];
 };
 
 struct PlocArgs {
-    uint32_t n;
+    SegBlocks sb;       // meshes; blocks over kFrontChunk chunks (for the init kernel)
+    uint32_t nseg;      // meshes
     int radius;
+    uint32_t m0, maxm0; // total and largest triangle count
     float4* box[2];     // cluster boxes (2 x float4 each), double-buffered
-    uint32_t* id[2];    // cluster node ids, double-buffered
-    int32_t* nn;        // nearest neighbour per cluster
-    DNode* nodes;       // 2n-1
-    PlocState* state;   // kPlocMaxIter + 2 slots
+    uint32_t* id[2];    // cluster node ids (mesh-local), double-buffered
+    uint32_t* sid[2];   // cluster mesh index, double-buffered (nullptr for one mesh)
+    int32_t* nn;        // nearest neighbour per cluster (global slot)
+    uint4* seg[2];      // per mesh and parity: first cluster, count, next node id, iterations
+    PlocState* state;   // kPlocMaxIter slots
     uint64_t* status;   // look-back words, one per merge tile
     uint32_t* tile_ctr; // merge tile counter
-    int final_slot;     // state slot the tail kernel writes
+    uint4* out;         // per mesh, written by the tail: count, next node id, iterations, flags
 };
 int ploc_small();
 int ploc_nn_block();
 int ploc_merge_tile();
-void launch_ploc_init(const PlocArgs& a, const uint64_t* keys, const float4* tri_boxes, uint32_t* indices,
-                      hipStream_t s);
+void launch_ploc_init(const PlocArgs& a, const uint64_t* keys, const KeyFormat& kf, const float4* tri_boxes,
+                      uint32_t* err, hipStream_t s);
+// One global iteration (no-op once every mesh is small); m_bound caps the grid.
 void launch_ploc_nn(const PlocArgs& a, int k, uint32_t m_bound, hipStream_t s);
 void launch_ploc_merge(const PlocArgs& a, int k, uint32_t m_bound, hipStream_t s);
+// One workgroup per mesh; reads state slot k.
 void launch_ploc_tail(const PlocArgs& a, int k, hipStream_t s);
 
 }  // namespace bvhk

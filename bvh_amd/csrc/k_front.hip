@@ -1,5 +1,6 @@
-// k_front.hip — front end of every builder: per-triangle boxes, the scene box,
-// and Morton keys with all LSD digit histograms fused into the same pass.
+// k_front.hip — front end of every builder: per-triangle boxes, each mesh's
+// scene box, and Morton keys with all LSD digit histograms fused into the
+// same pass.
 //
 // Reference stages replaced:
 //   kComputeBoundsAndCentroids_nt/_P/_ploc  (lbvh_builder.cu:111-130,
@@ -14,6 +15,10 @@
 // fold into six order-mapped uint32 words with integer atomics, which is
 // exact and order-independent.  The Morton pass also counts every radix digit
 // of every sort pass in LDS, so the sort needs no separate histogram read.
+//
+// Every kernel works on segments (meshes): a block covers a chunk of one
+// mesh, described by a SegDesc.  A single build is a batch of one mesh, so
+// batched and standalone builds run the same per-element arithmetic.
 #include "dev_common.hpp"
 #include "engine.hpp"
 
@@ -21,44 +26,54 @@ namespace bvhk {
 
 constexpr int kFrontBlock = 256;
 
-int front_grid(uint32_t n) {
-    uint32_t per = kFrontBlock * 4;
-    uint32_t g = (n + per - 1) / per;
-    if (g < 1) g = 1;
-    if (g > 2048) g = 2048;
-    return (int)g;
+__device__ __forceinline__ void block_segment(const SegBlocks& sb, uint32_t& seg, uint32_t& chunk, SegDesc& d) {
+    if (sb.blocks) {
+        const uint2 e = sb.blocks[blockIdx.x];
+        seg = e.x;
+        chunk = e.y;
+        d = sb.segs[seg];
+    } else {
+        seg = 0;
+        chunk = blockIdx.x;
+        d = sb.single;
+    }
 }
 
-// Scene block: words 0-2 order-mapped min, 3-5 max, 6 non-finite flag.
-__global__ void k_init_scene(uint32_t* scene_ord) {
-    int t = threadIdx.x;
-    if (t < 3) scene_ord[t] = 0xFFFFFFFFu;      // min slots: start at +max
-    else if (t < 6) scene_ord[t] = 0u;           // max slots: start at -max
-    else if (t < 16) scene_ord[t] = 0u;
+// Per-segment scene block: words 0-2 order-mapped min, 3-5 max, 6 non-finite
+// flag, 7 internal-error flag.  Also advances the build epoch.
+__global__ void k_init_build(uint32_t* seg_scene, uint32_t k, uint32_t* epoch) {
+    const uint32_t stride = gridDim.x * blockDim.x;
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < k * 8; i += stride) {
+        const uint32_t w = i & 7;
+        seg_scene[i] = (w < 3) ? 0xFFFFFFFFu : 0u;
+    }
+    if (blockIdx.x == 0 && threadIdx.x == 0 && epoch) *epoch += 1u;
 }
 
-void launch_init_scene(uint32_t* scene_ord, hipStream_t s) {
-    hipLaunchKernelGGL(k_init_scene, dim3(1), dim3(64), 0, s, scene_ord);
+void launch_init_build(uint32_t* seg_scene, uint32_t k, uint32_t* epoch, hipStream_t s) {
+    uint32_t g = (k * 8 + 255) / 256;
+    hipLaunchKernelGGL(k_init_build, dim3(g ? g : 1), dim3(256), 0, s, seg_scene, k, epoch);
 }
-
-struct Soa9 {
-    const float* v[9];
-};
 
 __device__ __forceinline__ uint32_t nonfinite(float f) {
     return ((__float_as_uint(f) & 0x7f800000u) == 0x7f800000u) ? 1u : 0u;
 }
 
-__global__ __launch_bounds__(kFrontBlock) void k_tri_bounds(Soa9 m, uint32_t n, float4* __restrict__ tri_boxes,
-                                                            uint32_t* __restrict__ scene_ord) {
+__global__ __launch_bounds__(kFrontBlock) void k_tri_bounds(SegBlocks sb, float4* __restrict__ tri_boxes,
+                                                            uint32_t* __restrict__ seg_scene) {
+    uint32_t seg, chunk;
+    SegDesc d;
+    block_segment(sb, seg, chunk, d);
     uint32_t omn[3] = {0xFFFFFFFFu, 0xFFFFFFFFu, 0xFFFFFFFFu};
     uint32_t omx[3] = {0u, 0u, 0u};
     uint32_t bad = 0;
-    const uint32_t stride = gridDim.x * kFrontBlock;
-    for (uint32_t i = blockIdx.x * kFrontBlock + threadIdx.x; i < n; i += stride) {
-        float a0 = m.v[0][i], a1 = m.v[1][i], a2 = m.v[2][i];
-        float b0 = m.v[3][i], b1 = m.v[4][i], b2 = m.v[5][i];
-        float c0 = m.v[6][i], c1 = m.v[7][i], c2 = m.v[8][i];
+    const uint32_t lo = chunk * kFrontChunk;
+    const uint32_t hi = (lo + kFrontChunk < d.n) ? lo + kFrontChunk : d.n;
+    float4* __restrict__ out = tri_boxes + 2 * (size_t)d.start;
+    for (uint32_t i = lo + threadIdx.x; i < hi; i += kFrontBlock) {
+        float a0 = d.v[0][i], a1 = d.v[1][i], a2 = d.v[2][i];
+        float b0 = d.v[3][i], b1 = d.v[4][i], b2 = d.v[5][i];
+        float c0 = d.v[6][i], c1 = d.v[7][i], c2 = d.v[8][i];
         bad |= nonfinite(a0) | nonfinite(a1) | nonfinite(a2) | nonfinite(b0) | nonfinite(b1) | nonfinite(b2) |
                nonfinite(c0) | nonfinite(c1) | nonfinite(c2);
         Box b;
@@ -68,7 +83,7 @@ __global__ __launch_bounds__(kFrontBlock) void k_tri_bounds(Soa9 m, uint32_t n, 
         b.mxx = fmax_s(a0, fmax_s(b0, c0));
         b.mxy = fmax_s(a1, fmax_s(b1, c1));
         b.mxz = fmax_s(a2, fmax_s(b2, c2));
-        store_box(tri_boxes + 2 * (size_t)i, b);
+        store_box(out + 2 * (size_t)i, b);
         uint32_t o;
         o = f2ord(b.mnx); omn[0] = o < omn[0] ? o : omn[0];
         o = f2ord(b.mny); omn[1] = o < omn[1] ? o : omn[1];
@@ -86,6 +101,7 @@ __global__ __launch_bounds__(kFrontBlock) void k_tri_bounds(Soa9 m, uint32_t n, 
         if (lane == 0) { red[wave][k] = a; red[wave][3 + k] = b; }
     }
     __syncthreads();
+    uint32_t* sc = seg_scene + 8 * (size_t)seg;
     if (threadIdx.x < 6) {
         int k = threadIdx.x;
         uint32_t v = red[0][k];
@@ -93,20 +109,17 @@ __global__ __launch_bounds__(kFrontBlock) void k_tri_bounds(Soa9 m, uint32_t n, 
             uint32_t x = red[w][k];
             v = (k < 3) ? (x < v ? x : v) : (x > v ? x : v);
         }
-        if (k < 3) atomicMin(scene_ord + k, v);
-        else atomicMax(scene_ord + k, v);
+        if (k < 3) atomicMin(sc + k, v);
+        else atomicMax(sc + k, v);
     }
-    // Word 6 of the scene block is the non-finite-input flag.
-    if (__ballot(bad != 0) != 0 && (threadIdx.x % kWave) == 0) atomicOr(scene_ord + 6, 1u);
+    if (__ballot(bad != 0) != 0 && lane == 0) atomicOr(sc + 6, 1u);
 }
 
-void launch_tri_bounds(const MeshView& m, float4* tri_boxes, uint32_t* scene_ord, hipStream_t s) {
-    Soa9 soa;
-    for (int k = 0; k < 9; ++k) soa.v[k] = m.v[k];
-    hipLaunchKernelGGL(k_tri_bounds, dim3(front_grid(m.n)), dim3(kFrontBlock), 0, s, soa, m.n, tri_boxes, scene_ord);
+void launch_tri_bounds(const SegBlocks& sb, float4* tri_boxes, uint32_t* seg_scene, hipStream_t s) {
+    hipLaunchKernelGGL(k_tri_bounds, dim3(sb.nblocks), dim3(kFrontBlock), 0, s, sb, tri_boxes, seg_scene);
 }
 
-SortPlan make_sort_plan(uint32_t n, int begin_bit, int end_bit, int bits) {
+SortPlan make_sort_plan(uint32_t n, int begin_bit, int end_bit, int bits, int hist_blocks) {
     SortPlan p;
     p.bits = bits;
     int span = end_bit - begin_bit;
@@ -120,9 +133,12 @@ SortPlan make_sort_plan(uint32_t n, int begin_bit, int end_bit, int bits) {
     p.tile = 256 * 16;
     p.tiles = (n + p.tile - 1) / p.tile;
     if (p.tiles < 1) p.tiles = 1;
-    // Histogram producers: about 4K keys per block, at most 1024 blocks (4 per CU).
-    uint32_t hb = (n + 4095) / 4096;
-    p.hist_blocks = (int)(hb < 1 ? 1 : (hb > 1024 ? 1024 : hb));
+    if (hist_blocks > 0) {
+        p.hist_blocks = hist_blocks;
+    } else {
+        uint32_t hb = (n + 4095) / 4096;
+        p.hist_blocks = (int)(hb < 1 ? 1 : (hb > 1024 ? 1024 : hb));
+    }
     return p;
 }
 
@@ -139,40 +155,47 @@ struct Shifts {
 
 // Morton keys + per-block digit histograms for all passes + zeroing.
 template <int BITS, bool M60>
-__global__ __launch_bounds__(kFrontBlock) void k_morton(const float4* __restrict__ tri_boxes, uint32_t n,
-                                                        const uint32_t* __restrict__ scene_ord,
+__global__ __launch_bounds__(kFrontBlock) void k_morton(SegBlocks sb, const float4* __restrict__ tri_boxes,
+                                                        const uint32_t* __restrict__ seg_scene, KeyFormat kf,
                                                         uint64_t* __restrict__ keys, uint32_t* __restrict__ vals,
                                                         int passes, Shifts sh, uint32_t* __restrict__ hist_part,
                                                         uint4* __restrict__ zero_ptr, size_t zero_vec4) {
     constexpr int BINS = 1 << BITS;
     extern __shared__ uint32_t s_hist[];  // passes * BINS
     for (int i = threadIdx.x; i < passes * BINS; i += kFrontBlock) s_hist[i] = 0;
-    // Clear the sort lookback area (consumed by the passes that follow).
     {
         const size_t gstride = (size_t)gridDim.x * kFrontBlock;
         for (size_t j = (size_t)blockIdx.x * kFrontBlock + threadIdx.x; j < zero_vec4; j += gstride)
             zero_ptr[j] = make_uint4(0, 0, 0, 0);
     }
-    const float smnx = ord2f(scene_ord[0]), smny = ord2f(scene_ord[1]), smnz = ord2f(scene_ord[2]);
-    const float smxx = ord2f(scene_ord[3]), smxy = ord2f(scene_ord[4]), smxz = ord2f(scene_ord[5]);
+    uint32_t seg, chunk;
+    SegDesc d;
+    block_segment(sb, seg, chunk, d);
+    const uint32_t* sc = seg_scene + 8 * (size_t)seg;
+    const float smnx = ord2f(sc[0]), smny = ord2f(sc[1]), smnz = ord2f(sc[2]);
+    const float smxx = ord2f(sc[3]), smxy = ord2f(sc[4]), smxz = ord2f(sc[5]);
+    const uint64_t segbits = (uint64_t)seg << kf.seg_shift;
     __syncthreads();
-    const uint32_t stride = gridDim.x * kFrontBlock;
-    for (uint32_t i = blockIdx.x * kFrontBlock + threadIdx.x; i < n; i += stride) {
-        Box b = load_box(tri_boxes + 2 * (size_t)i);
+    const uint32_t lo = chunk * kFrontChunk;
+    const uint32_t hi = (lo + kFrontChunk < d.n) ? lo + kFrontChunk : d.n;
+    const float4* __restrict__ bx = tri_boxes + 2 * (size_t)d.start;
+    for (uint32_t i = lo + threadIdx.x; i < hi; i += kFrontBlock) {
+        Box b = load_box(bx + 2 * (size_t)i);
         float x = norm_coord(b.mnx, b.mxx, smnx, smxx);
         float y = norm_coord(b.mny, b.mxy, smny, smxy);
         float z = norm_coord(b.mnz, b.mxz, smnz, smxz);
         uint64_t key;
+        const size_t g = (size_t)d.start + i;
         if (M60) {
             key = morton60(x, y, z);
-            vals[i] = i;
+            vals[g] = i;
         } else {
-            key = ((uint64_t)morton30(x, y, z) << 32) | (uint64_t)i;
+            key = segbits | ((uint64_t)morton30(x, y, z) << kf.idx_bits) | (uint64_t)i;
         }
-        keys[i] = key;
+        keys[g] = key;
         for (int p = 0; p < passes; ++p) {
-            uint32_t d = (uint32_t)(key >> sh.s[p]) & sh.m[p];
-            atomicAdd(&s_hist[p * BINS + d], 1u);
+            uint32_t dg = (uint32_t)(key >> sh.s[p]) & sh.m[p];
+            atomicAdd(&s_hist[p * BINS + dg], 1u);
         }
     }
     __syncthreads();
@@ -181,56 +204,60 @@ __global__ __launch_bounds__(kFrontBlock) void k_morton(const float4* __restrict
 }
 
 template <int BITS, bool M60>
-static void morton_launch(const float4* tri_boxes, uint32_t n, const uint32_t* scene_ord, uint64_t* keys,
-                          uint32_t* vals, const SortPlan& plan, uint32_t* hist_part, uint32_t* zero_ptr,
-                          size_t zero_words, hipStream_t s) {
+static void morton_launch(const SegBlocks& sb, const float4* tri_boxes, const uint32_t* seg_scene, const KeyFormat& kf,
+                          uint64_t* keys, uint32_t* vals, const SortPlan& plan, uint32_t* hist_part,
+                          uint32_t* zero_ptr, size_t zero_words, hipStream_t s) {
     Shifts sh;
     for (int i = 0; i < 8; ++i) {
         sh.s[i] = plan.shift[i];
         sh.m[i] = plan.mask[i];
     }
     size_t lds = (size_t)plan.passes * (1u << BITS) * 4;
-    hipLaunchKernelGGL((k_morton<BITS, M60>), dim3(plan.hist_blocks), dim3(kFrontBlock), lds, s, tri_boxes, n,
-                       scene_ord, keys, vals, plan.passes, sh, hist_part, reinterpret_cast<uint4*>(zero_ptr),
-                       zero_words / 4);
+    hipLaunchKernelGGL((k_morton<BITS, M60>), dim3(sb.nblocks), dim3(kFrontBlock), lds, s, sb, tri_boxes, seg_scene, kf,
+                       keys, vals, plan.passes, sh, hist_part, reinterpret_cast<uint4*>(zero_ptr), zero_words / 4);
 }
 
-void launch_morton(const float4* tri_boxes, uint32_t n, const uint32_t* scene_ord, bool m60, uint64_t* keys,
-                   uint32_t* vals, const SortPlan& plan, uint32_t* hist_part, uint32_t* zero_ptr, size_t zero_words,
-                   hipStream_t s) {
+void launch_morton(const SegBlocks& sb, const float4* tri_boxes, const uint32_t* seg_scene, bool m60,
+                   const KeyFormat& kf, uint64_t* keys, uint32_t* vals, const SortPlan& plan, uint32_t* hist_part,
+                   uint32_t* zero_ptr, size_t zero_words, hipStream_t s) {
     if (plan.bits == 8) {
-        if (m60) morton_launch<8, true>(tri_boxes, n, scene_ord, keys, vals, plan, hist_part, zero_ptr, zero_words, s);
-        else morton_launch<8, false>(tri_boxes, n, scene_ord, keys, vals, plan, hist_part, zero_ptr, zero_words, s);
+        if (m60) morton_launch<8, true>(sb, tri_boxes, seg_scene, kf, keys, vals, plan, hist_part, zero_ptr, zero_words, s);
+        else morton_launch<8, false>(sb, tri_boxes, seg_scene, kf, keys, vals, plan, hist_part, zero_ptr, zero_words, s);
     } else {
-        if (m60) morton_launch<10, true>(tri_boxes, n, scene_ord, keys, vals, plan, hist_part, zero_ptr, zero_words, s);
-        else morton_launch<10, false>(tri_boxes, n, scene_ord, keys, vals, plan, hist_part, zero_ptr, zero_words, s);
+        if (m60) morton_launch<10, true>(sb, tri_boxes, seg_scene, kf, keys, vals, plan, hist_part, zero_ptr, zero_words, s);
+        else morton_launch<10, false>(sb, tri_boxes, seg_scene, kf, keys, vals, plan, hist_part, zero_ptr, zero_words, s);
     }
 }
 
 template <bool M60>
-__global__ __launch_bounds__(kFrontBlock) void k_morton_codes(const float4* __restrict__ tri_boxes, uint32_t n,
-                                                              const uint32_t* __restrict__ scene_ord,
+__global__ __launch_bounds__(kFrontBlock) void k_morton_codes(SegBlocks sb, const float4* __restrict__ tri_boxes,
+                                                              const uint32_t* __restrict__ seg_scene,
                                                               uint64_t* __restrict__ codes) {
-    const float smnx = ord2f(scene_ord[0]), smny = ord2f(scene_ord[1]), smnz = ord2f(scene_ord[2]);
-    const float smxx = ord2f(scene_ord[3]), smxy = ord2f(scene_ord[4]), smxz = ord2f(scene_ord[5]);
-    const uint32_t stride = gridDim.x * kFrontBlock;
-    for (uint32_t i = blockIdx.x * kFrontBlock + threadIdx.x; i < n; i += stride) {
-        Box b = load_box(tri_boxes + 2 * (size_t)i);
+    uint32_t seg, chunk;
+    SegDesc d;
+    block_segment(sb, seg, chunk, d);
+    const uint32_t* sc = seg_scene + 8 * (size_t)seg;
+    const float smnx = ord2f(sc[0]), smny = ord2f(sc[1]), smnz = ord2f(sc[2]);
+    const float smxx = ord2f(sc[3]), smxy = ord2f(sc[4]), smxz = ord2f(sc[5]);
+    const uint32_t lo = chunk * kFrontChunk;
+    const uint32_t hi = (lo + kFrontChunk < d.n) ? lo + kFrontChunk : d.n;
+    for (uint32_t i = lo + threadIdx.x; i < hi; i += kFrontBlock) {
+        Box b = load_box(tri_boxes + 2 * ((size_t)d.start + i));
         float x = norm_coord(b.mnx, b.mxx, smnx, smxx);
         float y = norm_coord(b.mny, b.mxy, smny, smxy);
         float z = norm_coord(b.mnz, b.mxz, smnz, smxz);
-        codes[i] = M60 ? morton60(x, y, z) : (uint64_t)morton30(x, y, z);
+        codes[(size_t)d.start + i] = M60 ? morton60(x, y, z) : (uint64_t)morton30(x, y, z);
     }
 }
 
-void launch_morton_codes(const float4* tri_boxes, uint32_t n, const uint32_t* scene_ord, bool m60, uint64_t* codes,
-                         hipStream_t s) {
+void launch_morton_codes(const SegBlocks& sb, const float4* tri_boxes, const uint32_t* seg_scene, bool m60,
+                         uint64_t* codes, hipStream_t s) {
     if (m60)
-        hipLaunchKernelGGL(k_morton_codes<true>, dim3(front_grid(n)), dim3(kFrontBlock), 0, s, tri_boxes, n, scene_ord,
+        hipLaunchKernelGGL(k_morton_codes<true>, dim3(sb.nblocks), dim3(kFrontBlock), 0, s, sb, tri_boxes, seg_scene,
                            codes);
     else
-        hipLaunchKernelGGL(k_morton_codes<false>, dim3(front_grid(n)), dim3(kFrontBlock), 0, s, tri_boxes, n,
-                           scene_ord, codes);
+        hipLaunchKernelGGL(k_morton_codes<false>, dim3(sb.nblocks), dim3(kFrontBlock), 0, s, sb, tri_boxes, seg_scene,
+                           codes);
 }
 
 }  // namespace bvhk

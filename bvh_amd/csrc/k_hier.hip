@@ -25,9 +25,12 @@
 // minima of the chunk's delta values, which both children evaluate
 // identically) hands off through an LDS word with workgroup-scope ordering.
 // Only nodes that straddle chunk boundaries (the top of the tree, roughly
-// 2n/2048 nodes) hand off through an epoch-tagged 64-bit word in HBM with an
-// agent-scope acquire-release exchange, the only place that pays for
-// cross-XCD coherence.  Epoch tags mean the handoff array is never cleared.
+// 2n/2048 nodes) hand off through an epoch-tagged 64-bit word in HBM; their
+// records are written with agent-scope (sc1) atomic stores and read with
+// agent-scope loads, with a full wait before the exchange, so no L2 writeback
+// or invalidate is ever needed.  Epoch tags mean the handoff array is never
+// cleared.  Blocks map onto (mesh, chunk) pairs, so a batch of meshes builds
+// in one launch with chunk and root boundaries at every mesh boundary.
 #include "dev_common.hpp"
 #include "engine.hpp"
 
@@ -129,15 +132,33 @@ __global__ __launch_bounds__(kHierBlock) void k_hierarchy(HierArgs p) {
     __shared__ uint32_t s_slot[kChunk];   // LDS handoff words (far end + 1)
     __shared__ int s_w[kHierBlock / kWave];
 
-    const uint32_t n = p.n;
-    const uint32_t a = blockIdx.x * kChunk;
+    // This block's mesh (segment) and chunk; everything below is in the
+    // mesh's local coordinates.
+    SegDesc d;
+    uint32_t chunk;
+    if (p.sb.blocks) {
+        const uint2 e = p.sb.blocks[blockIdx.x];
+        d = p.sb.segs[e.x];
+        chunk = e.y;
+    } else {
+        d = p.sb.single;
+        chunk = blockIdx.x;
+    }
+    const uint32_t n = d.n;
+    const uint64_t* __restrict__ keys = p.keys + d.start;
+    const uint32_t* __restrict__ vals = K60 ? p.vals + d.start : nullptr;
+    const float4* __restrict__ tboxes = p.tri_boxes + 2 * (size_t)d.start;
+    uint64_t* __restrict__ handoff = p.handoff + d.start;
+    const uint32_t epoch = *p.epoch;
+    const uint64_t idx_mask = (p.kf.idx_bits >= 32) ? 0xFFFFFFFFull : ((1ull << p.kf.idx_bits) - 1);
+    const uint32_t a = chunk * kChunk;
     const uint32_t b = (a + kChunk < n) ? a + kChunk : n;
     const int len = (int)(b - a);
     const int tid = threadIdx.x;
 
     for (int q = tid; q <= len; q += kHierBlock) {
         int64_t j = (int64_t)a - 1 + q;
-        s_d[q] = (j < 0 || j >= (int64_t)n - 1) ? (int8_t)-1 : (int8_t)key_delta<K60>(p.keys, p.vals, (uint32_t)j);
+        s_d[q] = (j < 0 || j >= (int64_t)n - 1) ? (int8_t)-1 : (int8_t)key_delta<K60>(keys, vals, (uint32_t)j);
     }
     for (int q = tid; q < len; q += kHierBlock) s_slot[q] = 0;
     __syncthreads();
@@ -148,24 +169,24 @@ __global__ __launch_bounds__(kHierBlock) void k_hierarchy(HierArgs p) {
     auto dget = [&](int64_t j) -> int {
         if (j < 0 || j >= (int64_t)n - 1) return -1;
         if (j >= (int64_t)a - 1 && j <= (int64_t)b - 1) return s_d[j - ((int64_t)a - 1)];
-        return key_delta<K60>(p.keys, p.vals, (uint32_t)j);
+        return key_delta<K60>(keys, vals, (uint32_t)j);
     };
 
-    DNode* __restrict__ nodes = p.nodes;
+    DNode* __restrict__ nodes = d.nodes;
     for (int k = 0; k < kHierItems; ++k) {
         const uint32_t i = a + k * kHierBlock + tid;
         if (i >= b) continue;
-        uint32_t prim = K60 ? p.vals[i] : (uint32_t)p.keys[i];
+        uint32_t prim = K60 ? vals[i] : (uint32_t)(keys[i] & idx_mask);
         if (prim >= n) {  // cannot happen for a correct sort; never index out of bounds
             atomicOr(p.err, 1u);
             prim = 0;
         }
-        p.indices[i] = prim;
+        d.indices[i] = prim;
         // The current node: its range, box, children and index.  Its record is
         // written once its handoff kind is known.
         uint32_t l = i, r = i;
         uint32_t my = n - 1 + i;
-        Box box = load_box(p.tri_boxes + 2 * (size_t)prim);
+        Box box = load_box(tboxes + 2 * (size_t)prim);
         int32_t lc = (int32_t)(prim | 0x80000000u), rc = -1;
         int dl = dget((int64_t)l - 1);
         int dr = dget((int64_t)r);
@@ -203,9 +224,9 @@ __global__ __launch_bounds__(kHierBlock) void k_hierarchy(HierArgs p) {
                 sbox = node_box(nodes, sib);
             } else {
                 local = false;
-                uint64_t w = ((uint64_t)p.epoch << 32) | far;
-                uint64_t old = __hip_atomic_exchange(&p.handoff[s], w, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                if ((uint32_t)(old >> 32) != p.epoch) break;
+                uint64_t w = ((uint64_t)epoch << 32) | far;
+                uint64_t old = __hip_atomic_exchange(&handoff[s], w, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                if ((uint32_t)(old >> 32) != epoch) break;
                 other = (uint32_t)old;
                 if (isLeft) sib = (s + 1 == other) ? (n - 1 + other) : (s + 1);
                 else sib = (other == s) ? (n - 1 + s) : s;
@@ -231,9 +252,8 @@ __global__ __launch_bounds__(kHierBlock) void k_hierarchy(HierArgs p) {
 }
 
 void launch_hierarchy(const HierArgs& a, hipStream_t s) {
-    uint32_t grid = (a.n + kChunk - 1) / kChunk;
-    if (a.vals) hipLaunchKernelGGL(k_hierarchy<true>, dim3(grid), dim3(kHierBlock), 0, s, a);
-    else hipLaunchKernelGGL(k_hierarchy<false>, dim3(grid), dim3(kHierBlock), 0, s, a);
+    if (a.vals) hipLaunchKernelGGL(k_hierarchy<true>, dim3(a.sb.nblocks), dim3(kHierBlock), 0, s, a);
+    else hipLaunchKernelGGL(k_hierarchy<false>, dim3(a.sb.nblocks), dim3(kHierBlock), 0, s, a);
 }
 
 }  // namespace bvhk

@@ -1,37 +1,45 @@
-// k_ploc.hip — PLOC (Meister & Bittner 2017) on gfx950.
+// k_ploc.hip — PLOC (Meister & Bittner 2017) on gfx950, for one mesh or for a
+// batch of meshes (segments) in the same launches.
 //
 // Replaces kGatherAABBs/kInitLeafNodes_ploc/kInitializeSequence,
 // kFindNearestNeighbors_ploc, kMergeClusters_ploc, cub::DeviceScan +
 // kCompactIndices/kCompactAABBs, the per-iteration host round trips and the
 // host root swap of the reference (src/cuda/ploc_builder.cu:143-293,403-586).
 //
-// Semantics kept from the reference:
+// Semantics kept from the reference, per mesh:
 //   * leaves are nodes 0..n-1 in Morton order, internal nodes are numbered
 //     from n in creation order, and the root (last created) is swapped with
 //     node 0 (ploc_builder.cu:563-582) — applied here as an index remap at
 //     write time, so no host pass is needed;
-//   * nearest neighbour of cluster i = argmin over j in [i-r, i+r], j != i,
-//     of SA(box_i U box_j), ties to the lowest j, non-finite distances never
-//     chosen (the strict '<' scan of ploc_builder.cu:190-205);
+//   * nearest neighbour of cluster i = argmin over j in [i-r, i+r] ∩ mesh,
+//     j != i, of SA(box_i U box_j), ties to the lowest j, non-finite distances
+//     never chosen (the strict '<' scan of ploc_builder.cu:190-205);
 //   * mutual pairs merge into the lower slot, the higher slot is dropped.
 // Deliberate difference: new internal nodes are numbered by rank of the
 // merging slot (ascending) within each iteration, one legal serialisation of
 // the reference's nondeterministic atomicAdd (ploc_builder.cu:245).
 //
+// Segments: the clusters of all meshes live in one array, each mesh's
+// clusters contiguous and in mesh order.  A window never crosses a mesh
+// boundary, compaction keeps every mesh contiguous, and node ids are ranked
+// within the mesh by a segmented scan, so every mesh gets exactly the tree
+// its own build gives (a single build is a batch of one).
+//
 // MI355X mapping:
-//   * k_ploc_nn: a workgroup stages its 256 clusters plus an r-halo on each
-//     side in LDS (SoA floats, conflict-free), every lane scans its window
-//     from LDS and packs (distance bits << 32 | j) into one u64 minimum;
-//   * k_ploc_merge: per tile of 1024 clusters, mutual-pair detection, new
-//     nodes, and stream compaction of the survivors behind a single-pass
-//     decoupled look-back scan of packed (merges, survivors) counts — the
-//     reference's scan + 2 D2H copies + 2 compaction kernels in one pass;
-//   * loop control lives on the device: iteration k reads state[k] and its
-//     last tile writes state[k+1] (cluster count, node base, buffer parity;
-//     a ring of kPlocMaxIter slots), so the host launches batches of
-//     iterations and reads one 32-byte state per batch;
-//   * once <= kPlocSmall clusters remain, one workgroup finishes every
-//     remaining iteration inside LDS (k_ploc_tail).
+//   * k_ploc_nn: persistent blocks; each tile stages 256 clusters plus an
+//     r-halo on each side in LDS (SoA floats, conflict-free), every lane scans
+//     its window from LDS and packs (distance bits << 32 | j) into one u64
+//     minimum;
+//   * k_ploc_merge: persistent blocks take tiles of 1024 clusters in order
+//     and run mutual-pair detection, new nodes, and stream compaction of the
+//     survivors behind one decoupled look-back scan whose 64-bit words pack
+//     (segment-start flag, merges within the mesh, survivors overall);
+//   * loop control lives on the device: iteration k reads state[k] and
+//     writes state[k+1] (cluster count, largest mesh, buffer parity; a ring of
+//     kPlocMaxIter slots) and per-mesh records (first cluster, count, next
+//     node id), so the host launches (or captures) fixed sequences;
+//   * once every mesh has <= kPlocSmall clusters, one workgroup per mesh
+//     finishes all remaining iterations inside LDS (k_ploc_tail).
 #include "dev_common.hpp"
 #include "engine.hpp"
 
@@ -42,10 +50,17 @@ constexpr int kMergeBlock = 256;
 constexpr int kMergeItems = 4;
 constexpr int kMergeTile = kMergeBlock * kMergeItems;
 constexpr int kTailBlock = 1024;
-constexpr uint32_t kFlagNoProgress = 1u;
+constexpr uint32_t kNnGridMax = 2048;
+constexpr uint32_t kMergeGridMax = 1024;
+
+// Look-back word: bits 63-62 status, 61 segment start, 60-31 merges within
+// the current mesh, 30-0 survivors overall.
 constexpr uint64_t kStA = 1ull << 62;
 constexpr uint64_t kStP = 2ull << 62;
 constexpr uint64_t kStFlags = 3ull << 62;
+constexpr uint64_t kSegF = 1ull << 61;
+constexpr uint64_t kVMask = (1ull << 31) - 1;
+constexpr uint64_t kMMask = ((1ull << 30) - 1) << 31;
 
 int ploc_small() { return kPlocSmall; }
 int ploc_nn_block() { return kNnBlock; }
@@ -55,33 +70,63 @@ __device__ __forceinline__ uint32_t remap_id(uint32_t id, uint32_t root) {
     return id == 0 ? root : (id == root ? 0u : id);
 }
 
+__device__ __forceinline__ const SegDesc& seg_desc(const SegBlocks& sb, uint32_t s) {
+    return sb.blocks ? sb.segs[s] : sb.single;
+}
+
 // Packed neighbour candidate; non-finite distances are never candidates.
 __device__ __forceinline__ uint64_t nn_key(float dist, uint32_t j) {
     uint32_t bits = __float_as_uint(dist);
     return ((bits & 0x7f800000u) == 0x7f800000u) ? ~0ull : (((uint64_t)bits << 32) | j);
 }
 
-__global__ void k_ploc_init(PlocArgs a, const uint64_t* __restrict__ keys, const float4* __restrict__ tri_boxes,
-                            uint32_t* __restrict__ indices) {
-    const uint32_t n = a.n;
+// Segmented-scan operator on look-back values (a precedes b): survivors add,
+// merges add unless b starts a mesh.
+__device__ __forceinline__ uint64_t scomb(uint64_t a, uint64_t b) {
+    const uint64_t v = ((a & kVMask) + (b & kVMask)) & kVMask;
+    const uint64_t mm = (b & kSegF) ? (b & kMMask) : (((a & kMMask) + (b & kMMask)) & kMMask);
+    return ((a | b) & kSegF) | mm | v;
+}
+
+__global__ __launch_bounds__(256) void k_ploc_init(PlocArgs a, const uint64_t* __restrict__ keys, KeyFormat kf,
+                                                   const float4* __restrict__ tri_boxes, uint32_t* err) {
+    uint32_t seg, chunk;
+    SegDesc d;
+    if (a.sb.blocks) {
+        const uint2 e = a.sb.blocks[blockIdx.x];
+        seg = e.x;
+        chunk = e.y;
+        d = a.sb.segs[seg];
+    } else {
+        seg = 0;
+        chunk = blockIdx.x;
+        d = a.sb.single;
+    }
+    const uint32_t n = d.n;
     const uint32_t root = 2 * n - 2;
-    const uint32_t stride = gridDim.x * blockDim.x;
-    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
-        uint32_t prim = (uint32_t)keys[i];
-        if (prim >= n) {
-            atomicOr(&a.state[0].flags, 2u);
+    const uint64_t idx_mask = (kf.idx_bits >= 32) ? 0xFFFFFFFFull : ((1ull << kf.idx_bits) - 1);
+    const uint32_t lo = chunk * kFrontChunk;
+    const uint32_t hi = (lo + kFrontChunk < n) ? lo + kFrontChunk : n;
+    const float4* __restrict__ tb = tri_boxes + 2 * (size_t)d.start;
+    for (uint32_t i = lo + threadIdx.x; i < hi; i += 256) {
+        const size_t g = (size_t)d.start + i;
+        uint32_t prim = (uint32_t)(keys[g] & idx_mask);
+        if (prim >= n) {  // cannot happen for a correct sort; never index out of bounds
+            atomicOr(err, 1u);
             prim = 0;
         }
-        Box b = load_box(tri_boxes + 2 * (size_t)prim);
-        store_node(a.nodes, n > 1 ? remap_id(i, root) : i, b, (int32_t)(prim | 0x80000000u), -1);
-        indices[i] = prim;
-        store_box(a.box[0] + 2 * (size_t)i, b);
-        a.id[0][i] = i;
+        Box b = load_box(tb + 2 * (size_t)prim);
+        store_node(d.nodes, n > 1 ? remap_id(i, root) : i, b, (int32_t)(prim | 0x80000000u), -1);
+        d.indices[i] = prim;
+        store_box(a.box[0] + 2 * g, b);
+        a.id[0][g] = i;
+        if (a.sid[0]) a.sid[0][g] = seg;
     }
+    if (chunk == 0 && threadIdx.x == 0) a.seg[0][seg] = make_uint4(d.start, n, n, 0u);
     if (blockIdx.x == 0 && threadIdx.x == 0) {
         PlocState s;
-        s.m = n;
-        s.node_base = n;
+        s.m = a.m0;
+        s.maxm = a.maxm0;
         s.parity = 0;
         s.iters = 0;
         s.flags = 0;
@@ -90,221 +135,263 @@ __global__ void k_ploc_init(PlocArgs a, const uint64_t* __restrict__ keys, const
     }
 }
 
+__device__ __forceinline__ bool ploc_global_step(const PlocState& st) {
+    return st.maxm > (uint32_t)kPlocSmall && st.flags == 0;
+}
+
+template <bool SEG>
 __global__ __launch_bounds__(kNnBlock) void k_ploc_nn(PlocArgs a, int k) {
     extern __shared__ float s_nn[];  // 6 SoA arrays of (kNnBlock + 2r) floats
     const PlocState st = a.state[k];
+    if (!ploc_global_step(st)) return;
     const uint32_t m = st.m;
-    if (m <= (uint32_t)kPlocSmall || st.flags) return;
+    const uint32_t par = st.parity;
     const uint32_t tiles = (m + kNnBlock - 1) / kNnBlock;
-    if (blockIdx.x >= tiles) return;
+    const uint32_t mtiles = (m + kMergeTile - 1) / kMergeTile;
+    // Reset the merge kernel's look-back words, tile counter and next state.
+    for (uint32_t t = blockIdx.x * kNnBlock + threadIdx.x; t < mtiles; t += gridDim.x * kNnBlock) a.status[t] = 0;
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        *a.tile_ctr = 0;
+        PlocState ns;
+        ns.m = 0;
+        ns.maxm = 0;
+        ns.parity = par ^ 1u;
+        ns.iters = st.iters + 1;
+        ns.flags = 0;
+        ns.pad[0] = ns.pad[1] = ns.pad[2] = 0;
+        a.state[(k + 1) % kPlocMaxIter] = ns;
+    }
     const int r = a.radius;
     const int span = kNnBlock + 2 * r;
-    if (threadIdx.x == 0) {
-        a.status[blockIdx.x] = 0;  // look-back word of the merge tile with this index
-        if (blockIdx.x == 0) *a.tile_ctr = 0;
-    }
-    const float4* __restrict__ bx = a.box[st.parity];
-    const int64_t t0 = (int64_t)blockIdx.x * kNnBlock;
-    for (int q = threadIdx.x; q < span; q += kNnBlock) {
-        int64_t p = t0 - r + q;
-        if (p >= 0 && p < (int64_t)m) {
-            Box b = load_box(bx + 2 * (size_t)p);
-            s_nn[0 * span + q] = b.mnx;
-            s_nn[1 * span + q] = b.mny;
-            s_nn[2 * span + q] = b.mnz;
-            s_nn[3 * span + q] = b.mxx;
-            s_nn[4 * span + q] = b.mxy;
-            s_nn[5 * span + q] = b.mxz;
+    const float4* __restrict__ bx = a.box[par];
+    const uint4* __restrict__ segs = a.seg[par];
+    for (uint32_t tile = blockIdx.x; tile < tiles; tile += gridDim.x) {
+        __syncthreads();
+        const int64_t t0 = (int64_t)tile * kNnBlock;
+        for (int q = threadIdx.x; q < span; q += kNnBlock) {
+            int64_t p = t0 - r + q;
+            if (p >= 0 && p < (int64_t)m) {
+                Box b = load_box(bx + 2 * (size_t)p);
+                s_nn[0 * span + q] = b.mnx;
+                s_nn[1 * span + q] = b.mny;
+                s_nn[2 * span + q] = b.mnz;
+                s_nn[3 * span + q] = b.mxx;
+                s_nn[4 * span + q] = b.mxy;
+                s_nn[5 * span + q] = b.mxz;
+            }
         }
+        __syncthreads();
+        const int64_t i = t0 + threadIdx.x;
+        if (i >= (int64_t)m) continue;
+        const uint4 sg = segs[SEG ? a.sid[par][i] : 0u];
+        const int64_t slo = sg.x, shi = (int64_t)sg.x + sg.y - 1;
+        const int qi = threadIdx.x + r;
+        Box bi;
+        bi.mnx = s_nn[0 * span + qi]; bi.mny = s_nn[1 * span + qi]; bi.mnz = s_nn[2 * span + qi];
+        bi.mxx = s_nn[3 * span + qi]; bi.mxy = s_nn[4 * span + qi]; bi.mxz = s_nn[5 * span + qi];
+        const int64_t lo = (i - r > slo) ? i - r : slo;
+        const int64_t hi = (i + r < shi) ? i + r : shi;
+        uint64_t best = ~0ull;
+        for (int64_t j = lo; j <= hi; ++j) {
+            if (j == i) continue;
+            const int q = (int)(j - t0) + r;
+            Box bj;
+            bj.mnx = s_nn[0 * span + q]; bj.mny = s_nn[1 * span + q]; bj.mnz = s_nn[2 * span + q];
+            bj.mxx = s_nn[3 * span + q]; bj.mxy = s_nn[4 * span + q]; bj.mxz = s_nn[5 * span + q];
+            uint64_t key = nn_key(area(merge(bi, bj)), (uint32_t)j);
+            best = key < best ? key : best;
+        }
+        a.nn[i] = (best == ~0ull) ? -1 : (int32_t)(uint32_t)best;
     }
-    __syncthreads();
-    const int64_t i = t0 + threadIdx.x;
-    if (i >= (int64_t)m) return;
-    const int qi = threadIdx.x + r;
-    Box bi;
-    bi.mnx = s_nn[0 * span + qi]; bi.mny = s_nn[1 * span + qi]; bi.mnz = s_nn[2 * span + qi];
-    bi.mxx = s_nn[3 * span + qi]; bi.mxy = s_nn[4 * span + qi]; bi.mxz = s_nn[5 * span + qi];
-    const int64_t lo = (i - r > 0) ? i - r : 0;
-    const int64_t hi = (i + r < (int64_t)m - 1) ? i + r : (int64_t)m - 1;
-    uint64_t best = ~0ull;
-    for (int64_t j = lo; j <= hi; ++j) {
-        if (j == i) continue;
-        const int q = (int)(j - t0) + r;
-        Box bj;
-        bj.mnx = s_nn[0 * span + q]; bj.mny = s_nn[1 * span + q]; bj.mnz = s_nn[2 * span + q];
-        bj.mxx = s_nn[3 * span + q]; bj.mxy = s_nn[4 * span + q]; bj.mxz = s_nn[5 * span + q];
-        uint64_t key = nn_key(area(merge(bi, bj)), (uint32_t)j);
-        best = key < best ? key : best;
-    }
-    a.nn[i] = (best == ~0ull) ? -1 : (int32_t)(uint32_t)best;
 }
 
-__device__ __forceinline__ uint64_t wave_incl_sum64(uint64_t v) {
-    uint32_t l = __lane_id();
+__device__ __forceinline__ uint64_t wave_incl_scan_seg(uint64_t v) {
+    const uint32_t l = __lane_id();
 #pragma unroll
     for (int o = 1; o < 64; o <<= 1) {
         uint64_t w = __shfl_up(v, o, 64);
-        if (l >= (uint32_t)o) v += w;
+        if (l >= (uint32_t)o) v = scomb(w, v);
     }
     return v;
 }
 
-// Exclusive block scan of packed (merges << 31 | survivors) counts.
+// Exclusive block scan under scomb; *total = the block's combined value.
 template <int BLOCK>
-__device__ __forceinline__ uint64_t block_excl_scan64(uint64_t v, uint64_t* s_w, uint64_t* total) {
+__device__ __forceinline__ uint64_t block_excl_scan_seg(uint64_t v, uint64_t* s_w, uint64_t* total) {
     constexpr int WAVES = BLOCK / kWave;
     const int wave = threadIdx.x / kWave, lane = threadIdx.x % kWave;
-    uint64_t incl = wave_incl_sum64(v);
+    const uint64_t incl = wave_incl_scan_seg(v);
     if (lane == kWave - 1) s_w[wave] = incl;
     __syncthreads();
     uint64_t before = 0, all = 0;
 #pragma unroll
     for (int w = 0; w < WAVES; ++w) {
-        uint64_t x = s_w[w];
-        before += (w < wave) ? x : 0ull;
-        all += x;
+        const uint64_t x = s_w[w];
+        if (w < wave) before = scomb(before, x);
+        all = scomb(all, x);
     }
     __syncthreads();
     *total = all;
-    return before + incl - v;
+    uint64_t prev = __shfl_up(incl, 1, 64);
+    if (lane == 0) prev = 0;
+    return scomb(before, prev);
 }
 
+template <bool SEG>
 __global__ __launch_bounds__(kMergeBlock) void k_ploc_merge(PlocArgs a, int k) {
     __shared__ uint32_t s_tile;
     __shared__ uint64_t s_w[kMergeBlock / kWave];
     __shared__ uint64_t s_prefix;
     const PlocState st = a.state[k];
-    const uint32_t m = st.m;
-    if (m <= (uint32_t)kPlocSmall || st.flags) {
-        if (blockIdx.x == 0 && threadIdx.x == 0) a.state[(k + 1) % kPlocMaxIter] = st;
+    const int kn = (k + 1) % kPlocMaxIter;
+    if (!ploc_global_step(st)) {
+        if (blockIdx.x == 0 && threadIdx.x == 0) a.state[kn] = st;
         return;
     }
+    const uint32_t m = st.m;
+    const uint32_t par = st.parity;
     const uint32_t tiles = (m + kMergeTile - 1) / kMergeTile;
-    if (threadIdx.x == 0) s_tile = atomicAdd(a.tile_ctr, 1u);
-    __syncthreads();
-    const uint32_t tile = s_tile;
-    if (tile >= tiles) return;
-    const uint32_t n = a.n;
-    const uint32_t root = 2 * n - 2;
     const int32_t* __restrict__ nn = a.nn;
-    const uint32_t base = tile * kMergeTile + threadIdx.x * kMergeItems;
+    const uint4* __restrict__ segs = a.seg[par];
+    uint4* __restrict__ segs_out = a.seg[par ^ 1u];
+    const float4* __restrict__ bin = a.box[par];
+    const uint32_t* __restrict__ iin = a.id[par];
+    float4* __restrict__ bout = a.box[par ^ 1u];
+    uint32_t* __restrict__ iout = a.id[par ^ 1u];
+    while (true) {
+        __syncthreads();
+        if (threadIdx.x == 0) s_tile = atomicAdd(a.tile_ctr, 1u);
+        __syncthreads();
+        const uint32_t tile = s_tile;
+        if (tile >= tiles) break;
+        const uint32_t base = tile * kMergeTile + threadIdx.x * kMergeItems;
 
-    int32_t nj[kMergeItems];
-    uint32_t fl[kMergeItems];  // bit0 survivor, bit1 merging
-    uint32_t cv = 0, cm = 0;
+        int32_t nj[kMergeItems];
+        uint32_t sid[kMergeItems];
+        uint32_t fl[kMergeItems];  // bit0 survivor, bit1 merging, bit2 first of its mesh, bit3 last
+        uint64_t agg = 0;
 #pragma unroll
-    for (int it = 0; it < kMergeItems; ++it) {
-        const uint32_t i = base + it;
-        fl[it] = 0;
-        nj[it] = -1;
-        if (i < m) {
-            const int32_t j = nn[i];
-            nj[it] = j;
-            const bool mutual = (j >= 0) && nn[j] == (int32_t)i;
-            const bool merging = mutual && (int32_t)i < j;
-            const bool valid = !(mutual && j < (int32_t)i);
-            fl[it] = (valid ? 1u : 0u) | (merging ? 2u : 0u);
-            cv += valid ? 1u : 0u;
-            cm += merging ? 1u : 0u;
-        }
-    }
-    const uint64_t mine = ((uint64_t)cm << 31) | cv;
-    uint64_t agg;
-    const uint64_t excl = block_excl_scan64<kMergeBlock>(mine, s_w, &agg);
-    if (threadIdx.x == 0) {
-        uint64_t* sw = a.status + tile;
-        if (tile == 0) {
-            __hip_atomic_store(sw, kStP | agg, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            s_prefix = 0;
-        } else {
-            __hip_atomic_store(sw, kStA | agg, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            uint64_t pre = 0;
-            int t = (int)tile - 1;
-            while (true) {
-                uint64_t w = __hip_atomic_load(a.status + t, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                uint64_t f = w & kStFlags;
-                if (f == 0) {
-                    __builtin_amdgcn_s_sleep(1);
-                    continue;
-                }
-                pre += w & ~kStFlags;
-                if (f == kStP) break;
-                --t;
+        for (int it = 0; it < kMergeItems; ++it) {
+            const uint32_t i = base + it;
+            fl[it] = 0;
+            nj[it] = -1;
+            sid[it] = 0;
+            if (i < m) {
+                const uint32_t s = SEG ? a.sid[par][i] : 0u;
+                const uint4 sg = segs[s];
+                sid[it] = s;
+                const int32_t j = nn[i];
+                nj[it] = j;
+                const bool mutual = (j >= 0) && nn[j] == (int32_t)i;
+                const bool merging = mutual && (int32_t)i < j;
+                const bool valid = !(mutual && j < (int32_t)i);
+                const bool first = (i == sg.x);
+                const bool last = (i == sg.x + sg.y - 1);
+                fl[it] = (valid ? 1u : 0u) | (merging ? 2u : 0u) | (first ? 4u : 0u) | (last ? 8u : 0u);
+                const uint64_t e = (first ? kSegF : 0ull) | ((uint64_t)(merging ? 1u : 0u) << 31) | (valid ? 1ull : 0ull);
+                agg = scomb(agg, e);
             }
-            __hip_atomic_store(sw, kStP | (pre + agg), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            s_prefix = pre;
         }
-        if (tile == tiles - 1) {
-            const uint64_t tot = s_prefix + agg;
-            const uint32_t survivors = (uint32_t)(tot & 0x7FFFFFFFu);
-            const uint32_t merges = (uint32_t)(tot >> 31);
-            PlocState ns;
-            ns.m = survivors;
-            ns.node_base = st.node_base + merges;
-            ns.parity = st.parity ^ 1u;
-            ns.iters = st.iters + 1;
-            ns.flags = st.flags | (merges == 0 ? kFlagNoProgress : 0u);
-            ns.pad[0] = ns.pad[1] = ns.pad[2] = 0;
-            a.state[(k + 1) % kPlocMaxIter] = ns;
+        uint64_t tagg;
+        const uint64_t excl = block_excl_scan_seg<kMergeBlock>(agg, s_w, &tagg);
+        if (threadIdx.x == 0) {
+            uint64_t* sw = a.status + tile;
+            if (tile == 0) {
+                __hip_atomic_store(sw, kStP | tagg, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                s_prefix = 0;
+            } else {
+                __hip_atomic_store(sw, kStA | tagg, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                uint64_t pre = 0;
+                int t = (int)tile - 1;
+                while (true) {
+                    const uint64_t w = __hip_atomic_load(a.status + t, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                    const uint64_t f = w & kStFlags;
+                    if (f == 0) {
+                        __builtin_amdgcn_s_sleep(1);
+                        continue;
+                    }
+                    pre = scomb(w & ~kStFlags, pre);
+                    if (f == kStP) break;
+                    --t;
+                }
+                __hip_atomic_store(sw, kStP | scomb(pre, tagg), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                s_prefix = pre;
+            }
+            if (tile == tiles - 1) a.state[kn].m = (uint32_t)(scomb(s_prefix, tagg) & kVMask);
         }
-    }
-    __syncthreads();
-    const uint64_t run0 = s_prefix + excl;
-    uint32_t pos = (uint32_t)(run0 & 0x7FFFFFFFu);
-    uint32_t mrank = (uint32_t)(run0 >> 31);
-    const float4* __restrict__ bin = a.box[st.parity];
-    const uint32_t* __restrict__ iin = a.id[st.parity];
-    float4* __restrict__ bout = a.box[st.parity ^ 1u];
-    uint32_t* __restrict__ iout = a.id[st.parity ^ 1u];
+        __syncthreads();
+        const uint64_t run0 = scomb(s_prefix, excl);
+        uint32_t pos = (uint32_t)(run0 & kVMask);
+        uint32_t mrank = (uint32_t)((run0 & kMMask) >> 31);
 #pragma unroll
-    for (int it = 0; it < kMergeItems; ++it) {
-        const uint32_t i = base + it;
-        if (!(fl[it] & 1u)) continue;
-        Box b = load_box(bin + 2 * (size_t)i);
-        uint32_t cid = iin[i];
-        if (fl[it] & 2u) {
-            const uint32_t j = (uint32_t)nj[it];
-            const Box bj = load_box(bin + 2 * (size_t)j);
-            const uint32_t idj = iin[j];
-            b = merge(b, bj);
-            const uint32_t nid = st.node_base + mrank;
-            ++mrank;
-            if (nid <= root)
-                store_node(a.nodes, remap_id(nid, root), b, (int32_t)remap_id(cid, root),
-                           (int32_t)remap_id(idj, root));
-            cid = nid;
+        for (int it = 0; it < kMergeItems; ++it) {
+            const uint32_t i = base + it;
+            if (i >= m) continue;
+            const uint32_t f = fl[it];
+            const uint32_t s = sid[it];
+            if (f & 4u) {
+                mrank = 0;
+                segs_out[s].x = pos;
+            }
+            const uint4 sg = segs[s];
+            if (f & 1u) {
+                Box b = load_box(bin + 2 * (size_t)i);
+                uint32_t cid = iin[i];
+                if (f & 2u) {
+                    const SegDesc& d = seg_desc(a.sb, s);
+                    const uint32_t root = 2 * d.n - 2;
+                    const uint32_t j = (uint32_t)nj[it];
+                    const Box bj = load_box(bin + 2 * (size_t)j);
+                    const uint32_t idj = iin[j];
+                    b = merge(b, bj);
+                    const uint32_t nid = sg.z + mrank;
+                    ++mrank;
+                    if (nid <= root)
+                        store_node(d.nodes, remap_id(nid, root), b, (int32_t)remap_id(cid, root),
+                                   (int32_t)remap_id(idj, root));
+                    cid = nid;
+                }
+                store_box(bout + 2 * (size_t)pos, b);
+                iout[pos] = cid;
+                if (SEG) a.sid[par ^ 1u][pos] = s;
+                ++pos;
+            }
+            if (f & 8u) {
+                const uint32_t nm = sg.y - mrank;
+                segs_out[s].y = nm;
+                segs_out[s].z = sg.z + mrank;
+                segs_out[s].w = sg.w + (sg.y >= 2 ? 1u : 0u);
+                atomicMax(&a.state[kn].maxm, nm);
+                if (sg.y >= 2 && mrank == 0) atomicOr(&a.state[kn].flags, kPlocNoProgress);
+            }
         }
-        store_box(bout + 2 * (size_t)pos, b);
-        iout[pos] = cid;
-        ++pos;
     }
 }
 
-// Finishes every remaining iteration of one mesh inside one workgroup's LDS.
+// Finishes every remaining iteration of one mesh (block = mesh) in LDS.
 __global__ __launch_bounds__(kTailBlock) void k_ploc_tail(PlocArgs a, int k) {
     constexpr int PER = (kPlocSmall + kTailBlock - 1) / kTailBlock;
     __shared__ float s_b[6][kPlocSmall];
     __shared__ uint32_t s_id[kPlocSmall];
     __shared__ int32_t s_nn[kPlocSmall];
     __shared__ uint64_t s_w[kTailBlock / kWave];
+    const uint32_t seg = blockIdx.x;
     const PlocState st = a.state[k];
-    uint32_t m = st.m;
-    const uint32_t n = a.n;
-    const uint32_t root = 2 * n - 2;
+    const uint4 sg = a.seg[st.parity][seg];
+    const SegDesc& d = seg_desc(a.sb, seg);
+    uint32_t m = sg.y;
+    const uint32_t root = 2 * d.n - 2;
     const int r = a.radius;
-    uint32_t node_base = st.node_base;
-    uint32_t iters = st.iters;
+    uint32_t node_base = sg.z;
+    uint32_t iters = sg.w;
     uint32_t flags = st.flags;
+    if (st.maxm > (uint32_t)kPlocSmall) flags |= kPlocUnfinished;
     if (flags || m > (uint32_t)kPlocSmall) {
-        if (threadIdx.x == 0) {
-            PlocState f = st;
-            if (m > (uint32_t)kPlocSmall) f.flags |= 4u;
-            a.state[a.final_slot] = f;
-        }
+        if (threadIdx.x == 0) a.out[seg] = make_uint4(m, node_base, iters, flags);
         return;
     }
-    const float4* __restrict__ bx = a.box[st.parity];
-    const uint32_t* __restrict__ ids = a.id[st.parity];
+    const float4* __restrict__ bx = a.box[st.parity] + 2 * (size_t)sg.x;
+    const uint32_t* __restrict__ ids = a.id[st.parity] + sg.x;
     for (uint32_t i = threadIdx.x; i < m; i += kTailBlock) {
         Box b = load_box(bx + 2 * (size_t)i);
         s_b[0][i] = b.mnx; s_b[1][i] = b.mny; s_b[2][i] = b.mnz;
@@ -362,10 +449,11 @@ __global__ __launch_bounds__(kTailBlock) void k_ploc_tail(PlocArgs a, int k) {
                 }
             }
         }
+        // Plain (unsegmented) counts packed as merges << 31 | survivors.
         uint64_t tot;
-        const uint64_t ex = block_excl_scan64<kTailBlock>(((uint64_t)cm << 31) | cv, s_w, &tot);
-        uint32_t pos = (uint32_t)(ex & 0x7FFFFFFFu);
-        const uint32_t mbase = (uint32_t)(ex >> 31);
+        const uint64_t ex = block_excl_scan_seg<kTailBlock>(((uint64_t)cm << 31) | cv, s_w, &tot);
+        uint32_t pos = (uint32_t)(ex & kVMask);
+        const uint32_t mbase = (uint32_t)((ex & kMMask) >> 31);
         // All LDS reads of this iteration are done (the scan synchronised).
 #pragma unroll
         for (int it = 0; it < PER; ++it) {
@@ -375,7 +463,7 @@ __global__ __launch_bounds__(kTailBlock) void k_ploc_tail(PlocArgs a, int k) {
             if (fl[it] & 2u) {
                 const uint32_t nid = node_base + mbase + (fl[it] >> 2);
                 if (nid <= root)
-                    store_node(a.nodes, remap_id(nid, root), bb[it], (int32_t)remap_id(out_id, root),
+                    store_node(d.nodes, remap_id(nid, root), bb[it], (int32_t)remap_id(out_id, root),
                                (int32_t)remap_id(cid[it], root));
                 out_id = nid;
             }
@@ -394,47 +482,40 @@ __global__ __launch_bounds__(kTailBlock) void k_ploc_tail(PlocArgs a, int k) {
             s_id[p] = cid[it];
         }
         __syncthreads();
-        const uint32_t merges = (uint32_t)(tot >> 31);
-        m = (uint32_t)(tot & 0x7FFFFFFFu);
+        const uint32_t merges = (uint32_t)((tot & kMMask) >> 31);
+        m = (uint32_t)(tot & kVMask);
         node_base += merges;
         ++iters;
         if (merges == 0) {
-            flags |= kFlagNoProgress;
+            flags |= kPlocNoProgress;
             break;
         }
     }
-    if (threadIdx.x == 0) {
-        PlocState f;
-        f.m = m;
-        f.node_base = node_base;
-        f.parity = st.parity;
-        f.iters = iters;
-        f.flags = flags;
-        f.pad[0] = f.pad[1] = f.pad[2] = 0;
-        a.state[a.final_slot] = f;
-    }
+    if (threadIdx.x == 0) a.out[seg] = make_uint4(m, node_base, iters, flags);
 }
 
-void launch_ploc_init(const PlocArgs& a, const uint64_t* keys, const float4* tri_boxes, uint32_t* indices,
-                      hipStream_t s) {
-    uint32_t g = (a.n + 255) / 256;
-    if (g > 4096) g = 4096;
-    hipLaunchKernelGGL(k_ploc_init, dim3(g), dim3(256), 0, s, a, keys, tri_boxes, indices);
+void launch_ploc_init(const PlocArgs& a, const uint64_t* keys, const KeyFormat& kf, const float4* tri_boxes,
+                      uint32_t* err, hipStream_t s) {
+    hipLaunchKernelGGL(k_ploc_init, dim3(a.sb.nblocks), dim3(256), 0, s, a, keys, kf, tri_boxes, err);
 }
 
 void launch_ploc_nn(const PlocArgs& a, int k, uint32_t m_bound, hipStream_t s) {
     uint32_t g = (m_bound + kNnBlock - 1) / kNnBlock;
+    g = g < 1 ? 1 : (g > kNnGridMax ? kNnGridMax : g);
     size_t lds = (size_t)6 * (kNnBlock + 2 * a.radius) * sizeof(float);
-    hipLaunchKernelGGL(k_ploc_nn, dim3(g ? g : 1), dim3(kNnBlock), lds, s, a, k);
+    if (a.sid[0]) hipLaunchKernelGGL(k_ploc_nn<true>, dim3(g), dim3(kNnBlock), lds, s, a, k);
+    else hipLaunchKernelGGL(k_ploc_nn<false>, dim3(g), dim3(kNnBlock), lds, s, a, k);
 }
 
 void launch_ploc_merge(const PlocArgs& a, int k, uint32_t m_bound, hipStream_t s) {
     uint32_t g = (m_bound + kMergeTile - 1) / kMergeTile;
-    hipLaunchKernelGGL(k_ploc_merge, dim3(g ? g : 1), dim3(kMergeBlock), 0, s, a, k);
+    g = g < 1 ? 1 : (g > kMergeGridMax ? kMergeGridMax : g);
+    if (a.sid[0]) hipLaunchKernelGGL(k_ploc_merge<true>, dim3(g), dim3(kMergeBlock), 0, s, a, k);
+    else hipLaunchKernelGGL(k_ploc_merge<false>, dim3(g), dim3(kMergeBlock), 0, s, a, k);
 }
 
 void launch_ploc_tail(const PlocArgs& a, int k, hipStream_t s) {
-    hipLaunchKernelGGL(k_ploc_tail, dim3(1), dim3(kTailBlock), 0, s, a, k);
+    hipLaunchKernelGGL(k_ploc_tail, dim3(a.nseg), dim3(kTailBlock), 0, s, a, k);
 }
 
 }  // namespace bvhk

@@ -1,0 +1,467 @@
+// server.cpp — continuous-batching build+trace server over the C ABI
+// (include/bvh_server.h).  Host C++ only: every device operation goes through
+// bvh_hip.h, each slot on its own context (stream + workspace).
+#include "bvh_server.h"
+
+#include <algorithm>
+#include <chrono>
+#include <condition_variable>
+#include <cstring>
+#include <deque>
+#include <memory>
+#include <mutex>
+#include <string>
+#include <thread>
+#include <unordered_map>
+#include <vector>
+
+#include "bvh_hip.h"
+
+namespace {
+
+uint64_t mesh_bytes(uint32_t n) {
+    const uint64_t stride = ((uint64_t)n + 63) & ~(uint64_t)63;
+    return stride * 9 * 4 + (uint64_t)n * 48;  // SoA copy + traversal AoS copy
+}
+
+uint64_t tree_bytes(uint32_t n) { return (2 * (uint64_t)n - 1) * 48 + (uint64_t)n * 4; }
+
+struct Resident {
+    uint64_t key = 0;
+    bvh_mesh_t mesh = nullptr;
+    bvh_tree_t tree = nullptr;
+    uint32_t ntri = 0;
+    uint64_t bytes = 0;
+    int users = 0;        // admitted requests that still need it
+    bool ready = false;   // built and complete on the device
+    uint64_t last_use = 0;
+};
+
+struct Req {
+    bvh_request r;
+    bool build = false;          // this request builds its mesh
+    Resident* res = nullptr;     // keyed residency (built here or used)
+    uint64_t transient = 0;      // bytes released at completion
+    double admitted = 0.0;
+    bvh_mesh_t mesh = nullptr;   // unkeyed build: freed at completion
+    bvh_tree_t tree = nullptr;
+};
+
+}  // namespace
+
+struct bvh_server_rec {
+    bvh_server_config cfg;
+    std::chrono::steady_clock::time_point t0;
+    std::mutex mu;
+    std::condition_variable cv_work, cv_done;
+    std::deque<Req> queue;
+    std::deque<bvh_completion> done;
+    std::unordered_map<uint64_t, std::unique_ptr<Resident>> residents;
+    std::vector<bvh_ctx_t> ctxs;
+    std::vector<std::thread> threads;
+    bool stop = false;
+    uint64_t in_use = 0, peak = 0, resident_bytes = 0, lru = 0;
+    uint32_t in_flight = 0, groups = 0, completed = 0, evictions = 0, rejected = 0, max_group_seen = 0;
+    std::string err;
+
+    double now() const {
+        return std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    }
+    void run_slot(int slot);
+    bool form_group(bvh_ctx_t own, std::vector<Req>& group);
+    void finish(int slot, std::vector<Req>& group, const std::vector<int>& status);
+    void complete_locked(const Req& q, int status, int slot, uint32_t gsize);
+};
+
+extern "C" void bvh_serve_footprint(uint32_t ntri, uint32_t nrays, int32_t algo, uint64_t* transient,
+                                    uint64_t* resident) {
+    // Build workspace per triangle: boxes 32 + two key buffers 16 + handoff 8;
+    // PLOC adds two cluster buffers (box, id, mesh index) 80 + neighbour 4.
+    const uint64_t build = (uint64_t)ntri * (algo == BVH_ALGO_PLOC ? 140 : 56);
+    const uint64_t rays = (uint64_t)nrays * 40;  // rays 24 + t 4 + prim 4 + overflow list 4, padded
+    if (transient) *transient = build + rays;
+    if (resident) *resident = ntri ? mesh_bytes(ntri) + tree_bytes(ntri) : 0;
+}
+
+extern "C" uint32_t bvh_serve_form_group(const uint64_t* bytes, const uint32_t* tris, const uint8_t* runnable,
+                                         uint32_t pending, uint64_t free_bytes, uint32_t max_group,
+                                         uint32_t max_group_tris) {
+    uint64_t used = 0;
+    uint64_t t = 0;
+    uint32_t g = 0;
+    while (g < pending && g < max_group) {
+        if (!runnable[g]) break;
+        if (used + bytes[g] > free_bytes) break;
+        if (g > 0 && t + tris[g] > max_group_tris) break;
+        used += bytes[g];
+        t += tris[g];
+        ++g;
+        if (t > max_group_tris) break;  // an oversized head runs alone
+    }
+    return g;
+}
+
+void bvh_server_rec::complete_locked(const Req& q, int status, int slot, uint32_t gsize) {
+    bvh_completion c;
+    c.id = q.r.id;
+    c.status = status;
+    c.slot = slot;
+    c.group_size = gsize;
+    c.built = q.build ? 1u : 0u;
+    c.arrival = q.r.arrival;
+    c.admitted = q.admitted;
+    c.done = now();
+    done.push_back(c);
+    ++completed;
+    cv_done.notify_all();
+}
+
+// Called with the lock held by the slot whose context is `own`.  Takes the
+// next group off the queue (evicting idle resident meshes when memory is
+// short), or rejects requests that can never run.
+bool bvh_server_rec::form_group(bvh_ctx_t own, std::vector<Req>& group) {
+    while (!queue.empty()) {
+        const uint32_t look = std::min<uint32_t>((uint32_t)queue.size(), (uint32_t)cfg.max_group);
+        std::vector<uint64_t> bytes(look);
+        std::vector<uint32_t> tris(look);
+        std::vector<uint8_t> run(look);
+        std::vector<Resident*> res(look, nullptr);
+        std::vector<uint8_t> builds(look);
+        std::unordered_map<uint64_t, uint32_t> new_keys;  // keys first built inside this prefix
+        for (uint32_t i = 0; i < look; ++i) {
+            const bvh_request& r = queue[i].r;
+            const bool has_soa = r.soa[0] != nullptr;
+            uint64_t tr = 0, rs = 0;
+            bvh_serve_footprint(r.ntri, r.nrays, cfg.algo, &tr, &rs);
+            Resident* rp = nullptr;
+            if (r.mesh_key) {
+                auto it = residents.find(r.mesh_key);
+                if (it != residents.end()) rp = it->second.get();
+            }
+            res[i] = rp;
+            if (rp) {  // trace against a resident (or one being built)
+                run[i] = rp->ready ? 1 : 0;
+                builds[i] = 0;
+                bytes[i] = (uint64_t)r.nrays * 40;
+                tris[i] = 0;
+            } else if (has_soa) {
+                const bool dup = r.mesh_key && new_keys.count(r.mesh_key);
+                run[i] = dup ? 0 : 1;  // a second request for a key built in this prefix waits for it
+                builds[i] = 1;
+                bytes[i] = tr + rs;
+                tris[i] = r.ntri;
+                if (r.mesh_key) new_keys[r.mesh_key] = i;
+            } else {
+                run[i] = 2;  // names no mesh: rejected below if at the head
+                builds[i] = 0;
+                bytes[i] = 0;
+                tris[i] = 0;
+            }
+        }
+        if (run[0] == 2) {
+            Req q = queue.front();
+            queue.pop_front();
+            q.admitted = now();
+            err = "request names a mesh_key that is not resident";
+            ++rejected;
+            complete_locked(q, BVH_E_STATE, -1, 0);
+            continue;
+        }
+        for (auto& v : run)
+            if (v == 2) v = 0;
+        const uint64_t free_bytes = cfg.memory_budget > in_use ? cfg.memory_budget - in_use : 0;
+        uint32_t g = bvh_serve_form_group(bytes.data(), tris.data(), run.data(), look, free_bytes,
+                                          (uint32_t)cfg.max_group, cfg.max_group_tris);
+        if (g == 0 && run[0]) {
+            // Memory: evict the least recently used idle resident, then look again.
+            Resident* victim = nullptr;
+            for (auto& kv : residents) {
+                Resident* r = kv.second.get();
+                if (r->ready && r->users == 0 && (!victim || r->last_use < victim->last_use)) victim = r;
+            }
+            if (victim) {
+                bvh_tree_free(own, victim->tree);
+                bvh_mesh_free(own, victim->mesh);
+                in_use -= victim->bytes;
+                resident_bytes -= victim->bytes;
+                ++evictions;
+                residents.erase(victim->key);
+                continue;
+            }
+            if (in_flight == 0) {
+                Req q = queue.front();
+                queue.pop_front();
+                q.admitted = now();
+                err = "request needs more device memory than the budget";
+                ++rejected;
+                complete_locked(q, BVH_E_BUDGET, -1, 0);
+                continue;
+            }
+        }
+        if (g == 0) return false;
+        const double t = now();
+        for (uint32_t i = 0; i < g; ++i) {
+            Req q = queue.front();
+            queue.pop_front();
+            q.admitted = t;
+            q.build = builds[i] != 0;
+            uint64_t tr = 0, rs = 0;
+            bvh_serve_footprint(q.r.ntri, q.r.nrays, cfg.algo, &tr, &rs);
+            if (q.build) {
+                if (q.r.mesh_key) {
+                    auto r = std::make_unique<Resident>();
+                    r->key = q.r.mesh_key;
+                    r->ntri = q.r.ntri;
+                    r->bytes = rs;
+                    r->users = 1;
+                    r->last_use = ++lru;
+                    q.res = r.get();
+                    residents[q.r.mesh_key] = std::move(r);
+                    q.transient = tr;
+                    resident_bytes += rs;
+                } else {
+                    q.transient = tr + rs;
+                }
+                in_use += tr + rs;
+            } else {
+                q.res = res[i];
+                q.res->users++;
+                q.res->last_use = ++lru;
+                q.transient = (uint64_t)q.r.nrays * 40;
+                in_use += q.transient;
+            }
+            group.push_back(q);
+        }
+        peak = std::max(peak, in_use);
+        in_flight += g;
+        ++groups;
+        max_group_seen = std::max(max_group_seen, g);
+        return true;
+    }
+    return false;
+}
+
+void bvh_server_rec::finish(int slot, std::vector<Req>& group, const std::vector<int>& status) {
+    bvh_ctx_t ctx = ctxs[slot];
+    // Unkeyed meshes end with their request; keyed ones that failed to build are dropped.
+    for (size_t i = 0; i < group.size(); ++i) {
+        Req& q = group[i];
+        if (q.build && !q.res) {
+            if (q.tree) bvh_tree_free(ctx, q.tree);
+            if (q.mesh) bvh_mesh_free(ctx, q.mesh);
+        }
+    }
+    std::lock_guard<std::mutex> lk(mu);
+    for (size_t i = 0; i < group.size(); ++i) {
+        Req& q = group[i];
+        in_use -= q.transient;
+        if (q.res) {
+            if (q.build) {
+                if (status[i] == BVH_OK) {
+                    q.res->mesh = q.mesh;
+                    q.res->tree = q.tree;
+                    q.res->ready = true;
+                } else {
+                    if (q.tree) bvh_tree_free(ctx, q.tree);
+                    if (q.mesh) bvh_mesh_free(ctx, q.mesh);
+                    in_use -= q.res->bytes;
+                    resident_bytes -= q.res->bytes;
+                    residents.erase(q.res->key);
+                    q.res = nullptr;
+                }
+            }
+            if (q.res) q.res->users--;
+        }
+        complete_locked(q, status[i], slot, (uint32_t)group.size());
+    }
+    in_flight -= (uint32_t)group.size();
+    cv_work.notify_all();
+}
+
+void bvh_server_rec::run_slot(int slot) {
+    bvh_ctx_t ctx = ctxs[slot];
+    while (true) {
+        std::vector<Req> group;
+        {
+            std::unique_lock<std::mutex> lk(mu);
+            while (!stop && !form_group(ctx, group)) cv_work.wait(lk);
+            if (group.empty()) return;  // stopping
+        }
+        std::vector<int> status(group.size(), BVH_OK);
+        // 1. upload the meshes this group builds
+        std::vector<bvh_mesh_t> bm;
+        std::vector<bvh_tree_t> bt;
+        std::vector<size_t> who;
+        for (size_t i = 0; i < group.size(); ++i) {
+            Req& q = group[i];
+            if (!q.build) continue;
+            int st = bvh_mesh_upload(ctx, q.r.soa, q.r.ntri, &q.mesh);
+            if (st == BVH_OK) st = bvh_tree_create(ctx, &q.tree);
+            if (st != BVH_OK) {
+                status[i] = st;
+                continue;
+            }
+            bm.push_back(q.mesh);
+            bt.push_back(q.tree);
+            who.push_back(i);
+        }
+        // 2. one segmented build for all of them
+        if (!bm.empty()) {
+            bvh_build_params p;
+            std::memset(&p, 0, sizeof(p));
+            p.algo = cfg.algo;
+            p.ploc_radius = cfg.ploc_radius;
+            int st = bm.size() == 1 ? bvh_build(ctx, bm[0], &p, bt[0], nullptr)
+                                    : bvh_build_batched(ctx, bm.data(), bt.data(), (uint32_t)bm.size(), &p, nullptr);
+            if (st != BVH_OK) {
+                std::lock_guard<std::mutex> lk(mu);
+                err = bvh_last_error(ctx);
+                for (size_t i : who) status[i] = st;
+            }
+        }
+        // 3. each request's rays against its own tree
+        for (size_t i = 0; i < group.size(); ++i) {
+            if (status[i] != BVH_OK) continue;
+            Req& q = group[i];
+            bvh_mesh_t m = q.build ? q.mesh : q.res->mesh;
+            bvh_tree_t t = q.build ? q.tree : q.res->tree;
+            if (q.r.nrays) {
+                int st = bvh_trace_host(ctx, t, m, q.r.rays, q.r.nrays, q.r.t_out, q.r.prim_out, nullptr, nullptr);
+                if (st != BVH_OK) {
+                    std::lock_guard<std::mutex> lk(mu);
+                    err = bvh_last_error(ctx);
+                    status[i] = st;
+                }
+            }
+        }
+        finish(slot, group, status);
+    }
+}
+
+extern "C" {
+
+int bvh_server_create(const bvh_server_config* cfg, bvh_server_t* out) {
+    if (!cfg || !out) return BVH_E_ARG;
+    *out = nullptr;
+    if (cfg->algo != BVH_ALGO_LBVH && cfg->algo != BVH_ALGO_PLOC) return BVH_E_UNSUPPORTED;
+    if (cfg->algo == BVH_ALGO_PLOC && (cfg->ploc_radius < 1 || cfg->ploc_radius > 1024)) return BVH_E_ARG;
+    if (cfg->slots < 0 || cfg->slots > 16 || cfg->max_group < 0) return BVH_E_ARG;
+    auto s = std::make_unique<bvh_server_rec>();
+    s->cfg = *cfg;
+    if (s->cfg.slots == 0) s->cfg.slots = 4;
+    if (s->cfg.max_group == 0) s->cfg.max_group = 64;
+    if (s->cfg.max_group_tris == 0) s->cfg.max_group_tris = 1u << 24;
+    s->t0 = std::chrono::steady_clock::now();
+    for (int i = 0; i < s->cfg.slots; ++i) {
+        bvh_ctx_t c = nullptr;
+        int st = bvh_ctx_create(cfg->device, &c);
+        if (st != BVH_OK) {
+            for (bvh_ctx_t x : s->ctxs) bvh_ctx_destroy(x);
+            return st;
+        }
+        s->ctxs.push_back(c);
+    }
+    if (s->cfg.memory_budget == 0) {
+        uint64_t fr = 0, tot = 0;
+        if (bvh_device_memory(s->ctxs[0], &fr, &tot) != BVH_OK) {
+            for (bvh_ctx_t x : s->ctxs) bvh_ctx_destroy(x);
+            return BVH_E_HIP;
+        }
+        s->cfg.memory_budget = fr / 10 * 9;
+    }
+    bvh_server_rec* p = s.get();
+    for (int i = 0; i < p->cfg.slots; ++i) p->threads.emplace_back([p, i] { p->run_slot(i); });
+    *out = s.release();
+    return BVH_OK;
+}
+
+int bvh_server_submit(bvh_server_t s, const bvh_request* req) {
+    if (!s || !req) return BVH_E_ARG;
+    const bool has_soa = req->soa[0] != nullptr;
+    if (has_soa) {
+        if (req->ntri == 0) return BVH_E_ARG;
+        for (int k = 0; k < 9; ++k)
+            if (!req->soa[k]) return BVH_E_ARG;
+    } else if (req->mesh_key == 0) {
+        return BVH_E_ARG;
+    }
+    if (req->nrays && (!req->rays || !req->t_out || !req->prim_out)) return BVH_E_ARG;
+    Req q;
+    q.r = *req;
+    {
+        std::lock_guard<std::mutex> lk(s->mu);
+        if (s->stop) return BVH_E_STATE;
+        s->queue.push_back(q);
+    }
+    s->cv_work.notify_all();
+    return BVH_OK;
+}
+
+int bvh_server_poll(bvh_server_t s, bvh_completion* out, uint32_t max, uint32_t* count, int32_t wait_ms) {
+    if (!s || !count || (max && !out)) return BVH_E_ARG;
+    std::unique_lock<std::mutex> lk(s->mu);
+    if (s->done.empty() && wait_ms > 0)
+        s->cv_done.wait_for(lk, std::chrono::milliseconds(wait_ms), [s] { return !s->done.empty(); });
+    uint32_t n = 0;
+    while (n < max && !s->done.empty()) {
+        out[n++] = s->done.front();
+        s->done.pop_front();
+    }
+    *count = n;
+    return BVH_OK;
+}
+
+double bvh_server_now(bvh_server_t s) { return s ? s->now() : 0.0; }
+
+int bvh_server_get_stats(bvh_server_t s, bvh_server_stats* st) {
+    if (!s || !st) return BVH_E_ARG;
+    std::lock_guard<std::mutex> lk(s->mu);
+    st->budget = s->cfg.memory_budget;
+    st->in_use = s->in_use;
+    st->peak = s->peak;
+    st->resident_bytes = s->resident_bytes;
+    st->resident_meshes = (uint32_t)s->residents.size();
+    st->queued = (uint32_t)s->queue.size();
+    st->in_flight = s->in_flight;
+    st->groups = s->groups;
+    st->completed = s->completed;
+    st->evictions = s->evictions;
+    st->rejected = s->rejected;
+    st->max_group_seen = s->max_group_seen;
+    return BVH_OK;
+}
+
+int bvh_server_drain(bvh_server_t s, int32_t wait_ms) {
+    if (!s) return BVH_E_ARG;
+    const auto until = std::chrono::steady_clock::now() + std::chrono::milliseconds(wait_ms);
+    std::unique_lock<std::mutex> lk(s->mu);
+    while (!(s->queue.empty() && s->in_flight == 0)) {
+        if (s->cv_done.wait_until(lk, until) == std::cv_status::timeout &&
+            !(s->queue.empty() && s->in_flight == 0))
+            return BVH_E_STATE;
+    }
+    return BVH_OK;
+}
+
+const char* bvh_server_last_error(bvh_server_t s) { return s ? s->err.c_str() : "null server"; }
+
+int bvh_server_destroy(bvh_server_t s) {
+    if (!s) return BVH_E_ARG;
+    {
+        std::lock_guard<std::mutex> lk(s->mu);
+        s->stop = true;
+    }
+    s->cv_work.notify_all();
+    for (auto& t : s->threads) t.join();
+    bvh_ctx_t c = s->ctxs.empty() ? nullptr : s->ctxs[0];
+    for (auto& kv : s->residents) {
+        if (kv.second->tree) bvh_tree_free(c, kv.second->tree);
+        if (kv.second->mesh) bvh_mesh_free(c, kv.second->mesh);
+    }
+    s->residents.clear();
+    for (bvh_ctx_t x : s->ctxs) bvh_ctx_destroy(x);
+    delete s;
+    return BVH_OK;
+}
+
+}  // extern "C"

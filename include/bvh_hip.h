@@ -25,6 +25,11 @@
  *                        over caller-supplied rays
  *   bvh_sort_pairs_u64   cub::DeviceRadixSort::SortPairs (lbvh_builder.cu:485-493) and
  *                        thrust::sort_by_key (lbvh_plus_builder.cu:465, ploc_builder.cu:459-461)
+ *   bvh_batch_*          no reference counterpart: the reference builds one mesh
+ *                        per builder->build call (src/batch_runner.cpp:72-120 loops
+ *                        over models and iterations); a batch builds k meshes in
+ *                        one captured launch sequence, each tree identical to
+ *                        its own bvh_build
  */
 #ifndef BVH_HIP_H
 #define BVH_HIP_H
@@ -39,6 +44,7 @@ extern "C" {
 typedef struct bvh_context* bvh_ctx_t;
 typedef struct bvh_mesh_rec* bvh_mesh_t;
 typedef struct bvh_tree_rec* bvh_tree_t;
+typedef struct bvh_batch_rec* bvh_batch_t;
 
 /* Status codes. */
 #define BVH_OK 0
@@ -59,6 +65,7 @@ typedef struct bvh_tree_rec* bvh_tree_t;
 
 /* Build flags. */
 #define BVH_BUILD_KEEP_KEYS 1u /* keep the sorted keys for bvh_debug_sorted_keys */
+#define BVH_BUILD_TUNE 2u      /* batches: time candidate launch configurations at creation, keep the fastest */
 
 typedef struct bvh_build_params {
     int32_t algo;           /* BVH_ALGO_* */
@@ -83,6 +90,8 @@ int bvh_device_count(int* count);
 int bvh_ctx_create(int device, bvh_ctx_t* out);
 int bvh_ctx_destroy(bvh_ctx_t ctx);
 const char* bvh_last_error(bvh_ctx_t ctx);
+/* Free and total device memory of the context's device (hipMemGetInfo). */
+int bvh_device_memory(bvh_ctx_t ctx, uint64_t* free_bytes, uint64_t* total_bytes);
 const char* bvh_status_string(int status);
 
 /* ---- raw device memory (used by tests, the renderer and the server) ---- */
@@ -111,6 +120,47 @@ int bvh_tree_from_nodes(bvh_ctx_t ctx, const void* nodes, uint32_t num_nodes, bv
 /* Synchronous: returns once the tree is complete; times may be NULL. */
 int bvh_build(bvh_ctx_t ctx, bvh_mesh_t mesh, const bvh_build_params* params, bvh_tree_t tree,
               bvh_stage_times* times);
+
+/* ---- batched builds ------------------------------------------------------ */
+/* A batch builds k resident meshes into k trees with one algorithm (LBVH or
+ * PLOC) in one hipGraph-captured launch sequence.  Every tree is bit-identical
+ * to the tree bvh_build gives for its mesh alone.  Meshes and trees stay owned
+ * by the caller and must outlive the batch; a tree of a batch must not be
+ * rebuilt by another call while the batch may run.  Limits: every mesh has
+ * >= 1 triangle; ceil(log2(max n)) + 30 + ceil(log2(k)) <= 64 key bits. */
+typedef struct bvh_batch_info {
+    uint32_t meshes;
+    uint32_t triangles;                /* total over the batch */
+    uint32_t max_triangles;            /* largest mesh */
+    int32_t algo;
+    int32_t radix_bits;                /* digit width in use */
+    int32_t sort_passes;
+    int32_t key_bits;                  /* sorted key bits: index | morton30 | mesh */
+    int32_t ploc_captured_iterations;  /* global PLOC iterations in the graph */
+    int32_t ploc_iterations;           /* global PLOC iterations the last run needed */
+    int32_t ploc_nn_grid;              /* persistent grid of the neighbour search */
+    int32_t ploc_merge_grid;           /* persistent grid of merge + compaction */
+    int32_t graph_nodes;               /* kernel nodes in the captured graph */
+    int32_t runs;
+    int32_t eager_continuations;       /* runs that needed iterations beyond the graph */
+    int32_t tune_count;                /* candidates timed by BVH_BUILD_TUNE */
+    float tune_ms[8];                  /* their median device times */
+    int32_t tune_radix_bits[8];
+    int32_t tune_nn_grid[8];
+} bvh_batch_info;
+
+/* The same segmented build launched directly on the context stream (no
+ * graph), for mesh sets that change from call to call (the serving loop).
+ * times (optional) covers the whole build. */
+int bvh_build_batched(bvh_ctx_t ctx, const bvh_mesh_t* meshes, const bvh_tree_t* trees, uint32_t k,
+                      const bvh_build_params* params, bvh_stage_times* times);
+
+int bvh_batch_create(bvh_ctx_t ctx, const bvh_mesh_t* meshes, const bvh_tree_t* trees, uint32_t k,
+                     const bvh_build_params* params, bvh_batch_t* out);
+/* Synchronous; ms (optional) = device time of the whole batch. */
+int bvh_batch_run(bvh_batch_t batch, float* ms);
+int bvh_batch_get_info(bvh_batch_t batch, bvh_batch_info* info);
+int bvh_batch_free(bvh_batch_t batch);
 
 /* ---- traversal ----------------------------------------------------------- */
 /* Closest hit of nrays rays (6 floats each: origin xyz, direction xyz) against

@@ -16,7 +16,8 @@ def declared(header):
     return sorted(set(DECL.findall(text)))
 
 
-@pytest.mark.parametrize("header,lib", [("bvh_hip.h", "libbvhhip.so"), ("bvh_scenes.h", "libbvhhost.so")])
+@pytest.mark.parametrize("header,lib", [("bvh_hip.h", "libbvhhip.so"), ("bvh_scenes.h", "libbvhhost.so"),
+                                        ("bvh_server.h", "libbvhhost.so")])
 def test_exports(header, lib):
     names = declared(header)
     assert len(names) > 5

@@ -1,0 +1,116 @@
+"""Serving loop on the CPU: the native admission policy (pure function of the
+C ABI, no device) and the open-loop driver's latency accounting against a
+synthetic-clock server whose schedule is known in closed form."""
+import math
+
+import numpy as np
+import pytest
+
+from bvh_amd import serving as S
+
+
+# ---- admission policy -----------------------------------------------------------
+
+def test_form_group_fifo_prefix_that_fits():
+    # FIFO prefix stops at the first request that does not fit (no skipping).
+    assert S.form_group([10, 10, 50, 1], [1, 1, 1, 1], [1, 1, 1, 1], 30, 64, 1 << 24) == 2
+
+
+def test_form_group_limits():
+    assert S.form_group([1] * 10, [1] * 10, [1] * 10, 100, 4, 1 << 24) == 4
+    assert S.form_group([1] * 5, [100, 100, 100, 100, 100], [1] * 5, 100, 64, 250) == 2
+    # an oversized head runs alone
+    assert S.form_group([1, 1], [1000, 1], [1, 1], 100, 64, 250) == 1
+
+
+def test_form_group_waits():
+    assert S.form_group([1, 1], [1, 1], [0, 1], 100, 64, 1 << 24) == 0  # head not runnable
+    assert S.form_group([200, 1], [1, 1], [1, 1], 100, 64, 1 << 24) == 0  # head does not fit
+    assert S.form_group([1, 1, 1], [1, 1, 1], [1, 0, 1], 100, 64, 1 << 24) == 1
+
+
+def test_footprint_grows_with_size():
+    t1, r1 = S.footprint(1000, 100, S.ALGO_LBVH)
+    t2, r2 = S.footprint(2000, 100, S.ALGO_LBVH)
+    t3, _ = S.footprint(1000, 100, S.ALGO_PLOC)
+    assert t2 > t1 and r2 > r1 and t3 > t1
+    # resident = mesh (SoA + AoS copy) + BVH nodes + indices
+    n = 1000
+    assert r1 == ((n + 63) // 64 * 64) * 36 + 48 * n + 48 * (2 * n - 1) + 4 * n
+
+
+# ---- driver + statistics on a synthetic clock ---------------------------------------
+
+class FakeServer:
+    """Single FIFO server with deterministic service times on a synthetic
+    clock: done_i = max(arrival_i, done_{i-1}) + service(i)."""
+
+    def __init__(self, service):
+        self.t = 0.0
+        self.service = service
+        self.free_at = 0.0
+        self.pending = []
+
+    def now(self):
+        return self.t
+
+    def wait_until(self, t):
+        self.t = max(self.t, t)
+
+    def submit(self, rid, mesh, rays, key=0, arrival=None):
+        start = max(arrival, self.free_at)
+        self.free_at = start + self.service(rid)
+        self.pending.append({"id": rid, "status": 0, "arrival": arrival, "admitted": start, "done": self.free_at})
+        return None, None
+
+    def poll(self, wait_ms=0):
+        if wait_ms:
+            self.t = max(self.t, max(c["done"] for c in self.pending) if self.pending else self.t)
+        out = [c for c in self.pending if c["done"] <= self.t]
+        self.pending = [c for c in self.pending if c["done"] > self.t]
+        return out
+
+
+def test_open_loop_latency_accounting():
+    srv = FakeServer(lambda rid: 2.0)
+    reqs = [{"id": i, "mesh": None, "rays": None, "key": 0} for i in range(4)]
+    done, _, t0 = S.run_open_loop(srv, reqs, [0.0, 1.0, 1.5, 4.0], lead=0.0)
+    lat = [c["done"] - c["arrival"] for c in done]
+    assert lat == pytest.approx([2.0, 3.0, 4.5, 4.0])
+    st = S.latency_stats(done, t0)
+    assert st["p50_ms"] == pytest.approx(3000.0)  # nearest rank: 2nd of 4
+    assert st["p99_ms"] == pytest.approx(4500.0)
+    assert st["throughput_rps"] == pytest.approx(4 / 8.0)
+
+
+def test_open_loop_md1_queue_matches_lindley():
+    rng = np.random.default_rng(5)
+    svc = rng.uniform(0.001, 0.003, 300)
+    arr = S.poisson_arrivals(400.0, 300, 23)
+    srv = FakeServer(lambda rid: float(svc[rid]))
+    reqs = [{"id": i, "mesh": None, "rays": None, "key": 0} for i in range(300)]
+    done, _, t0 = S.run_open_loop(srv, reqs, arr, lead=0.0)
+    d = 0.0
+    for i in range(300):
+        d = max(arr[i], d) + svc[i]
+        assert done[i]["done"] - t0 == pytest.approx(d)
+
+
+def test_poisson_arrivals_rate():
+    a = S.poisson_arrivals(100.0, 20000, 23)
+    assert a[0] == 0.0 and np.all(np.diff(a) >= 0)
+    assert 20000 / a[-1] == pytest.approx(100.0, rel=0.03)
+
+
+def test_log_uniform_sizes():
+    v = S.log_uniform_ints(np.random.default_rng(1), 50000, 10_000, 2_000_000)
+    assert v.min() >= 10_000 and v.max() < 2_000_000
+    # median of a log-uniform is the geometric mean of the bounds
+    assert np.median(v) == pytest.approx(math.sqrt(10_000 * 2_000_000), rel=0.05)
+
+
+def test_percentile_nearest_rank():
+    v = list(range(1, 101))
+    assert S.percentile_nearest_rank(v, 0.5) == 50
+    assert S.percentile_nearest_rank(v, 0.99) == 99
+    assert S.percentile_nearest_rank([7.0], 0.99) == 7.0

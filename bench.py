@@ -152,6 +152,51 @@ def batched_vs_plain(ctx, steps, k=512, lo=1000, hi=200_000, seed=11):
     return out
 
 
+def serving_latency(count=96, seed=23):
+    """Build+trace requests (blob meshes log-uniform 10K-2M triangles, rays
+    log-uniform 65,536-1,048,576 from the bounding sphere toward the box, 20%
+    follow-up traces on resident meshes), LBVH builds.  Capacity = closed-loop
+    saturation (all requests at t=0); then Poisson arrivals at 50% and 90% of
+    the coalescing server's capacity, served by the coalescing server (4
+    slots, groups of up to 32) and by the plain path (1 slot, one mesh per
+    launch), p50/p99 of done - arrival on the server clock."""
+    from bvh_amd import serving as S
+    log(f"[bench] serving: generating {count} requests (seed {seed})")
+    reqs = S.make_requests(seed, count)
+    warm = S.make_requests(seed + 1, 4, tri_lo=10_000, tri_hi=50_000, ray_lo=65_536, ray_hi=100_000, follow_up=0.0)
+    for q in warm:
+        q["key"] = 0
+    servers = {"coalesced": {"slots": 4, "max_group": 32}, "plain": {"slots": 1, "max_group": 1}}
+
+    def one(name, offsets):
+        with S.Server(algo=S.ALGO_LBVH, **servers[name]) as srv:
+            S.run_open_loop(srv, warm, np.zeros(len(warm)))
+            done, _, t0 = S.run_open_loop(srv, reqs, offsets)
+            st = S.latency_stats(done, t0)
+            ss = srv.stats()
+            st["groups"] = ss["groups"]
+            st["max_group"] = ss["max_group_seen"]
+            st["evictions"] = ss["evictions"]
+            st["peak_device_GB"] = round(ss["peak"] / 1e9, 2)
+            return st
+
+    out = {"requests": count, "seed": seed, "algo": "LBVH",
+           "triangles_total": int(sum(q["ntri"] for q in reqs)),
+           "rays_total": int(sum(q["rays"].shape[0] for q in reqs))}
+    for name in servers:
+        log(f"[bench] serving: capacity ({name})")
+        out[f"capacity_{name}"] = one(name, np.zeros(count))
+    cap = out["capacity_coalesced"]["throughput_rps"]
+    for load in (0.5, 0.9):
+        arr = S.poisson_arrivals(load * cap, count, seed)
+        for name in servers:
+            log(f"[bench] serving: {int(load * 100)}% load ({name})")
+            r = one(name, arr)
+            r["offered_rps"] = round(load * cap, 2)
+            out[f"load{int(load * 100)}_{name}"] = r
+    return out
+
+
 def cpu_baseline(mesh, gpu_nodes, gpu_idx):
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     import oracle as O
@@ -202,6 +247,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--lbvh-only", action="store_true", help="skip the LBVH+/PLOC/trace companions")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched-vs-plain measurement")
+    ap.add_argument("--no-serve", action="store_true", help="skip the serving-latency measurement")
     ap.add_argument("--calib", action="store_true",
                     help="only run the 256 MiB stream-copy kernel (reference point for HBM counters)")
     args = ap.parse_args()
@@ -227,6 +273,7 @@ def main():
     dm.free()
     batched = batched_vs_plain(ctx, args.steps) if not (args.lbvh_only or args.no_batch) else None
     ctx.close()
+    served = serving_latency() if not (args.lbvh_only or args.no_serve) else None
     ms_per_step = wall / args.steps * 1e3
     value = mesh.n / (ms_per_step / 1e3) / 1e6
 
@@ -272,6 +319,7 @@ def main():
         "parity_vs_oracle": parity,
         "config1_companions": extra,
         "batched_vs_plain": batched,
+        "serving": served,
     }
     print(json.dumps(out), flush=True)
 

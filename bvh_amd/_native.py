@@ -109,6 +109,7 @@ _HIP_SIGS = {
     "bvh_batch_run": (_i32, [_vp, C.POINTER(_f)]),
     "bvh_batch_get_info": (_i32, [_vp, C.POINTER(BatchInfo)]),
     "bvh_batch_free": (_i32, [_vp]),
+    "bvh_tree_prepare": (_i32, [_vp, _vp]),
     "bvh_trace": (_i32, [_vp, _vp, _vp, _vp, _u32, _vp, _vp, _vp, C.POINTER(_f)]),
     "bvh_trace_host": (_i32, [_vp, _vp, _vp, _vp, _u32, _vp, _vp, _vp, C.POINTER(_f)]),
 }

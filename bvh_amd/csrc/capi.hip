@@ -89,6 +89,8 @@ struct bvh_mesh_rec {
 
 struct bvh_tree_rec {
     DevBuf nodes, indices;
+    DevBuf rec;              // traversal records, built on first trace after a build
+    bool rec_valid = false;
     uint32_t num_nodes = 0;
     uint32_t num_prims = 0;
 };
@@ -229,7 +231,9 @@ int mark(StageClock* clk, const char* name) { return clk ? clk->mark(name) : BVH
 
 int pick_bits(int requested) {
     if (requested == 8 || requested == 10) return requested;
-    return 10;  // tuned default: 3 passes over 30 bits, 6 over 60
+    // Measured default: an 8-bit pass (256 bins) runs in about half the time
+    // of a 10-bit pass, so 4 passes over 30 bits beat 3 (and 8 over 60 beat 6).
+    return 8;
 }
 
 int ensure_tree(bvh_ctx_t ctx, bvh_tree_t tree, uint32_t n) {
@@ -237,6 +241,7 @@ int ensure_tree(bvh_ctx_t ctx, bvh_tree_t tree, uint32_t n) {
     TRY(ensure(ctx, tree->indices, (size_t)n * 4));
     tree->num_prims = n;
     tree->num_nodes = 2 * n - 1;
+    tree->rec_valid = false;
     return BVH_OK;
 }
 
@@ -868,6 +873,18 @@ int batch_tune(bvh_batch_t b) {
     return batch_capture(b);
 }
 
+// Builds the tree's traversal records once per build (complete on return, so
+// other contexts may trace the tree afterwards).
+int prepare_trace(bvh_ctx_t ctx, bvh_tree_t tree) {
+    if (tree->rec_valid) return BVH_OK;
+    TRY(ensure(ctx, tree->rec, (size_t)tree->num_nodes * 64));
+    launch_trace_layout(P<DNode>(tree->nodes), tree->num_nodes, P<float4>(tree->rec), ctx->stream);
+    TRY(check_launch(ctx, "k_trace_layout"));
+    HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+    tree->rec_valid = true;
+    return BVH_OK;
+}
+
 }  // namespace
 
 extern "C" {
@@ -1041,6 +1058,7 @@ int bvh_tree_free(bvh_ctx_t ctx, bvh_tree_t tree) {
     HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
     free_buf(tree->nodes);
     free_buf(tree->indices);
+    free_buf(tree->rec);
     delete tree;
     return BVH_OK;
 }
@@ -1074,6 +1092,7 @@ int bvh_tree_from_nodes(bvh_ctx_t ctx, const void* nodes, uint32_t num_nodes, bv
     HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
     tree->num_nodes = num_nodes;
     tree->num_prims = (num_nodes + 1) / 2;
+    tree->rec_valid = false;
     return BVH_OK;
 }
 
@@ -1210,6 +1229,13 @@ int bvh_batch_free(bvh_batch_t batch) {
     return BVH_OK;
 }
 
+int bvh_tree_prepare(bvh_ctx_t ctx, bvh_tree_t tree) {
+    if (!ctx || !tree) return BVH_E_ARG;
+    if (tree->num_nodes == 0) return set_err(ctx, BVH_E_STATE, "tree is empty");
+    HIP_TRY(ctx, hipSetDevice(ctx->device));
+    return prepare_trace(ctx, tree);
+}
+
 int bvh_trace(bvh_ctx_t ctx, bvh_tree_t tree, bvh_mesh_t mesh, const float* d_rays, uint32_t nrays, float* d_t,
               int32_t* d_prim, int32_t* d_counters, float* ms) {
     if (!ctx || !tree || !mesh || (nrays && (!d_rays || !d_t || !d_prim))) return BVH_E_ARG;
@@ -1219,8 +1245,11 @@ int bvh_trace(bvh_ctx_t ctx, bvh_tree_t tree, bvh_mesh_t mesh, const float* d_ra
     int st;
     if ((st = ensure(ctx, ctx->trace_ovf, 64))) return st;
     if ((st = ensure(ctx, ctx->trace_list, (size_t)nrays * 4))) return st;
+    HIP_TRY(ctx, hipEventRecord(ctx->ev[0], ctx->stream));
+    if ((st = prepare_trace(ctx, tree))) return st;
     TraceArgs a;
     a.nodes = P<DNode>(tree->nodes);
+    a.rec = P<float4>(tree->rec);
     a.num_nodes = tree->num_nodes;
     a.num_prims = mesh->n;
     a.tri = mesh->tri;
@@ -1231,7 +1260,6 @@ int bvh_trace(bvh_ctx_t ctx, bvh_tree_t tree, bvh_mesh_t mesh, const float* d_ra
     a.counters = d_counters;
     a.overflow_count = P<uint32_t>(ctx->trace_ovf);
     a.overflow_list = P<uint32_t>(ctx->trace_list);
-    HIP_TRY(ctx, hipEventRecord(ctx->ev[0], ctx->stream));
     launch_fill_zero(a.overflow_count, 4, ctx->stream);
     launch_trace(a, ctx->stream);
     launch_trace_deep(a, ctx->stream);

@@ -117,6 +117,7 @@ struct TraceArgs {
     uint32_t num_nodes;
     uint32_t num_prims;
     const float4* tri;          // 3 x float4 per triangle
+    const float4* rec;          // 4 x float4 per internal node (k_trace_layout)
     const float* rays;          // 6 floats per ray: origin, direction
     uint32_t nrays;
     float* t_out;
@@ -126,6 +127,8 @@ struct TraceArgs {
     uint32_t* overflow_list;    // nrays entries
 };
 void launch_tri_aos(const MeshView& m, float4* tri, hipStream_t s);
+// Per-internal-node traversal records (64 B each, indexed by node id).
+void launch_trace_layout(const DNode* nodes, uint32_t num_nodes, float4* rec, hipStream_t s);
 void launch_trace(const TraceArgs& a, hipStream_t s);
 void launch_trace_deep(const TraceArgs& a, hipStream_t s);
 

@@ -10,11 +10,15 @@
 // reference's definitions (nodes visited, AABB tests, triangle tests).
 //
 // MI355X mapping:
-//   * children's slab tests run when their parent is expanded (their boxes
-//     are read together, 2 x 32 B) and only children whose slab interval is
-//     non-empty are pushed, with their entry distance; the reference's
-//     pop-time test then reduces to tenter < t.  The visit order and every
-//     decision are unchanged;
+//   * children's slab tests run when their parent is expanded and only
+//     children whose slab interval is non-empty are pushed, with their entry
+//     distance; the reference's pop-time test then reduces to tenter < t.
+//     The visit order and every decision are unchanged;
+//   * the traversal reads a 64-byte record per internal node (both
+//     children's boxes and references, one cache-line-aligned load) built
+//     from the node array once per tree (k_trace_layout), so expanding a
+//     node costs one memory round trip instead of two dependent ones (node,
+//     then its children's boxes); leaf children carry their primitive;
 //   * the traversal stack is a short per-lane stack in LDS ([depth][lane],
 //     conflict-free), 24 entries of (node, tenter); a ray that would need a
 //     deeper stack is not truncated (the reference drops pushes past 128):
@@ -53,6 +57,62 @@ __device__ __forceinline__ float slab(const RayState& r, const DNode* nodes, int
     float texit = fmin_s(fmin_s(tmaxx, tmaxy), tmaxz);
     hit = texit >= fmax_s(tenter, 0.0f);
     return tenter;
+}
+
+__device__ __forceinline__ float slab_box(const RayState& r, const float4 mn, const float4 mx, bool& hit) {
+    float tx1 = (mn.x - r.ox) * r.ix, tx2 = (mx.x - r.ox) * r.ix;
+    float ty1 = (mn.y - r.oy) * r.iy, ty2 = (mx.y - r.oy) * r.iy;
+    float tz1 = (mn.z - r.oz) * r.iz, tz2 = (mx.z - r.oz) * r.iz;
+    float tminx = fmin_s(tx1, tx2), tmaxx = fmax_s(tx1, tx2);
+    float tminy = fmin_s(ty1, ty2), tmaxy = fmax_s(ty1, ty2);
+    float tminz = fmin_s(tz1, tz2), tmaxz = fmax_s(tz1, tz2);
+    float tenter = fmax_s(fmax_s(tminx, tminy), tminz);
+    float texit = fmin_s(fmin_s(tmaxx, tmaxy), tmaxz);
+    hit = texit >= fmax_s(tenter, 0.0f);
+    return tenter;
+}
+
+// Child reference in a traversal record: node index (>= 0), leaf
+// (0x80000000 | primitive) or absent (the reference skips out-of-range
+// children without counting them).
+constexpr int32_t kAbsent = 0x7FFFFFFF;
+
+__device__ __forceinline__ int32_t child_ref(const DNode* nodes, int32_t c, int32_t nn) {
+    if (c < 0 || c >= nn) return kAbsent;
+    const int32_t l = nodes[c].lr.x;
+    return l < 0 ? l : c;
+}
+
+// Record i (internal node i): {left min, left ref}, {left max, right ref},
+// {right min, 0}, {right max, 0}.
+__global__ void k_trace_layout(const DNode* __restrict__ nodes, int32_t nn, float4* __restrict__ rec) {
+    const int32_t stride = gridDim.x * blockDim.x;
+    for (int32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < nn; i += stride) {
+        const int4 lr = nodes[i].lr;
+        if (lr.x < 0) continue;  // leaf: no record
+        const int32_t lref = child_ref(nodes, lr.x, nn);
+        const int32_t rref = child_ref(nodes, lr.y, nn);
+        float4 lmn = make_float4(0.f, 0.f, 0.f, 0.f), lmx = lmn, rmn = lmn, rmx = lmn;
+        if (lref != kAbsent) {
+            lmn = nodes[lr.x].mn;
+            lmx = nodes[lr.x].mx;
+        }
+        if (rref != kAbsent) {
+            rmn = nodes[lr.y].mn;
+            rmx = nodes[lr.y].mx;
+        }
+        float4* o = rec + 4 * (size_t)i;
+        o[0] = make_float4(lmn.x, lmn.y, lmn.z, __int_as_float(lref));
+        o[1] = make_float4(lmx.x, lmx.y, lmx.z, __int_as_float(rref));
+        o[2] = make_float4(rmn.x, rmn.y, rmn.z, 0.f);
+        o[3] = make_float4(rmx.x, rmx.y, rmx.z, 0.f);
+    }
+}
+
+void launch_trace_layout(const DNode* nodes, uint32_t num_nodes, float4* rec, hipStream_t s) {
+    uint32_t g = (num_nodes + 255) / 256;
+    if (g > 8192) g = 8192;
+    hipLaunchKernelGGL(k_trace_layout, dim3(g ? g : 1), dim3(256), 0, s, nodes, (int32_t)num_nodes, rec);
 }
 
 // intersectTriangle (render_engine.cu:220-273); no contraction.
@@ -128,46 +188,49 @@ __device__ __forceinline__ bool traverse(const TraceArgs& a, const RayState& r, 
         ++visited;
         ++aabb;
         if (h) {
-            st.put(0, 0, te);
+            const int32_t l0 = nodes[0].lr.x;
+            st.put(0, l0 < 0 ? l0 : 0, te);
             sp = 1;
         }
     }
+    const float4* __restrict__ rec = a.rec;
     while (sp > 0) {
         if (--budget < 0) return false;
         --sp;
-        int32_t idx;
+        int32_t ref;
         float te;
-        st.get(sp, idx, te);
+        st.get(sp, ref, te);
         if (!(te < best)) continue;
-        const int4 lr = nodes[idx].lr;
-        if (lr.x < 0) {
+        if (ref < 0) {
             ++tris;
             float t;
-            const uint32_t p = (uint32_t)lr.x & 0x7FFFFFFFu;
+            const uint32_t p = (uint32_t)ref & 0x7FFFFFFFu;
             if (p < a.num_prims && hit_tri(r, a.tri, p, t) && t < best) {
                 best = t;
                 prim = (int32_t)p;
             }
             continue;
         }
-        const bool vr = lr.y >= 0 && lr.y < nn;
-        const bool vl = lr.x >= 0 && lr.x < nn;
+        const float4* q = rec + 4 * (size_t)ref;
+        const float4 A = q[0], B = q[1], C = q[2], D = q[3];
+        const int32_t lref = __float_as_int(A.w);
+        const int32_t rref = __float_as_int(B.w);
         bool hr = false, hl = false;
         float tr = 0.0f, tl = 0.0f;
-        if (vr) {
-            tr = slab(r, nodes, lr.y, hr);
+        if (rref != kAbsent) {
+            tr = slab_box(r, C, D, hr);
             ++visited;
             ++aabb;
         }
-        if (vl) {
-            tl = slab(r, nodes, lr.x, hl);
+        if (lref != kAbsent) {
+            tl = slab_box(r, A, B, hl);
             ++visited;
             ++aabb;
         }
         const int need = (hr ? 1 : 0) + (hl ? 1 : 0);
         if (sp + need > Stack::kCap) return false;
-        if (hr) st.put(sp++, lr.y, tr);
-        if (hl) st.put(sp++, lr.x, tl);
+        if (hr) st.put(sp++, rref, tr);
+        if (hl) st.put(sp++, lref, tl);
     }
     cnt[0] = visited;
     cnt[1] = aabb;

@@ -319,6 +319,14 @@ void bvh_server_rec::run_slot(int slot) {
                 for (size_t i : who) status[i] = st;
             }
         }
+        // Resident trees get their traversal records before other slots may use them.
+        for (size_t i = 0; i < group.size(); ++i) {
+            Req& q = group[i];
+            if (status[i] == BVH_OK && q.build && q.res) {
+                int st = bvh_tree_prepare(ctx, q.tree);
+                if (st != BVH_OK) status[i] = st;
+            }
+        }
         // 3. each request's rays against its own tree
         for (size_t i = 0; i < group.size(); ++i) {
             if (status[i] != BVH_OK) continue;

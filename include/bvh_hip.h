@@ -71,7 +71,7 @@ typedef struct bvh_build_params {
     int32_t algo;           /* BVH_ALGO_* */
     int32_t ploc_radius;    /* PLOC window radius, >= 1 (reference default 25) */
     int32_t treelet_rounds; /* LBVH+ optimisation rounds, >= 0 (reference: 1) */
-    int32_t radix_bits;     /* sort digit width: 0 = tuned default, or 8 / 10 / 11; result-invariant */
+    int32_t radix_bits;     /* sort digit width: 0 = measured default (8), or 8 / 10; result-invariant */
     uint32_t flags;         /* BVH_BUILD_* */
 } bvh_build_params;
 
@@ -163,6 +163,10 @@ int bvh_batch_get_info(bvh_batch_t batch, bvh_batch_info* info);
 int bvh_batch_free(bvh_batch_t batch);
 
 /* ---- traversal ----------------------------------------------------------- */
+/* Builds the tree's 64-byte-per-node traversal records now (the first trace
+ * after a build does it otherwise).  A tree may be traced from several
+ * contexts at once only after this (or one trace) has returned. */
+int bvh_tree_prepare(bvh_ctx_t ctx, bvh_tree_t tree);
 /* Closest hit of nrays rays (6 floats each: origin xyz, direction xyz) against
  * `tree` over `mesh`, all buffers in device memory.  t_out[i] = hit distance
  * (1e30 when missed), prim_out[i] = original triangle index or -1;

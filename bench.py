@@ -152,6 +152,37 @@ def batched_vs_plain(ctx, steps, k=512, lo=1000, hi=200_000, seed=11):
     return out
 
 
+def large_strands(ctx, steps=2):
+    """32,000,000 triangles: 200,000 random-walk strands x 160 segments (step
+    0.01, seed 99); PLOC r=100 and LBVH+ (3 rounds) builds, then 4,194,304
+    incoherent rays (uniform origins in the cube, uniform directions) traced
+    through each tree."""
+    import bvh_amd as B
+    log("[bench] large: generating 200,000 strands x 160 segments (seed 99)")
+    mesh = B.Mesh.strands(99, 200_000, 160)
+    dm = ctx.upload(mesh)
+    tree = ctx.tree()
+    rays = B.rays_uniform_cube(99, 4_194_304)
+    out = {"triangles": mesh.n, "rays": int(rays.shape[0])}
+    for name, kw in (("ploc_r100", {"algo": B.ALGO_PLOC, "radius": 100}),
+                     ("lbvh_plus_3_rounds", {"algo": B.ALGO_LBVH_PLUS, "rounds": 3})):
+        log(f"[bench] large: {name}")
+        wall, dev, st, iters = timed_builds(ctx, dm, tree, steps, 1, **kw)
+        r = {"ms_per_build": round(wall / steps * 1e3, 2), "device_ms": round(dev, 2),
+             "Mtri_s": round(mesh.n / (wall / steps) / 1e6, 1), "stages_ms": {k: round(v, 3) for k, v in st.items()}}
+        if iters:
+            r["iterations"] = iters
+        log(f"[bench] large: trace through {name}")
+        t, prim, ms = ctx.trace(tree, dm, rays)
+        r["trace_device_ms"] = round(ms, 2)
+        r["trace_Mrays_s"] = round(rays.shape[0] / ms / 1e3, 2)
+        r["hit_fraction"] = round(float((prim >= 0).mean()), 4)
+        out[name] = r
+    tree.free()
+    dm.free()
+    return out
+
+
 def serving_latency(count=96, seed=23):
     """Build+trace requests (blob meshes log-uniform 10K-2M triangles, rays
     log-uniform 65,536-1,048,576 from the bounding sphere toward the box, 20%
@@ -248,6 +279,7 @@ def main():
     ap.add_argument("--lbvh-only", action="store_true", help="skip the LBVH+/PLOC/trace companions")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched-vs-plain measurement")
     ap.add_argument("--no-serve", action="store_true", help="skip the serving-latency measurement")
+    ap.add_argument("--no-large", action="store_true", help="skip the 32M-triangle strands workload")
     ap.add_argument("--calib", action="store_true",
                     help="only run the 256 MiB stream-copy kernel (reference point for HBM counters)")
     args = ap.parse_args()
@@ -272,6 +304,7 @@ def main():
     tree.free()
     dm.free()
     batched = batched_vs_plain(ctx, args.steps) if not (args.lbvh_only or args.no_batch) else None
+    large = large_strands(ctx) if not (args.lbvh_only or args.no_large) else None
     ctx.close()
     served = serving_latency() if not (args.lbvh_only or args.no_serve) else None
     ms_per_step = wall / args.steps * 1e3
@@ -320,6 +353,7 @@ def main():
         "config1_companions": extra,
         "batched_vs_plain": batched,
         "serving": served,
+        "large_32m_strands": large,
     }
     print(json.dumps(out), flush=True)
 

@@ -20,12 +20,14 @@
 // second merges the boxes (left box first, as the reference merges
 // nodes[left] with nodes[right]) and climbs on.
 //
-// MI355X mapping: a workgroup owns a chunk of 2048 sorted leaves.  Every split
+// MI355X mapping: a workgroup owns a chunk of 1024 sorted leaves.  Every split
 // whose parent lies entirely inside the chunk (decided from prefix/suffix
 // minima of the chunk's delta values, which both children evaluate
-// identically) hands off through an LDS word with workgroup-scope ordering.
+// identically) hands off through an LDS word with workgroup-scope ordering;
+// the word names the depositing chain, whose subtree box waits in LDS, so
+// the second arrival never re-reads a node record from memory.
 // Only nodes that straddle chunk boundaries (the top of the tree, roughly
-// 2n/2048 nodes) hand off through an epoch-tagged 64-bit word in HBM; their
+// 2n/1024 nodes) hand off through an epoch-tagged 64-bit word in HBM; their
 // records are written with agent-scope (sc1) atomic stores and read with
 // agent-scope loads, with a full wait before the exchange, so no L2 writeback
 // or invalidate is ever needed.  Epoch tags mean the handoff array is never
@@ -37,7 +39,7 @@
 namespace bvhk {
 
 constexpr int kHierBlock = 256;
-constexpr int kHierItems = 8;
+constexpr int kHierItems = 4;
 constexpr int kChunk = kHierBlock * kHierItems;
 
 int hier_chunk() { return kChunk; }
@@ -129,7 +131,8 @@ __global__ __launch_bounds__(kHierBlock) void k_hierarchy(HierArgs p) {
     __shared__ int8_t s_d[kChunk + 1];    // delta of boundaries a-1 .. b-1
     __shared__ int8_t s_pre[kChunk + 1];  // prefix minima of s_d
     __shared__ int8_t s_suf[kChunk + 1];  // suffix minima of s_d
-    __shared__ uint32_t s_slot[kChunk];   // LDS handoff words (far end + 1)
+    __shared__ uint32_t s_slot[kChunk];   // LDS handoff words: (far end - a + 1) | depositing chain << 16
+    __shared__ float s_box[6][kChunk];    // each chain's current box, read by its partner
     __shared__ int s_w[kHierBlock / kWave];
 
     // This block's mesh (segment) and chunk; everything below is in the
@@ -174,7 +177,8 @@ __global__ __launch_bounds__(kHierBlock) void k_hierarchy(HierArgs p) {
 
     DNode* __restrict__ nodes = d.nodes;
     for (int k = 0; k < kHierItems; ++k) {
-        const uint32_t i = a + k * kHierBlock + tid;
+        const int chain = k * kHierBlock + tid;
+        const uint32_t i = a + chain;
         if (i >= b) continue;
         uint32_t prim = K60 ? vals[i] : (uint32_t)(keys[i] & idx_mask);
         if (prim >= n) {  // cannot happen for a correct sort; never index out of bounds
@@ -215,13 +219,20 @@ __global__ __launch_bounds__(kHierBlock) void k_hierarchy(HierArgs p) {
             Box sbox;
             uint32_t sib;
             if (lds) {
-                uint32_t old = __hip_atomic_exchange(&s_slot[s - a], far + 1, __ATOMIC_ACQ_REL,
+                // The partner reads this chain's box from LDS, not from the
+                // node record: no global round trip inside the chunk.
+                s_box[0][chain] = box.mnx; s_box[1][chain] = box.mny; s_box[2][chain] = box.mnz;
+                s_box[3][chain] = box.mxx; s_box[4][chain] = box.mxy; s_box[5][chain] = box.mxz;
+                const uint32_t word = (far - a + 1u) | ((uint32_t)chain << 16);
+                uint32_t old = __hip_atomic_exchange(&s_slot[s - a], word, __ATOMIC_ACQ_REL,
                                                      __HIP_MEMORY_SCOPE_WORKGROUP);
                 if (old == 0) break;
-                other = old - 1;
+                other = a + (old & 0xFFFFu) - 1u;
+                const int pc = (int)(old >> 16);
                 if (isLeft) sib = (s + 1 == other) ? (n - 1 + other) : (s + 1);
                 else sib = (other == s) ? (n - 1 + s) : s;
-                sbox = node_box(nodes, sib);
+                sbox.mnx = s_box[0][pc]; sbox.mny = s_box[1][pc]; sbox.mnz = s_box[2][pc];
+                sbox.mxx = s_box[3][pc]; sbox.mxy = s_box[4][pc]; sbox.mxz = s_box[5][pc];
             } else {
                 local = false;
                 uint64_t w = ((uint64_t)epoch << 32) | far;

@@ -17,15 +17,19 @@
 // Every float operation matches the oracle's serial transcription in order, so
 // the resulting trees are bit-identical to it.
 //
-// MI355X mapping: threads climb from the leaves as in the reference, but a
-// treelet is processed by its whole wavefront: the lanes that became second
-// arrivals are served one at a time, each treelet's subset boxes, costs and
-// partitions live in the wave's private LDS slice and every DP level runs with
-// one lane per subset.  A workgroup owns 2048 sorted leaves; a node whose leaf
-// span lies inside the chunk hands off through a workgroup-scope counter (no
-// cross-XCD coherence traffic), only chunk-straddling nodes use agent scope.
-// Leaf spans are kept per internal node (min/max sorted leaf position) and
-// updated when a treelet reassigns its internal nodes.
+// MI355X mapping: threads climb from the leaves as in the reference; at each
+// climbing step, every lane that became a second arrival forms its treelet
+// at once in its own registers (seven dependent rounds of node loads, issued
+// by all such lanes together), then the wave runs the dynamic programmes of
+// up to eight treelets side by side on equal lane groups (64 lanes for a
+// lone treelet, 8 each for eight), every subset size spread over the group
+// with partitions split across lanes and reduced in enumeration order, and
+// each owner lane rebuilds its treelet from the group's partition table.  A workgroup owns 2048 sorted
+// leaves; a node whose leaf span lies inside the chunk hands off through a
+// workgroup-scope counter (no cross-XCD coherence traffic), only
+// chunk-straddling nodes use agent scope.  Leaf spans are kept per internal
+// node (min/max sorted leaf position) and updated when a treelet reassigns
+// its internal nodes.
 #include "dev_common.hpp"
 #include "engine.hpp"
 
@@ -36,24 +40,33 @@ constexpr int kTItems = 8;
 constexpr int kTChunk = kTBlock * kTItems;
 constexpr int kTWaves = kTBlock / kWave;
 constexpr int kTMax = 7;
+constexpr int kGroups = 8;               // treelets per wave per round (groups of >= 8 lanes)
+constexpr float kCostInf = 1e30f;
 
-struct WaveLds {
-    float box[128][6];   // subset boxes (only needed for reconstruction)
+// One treelet in LDS: formed by its owner lane, optimised by one 8-lane
+// group, rebuilt by the owner.
+struct Slot {
+    float lb[kTMax][6];      // treelet leaf boxes, in treelet order
     float cost[128];
     uint8_t part[128];
-    int32_t act[8];      // treelet leaves (node ids)
-    float abox[8][6];    // their boxes
-    int32_t alo[8];      // their leaf spans
-    int32_t ahi[8];
-    int32_t exp_node[8]; // expanded (internal) nodes and their children, in order
-    int32_t exp_l[8];
-    int32_t exp_r[8];
-    int32_t avail[8];    // reconstruction scratch of the owner lane
-    int32_t stack[10];
+    int32_t act[kTMax];      // treelet leaves (node ids)
+    int32_t lc[kTMax];       // their children (when internal)
+    int32_t rc[kTMax];
+    int32_t lo[kTMax];       // their sorted-leaf spans
+    int32_t hi[kTMax];
+    int32_t exn[kTMax - 1];  // expanded internal nodes and their children, in order
+    int32_t exl[kTMax - 1];
+    int32_t exr[kTMax - 1];
+    int32_t n;
+    int32_t nexp;
+    int32_t avail[8];        // reconstruction scratch
+    int32_t stack[8];
     int32_t qmask[8];
     int32_t qnode[8];
-    int32_t cnt;
-    int32_t nexp;
+};
+
+struct WaveLds {
+    Slot slot[kGroups];
 };
 
 __device__ __forceinline__ void wave_sync() {
@@ -73,38 +86,66 @@ __device__ __forceinline__ void lds_put(float* f, const Box& b) {
     f[3] = b.mxx; f[4] = b.mxy; f[5] = b.mxz;
 }
 
+
+__device__ __forceinline__ Box node_box_of(const DNode& d) {
+    Box b;
+    b.mnx = d.mn.x; b.mny = d.mn.y; b.mnz = d.mn.z;
+    b.mxx = d.mx.x; b.mxy = d.mx.y; b.mxz = d.mx.z;
+    return b;
+}
+
+__device__ __forceinline__ void nodes_refit(const TreeletArgs& a, int32_t root, const Box& b) {
+    a.nodes[root].mn = make_float4(b.mnx, b.mny, b.mnz, 0.0f);
+    a.nodes[root].mx = make_float4(b.mxx, b.mxy, b.mxz, 0.0f);
+}
+
+
 __device__ __forceinline__ int2 leaf_span(const TreeletArgs& a, int32_t node) {
     const uint32_t first_leaf = a.n - 1;
     if ((uint32_t)node >= first_leaf) return make_int2((int)(node - first_leaf), (int)(node - first_leaf));
     return a.ranges[node];
 }
 
-// Refit + optimise the treelet rooted at `root`.  Called by the whole wave
-// (uniform control flow); `owner` is the lane that climbs on from `root`.
-__device__ void process_treelet(const TreeletArgs& a, WaveLds& w, int32_t root, int lane, int owner) {
-    DNode* __restrict__ nodes = a.nodes;
-    // Refit (the reference's initial union before optimising).
+// Union of the subset's leaf boxes in ascending index order from the empty
+// box (the oracle's order: min/max by selects are order-sensitive only for
+// signed zeros, and this keeps even those identical).
+template <class BoxAt>
+__device__ __forceinline__ Box subset_box(BoxAt at, int s) {
+    Box b = empty_box();
+#pragma unroll
+    for (int i = 0; i < kTMax; ++i)
+        if ((s >> i) & 1) b = merge(b, at(i));
+    return b;
+}
+
+// Greedy formation (reference lbvh_plus_builder.cu:79-99): expand the
+// treelet leaf of largest area that is not a BVH leaf (first maximum wins)
+// until 7 leaves.  Each act leaf's children come with its node record, so
+// an expansion costs one round of loads.  Returns the refitted root box
+// (the reference's union of the root's children) and the leaf count.
+__device__ int form(const TreeletArgs& a, int32_t root, Slot& sl, Box& rootbox) {
+    const DNode* __restrict__ nodes = a.nodes;
     const int4 rlr = nodes[root].lr;
-    const Box rb = merge(node_box(nodes, rlr.x), node_box(nodes, rlr.y));
-    if (lane == owner) {
-        nodes[root].mn = make_float4(rb.mnx, rb.mny, rb.mnz, 0.0f);
-        nodes[root].mx = make_float4(rb.mxx, rb.mxy, rb.mxz, 0.0f);
-    }
-    // Formation: serial on lane 0, results in LDS.
-    if (lane == 0) {
-        w.act[0] = root;
-        lds_put(w.abox[0], rb);
-        int2 sp = a.ranges[root];
-        w.alo[0] = sp.x;
-        w.ahi[0] = sp.y;
-        int cnt = 1, nexp = 0;
-        uint32_t leafmask = 0;  // bit i: treelet leaf i is a BVH leaf
-        for (int step = 0; step < kTMax - 1; ++step) {
-            int best = -1;
+    const int2 rsp = a.ranges[root];
+    sl.act[0] = root;
+    sl.lc[0] = rlr.x;
+    sl.rc[0] = rlr.y;
+    sl.lo[0] = rsp.x;
+    sl.hi[0] = rsp.y;
+    uint32_t leafmask = 0;
+    int n = 1, nexp = 0;
+    DNode cl = nodes[rlr.x];
+    DNode cr = nodes[rlr.y];
+    rootbox = merge(node_box_of(cl), node_box_of(cr));
+    lds_put(sl.lb[0], rootbox);
+    int best = 0;
+    for (int step = 0; step < kTMax - 1; ++step) {
+        if (step > 0) {
+            best = -1;
             float maxA = -1.0f;
-            for (int i = 0; i < cnt; ++i) {
+            for (int i = 0; i < n; ++i) {
                 if (!((leafmask >> i) & 1u)) {
-                    float ar = area(lds_box(w.abox[i]));
+                    const float ar = area(lds_box(sl.lb[i]));
                     if (ar > maxA) {
                         maxA = ar;
                         best = i;
@@ -112,134 +153,208 @@ __device__ void process_treelet(const TreeletArgs& a, WaveLds& w, int32_t root, 
                 }
             }
             if (best < 0) break;
-            const int32_t ex = w.act[best];
-            const int4 lr = nodes[ex].lr;
-            w.exp_node[nexp] = ex;
-            w.exp_l[nexp] = lr.x;
-            w.exp_r[nexp] = lr.y;
-            ++nexp;
-            const DNode nl = nodes[lr.x];
-            const DNode nr = nodes[lr.y];
-            w.act[best] = lr.x;
-            lds_put(w.abox[best], load_box(reinterpret_cast<const float4*>(&nl)));
-            leafmask = (leafmask & ~(1u << best)) | (((nl.lr.x & 0x80000000) != 0 ? 1u : 0u) << best);
-            int2 s1 = leaf_span(a, lr.x);
-            w.alo[best] = s1.x;
-            w.ahi[best] = s1.y;
-            w.act[cnt] = lr.y;
-            lds_put(w.abox[cnt], load_box(reinterpret_cast<const float4*>(&nr)));
-            leafmask |= ((nr.lr.x & 0x80000000) != 0 ? 1u : 0u) << cnt;
-            int2 s2 = leaf_span(a, lr.y);
-            w.alo[cnt] = s2.x;
-            w.ahi[cnt] = s2.y;
-            ++cnt;
+            cl = nodes[sl.lc[best]];
+            cr = nodes[sl.rc[best]];
+        } else if (!(area(rootbox) > -1.0f)) {
+            break;  // the reference's argmax finds nothing (non-finite area)
         }
-        w.cnt = cnt;
-        w.nexp = nexp;
+        const int32_t l = sl.lc[best], r = sl.rc[best];
+        const int2 sp_l = leaf_span(a, l), sp_r = leaf_span(a, r);
+        sl.exn[nexp] = sl.act[best];
+        sl.exl[nexp] = l;
+        sl.exr[nexp] = r;
+        ++nexp;
+        sl.act[best] = l;
+        lds_put(sl.lb[best], node_box_of(cl));
+        sl.lc[best] = cl.lr.x;
+        sl.rc[best] = cl.lr.y;
+        sl.lo[best] = sp_l.x;
+        sl.hi[best] = sp_l.y;
+        leafmask = (leafmask & ~(1u << best)) | ((cl.lr.x < 0 ? 1u : 0u) << best);
+        sl.act[n] = r;
+        lds_put(sl.lb[n], node_box_of(cr));
+        sl.lc[n] = cr.lr.x;
+        sl.rc[n] = cr.lr.y;
+        sl.lo[n] = sp_r.x;
+        sl.hi[n] = sp_r.y;
+        leafmask |= (cr.lr.x < 0 ? 1u : 0u) << n;
+        ++n;
     }
-    wave_sync();
-    const int n = w.cnt;
-    if (n < 2) return;
-    const int full = (1 << n) - 1;
-    // Subset boxes in the reference's ascending-index union order, singleton costs.
-    for (int s = lane + 1; s <= full; s += kWave) {
-        Box b = empty_box();
-        for (int i = 0; i < n; ++i)
-            if ((s >> i) & 1) b = merge(b, lds_box(w.abox[i]));
-        lds_put(w.box[s], b);
-        w.cost[s] = (__popc(s) == 1) ? 0.0f : 1e30f;
+    sl.n = n;
+    sl.nexp = nexp;
+    return n;
+}
+
+// Cost and best partition of subset s (reference lbvh_plus_builder.cu:131-160):
+// partitions in the reference's enumeration order, strict '<'.
+__device__ __forceinline__ void dp_subset(Slot& sl, const Box (&lb)[kTMax], int s) {
+    const Box b = subset_box([&](int i) { return lb[i]; }, s);
+    const float areaS = area(b);
+    float minCost = kCostInf;
+    int bestP = -1;
+    const int delta = (s - 1) & s;
+    int p = (-delta) & s;
+    do {
+        const float c = sl.cost[p] + sl.cost[s ^ p];
+        if (c < minCost) {
+            minCost = c;
+            bestP = p;
+        }
+        p = (p - delta) & s;
+    } while (p != 0);
+    sl.cost[s] = minCost + 1.0f * areaS;
+    sl.part[s] = (uint8_t)bestP;
+}
+
+// Subsets of 7 leaves grouped by size, increasing within a size (so the
+// subsets of n < 7 leaves are a prefix of each group), and binomials.
+__constant__ uint8_t c_sub[127] = {
+    1, 2, 4, 8, 16, 32, 64, 3, 5, 6, 9, 10, 12, 17, 18, 20, 24, 33, 34, 36, 40, 48, 65, 66, 68, 72, 80, 96,
+    7, 11, 13, 14, 19, 21, 22, 25, 26, 28, 35, 37, 38, 41, 42, 44, 49, 50, 52, 56, 67, 69, 70, 73, 74, 76, 81,
+    82, 84, 88, 97, 98, 100, 104, 112, 15, 23, 27, 29, 30, 39, 43, 45, 46, 51, 53, 54, 57, 58, 60, 71, 75, 77,
+    78, 83, 85, 86, 89, 90, 92, 99, 101, 102, 105, 106, 108, 113, 114, 116, 120, 31, 47, 55, 59, 61, 62, 79,
+    87, 91, 93, 94, 103, 107, 109, 110, 115, 117, 118, 121, 122, 124, 63, 95, 111, 119, 123, 125, 126, 127};
+__constant__ uint8_t c_subk[9] = {0, 0, 7, 28, 63, 98, 119, 126, 127};
+__constant__ uint8_t c_binom[8][8] = {{1, 0, 0, 0, 0, 0, 0, 0},    {1, 1, 0, 0, 0, 0, 0, 0},
+                                      {1, 2, 1, 0, 0, 0, 0, 0},    {1, 3, 3, 1, 0, 0, 0, 0},
+                                      {1, 4, 6, 4, 1, 0, 0, 0},    {1, 5, 10, 10, 5, 1, 0, 0},
+                                      {1, 6, 15, 20, 15, 6, 1, 0}, {1, 7, 21, 35, 35, 21, 7, 1}};
+
+// The t-th (0-based) nonzero submask of `mask` in increasing order: the
+// reference's partition enumeration p -> (p - delta) & s visits exactly these.
+__device__ __forceinline__ int submask(int t, int mask) {
+    int v = t + 1, r = 0;
+    for (int m = mask; m != 0; m &= m - 1) {
+        if (v & 1) r |= m & -m;
+        v >>= 1;
     }
+    return r;
+}
+
+// The dynamic programme of one treelet on a group of G lanes (G a power of
+// two, 8..64; sub = lane in group).  Subsets of one size are independent:
+// each gets L lanes (the largest power of two with L * count <= G), which
+// split its partitions by enumeration position and reduce by (cost,
+// position) -- the reference's first-minimum rule; with L = 1 a lane takes
+// whole subsets.
+__device__ void dp_group(Slot& sl, int sub, int G) {
+    const int n = sl.n;
+    Box lb[kTMax];
+#pragma unroll
+    for (int i = 0; i < kTMax; ++i) lb[i] = lds_box(sl.lb[i]);
+    if (sub < n) sl.cost[1 << sub] = 0.0f;
     wave_sync();
     for (int k = 2; k <= n; ++k) {
-        for (int s = lane + 1; s <= full; s += kWave) {
-            if (__popc(s) != k) continue;
-            const float areaS = area(lds_box(w.box[s]));
-            float minCost = 1e30f;
-            int bestP = -1;
-            const int delta = (s - 1) & s;
-            int p = (-delta) & s;
-            do {
-                const float c = w.cost[p] + w.cost[s ^ p];
-                if (c < minCost) {
-                    minCost = c;
-                    bestP = p;
+        const int cnt = c_binom[n][k];
+        int L = 1;
+        while (L * 2 * cnt <= G) L *= 2;
+        if (L == 1) {
+            for (int q = sub; q < cnt; q += G) dp_subset(sl, lb, c_sub[c_subk[k] + q]);
+        } else {
+            const int q = sub / L, j = sub & (L - 1);
+            float best = kCostInf;
+            int bestP = -1, bestT = 1 << 30;
+            int s = 0;
+            float areaS = 0.0f;
+            if (q < cnt) {
+                s = c_sub[c_subk[k] + q];
+                areaS = area(subset_box([&](int i) { return lb[i]; }, s));
+                const int delta = (s - 1) & s;
+                const int m = (1 << __popc(delta)) - 1;
+                for (int t = j; t < m; t += L) {
+                    const int p = submask(t, delta);
+                    const float c = sl.cost[p] + sl.cost[s ^ p];
+                    if (c < best) {
+                        best = c;
+                        bestP = p;
+                        bestT = t;
+                    }
                 }
-                p = (p - delta) & s;
-            } while (p != 0);
-            w.cost[s] = minCost + 1.0f * areaS;
-            w.part[s] = (uint8_t)bestP;
+            }
+            for (int off = L / 2; off >= 1; off >>= 1) {
+                const float ob = __shfl_xor(best, off, L);
+                const int op = __shfl_xor(bestP, off, L);
+                const int ot = __shfl_xor(bestT, off, L);
+                if (ob < best || (ob == best && ot < bestT)) {
+                    best = ob;
+                    bestP = op;
+                    bestT = ot;
+                }
+            }
+            if (q < cnt && j == 0) {
+                sl.cost[s] = best + 1.0f * areaS;
+                sl.part[s] = (uint8_t)bestP;
+            }
         }
         wave_sync();
     }
-    // Reconstruction on the owner lane.
-    if (lane == owner) {
-        int32_t* avail = w.avail;
-        int navail = 0;
-        int32_t* stack = w.stack;
-        int top = 0;
-        stack[top++] = root;
-        while (top > 0) {
-            const int32_t curr = stack[--top];
-            bool is_tleaf = false;
-            for (int i = 0; i < n; ++i)
-                if (w.act[i] == curr) {
-                    is_tleaf = true;
-                    break;
-                }
-            if (is_tleaf) continue;
-            avail[navail++] = curr;
-            int32_t cl = -1, cr = -1;
-            for (int e = 0; e < w.nexp; ++e)
-                if (w.exp_node[e] == curr) {
-                    cl = w.exp_l[e];
-                    cr = w.exp_r[e];
-                }
-            stack[top++] = cl;
-            stack[top++] = cr;
-        }
-        int32_t* qmask = w.qmask;
-        int32_t* qnode = w.qnode;
-        int qh = 0, qt = 0, used = 1;
-        qmask[qt] = full;
-        qnode[qt] = avail[0];
-        ++qt;
-        while (qh < qt) {
-            const int mask = qmask[qh];
-            const int32_t cur = qnode[qh];
-            ++qh;
-            const int lm = w.part[mask];
-            const int rm = mask ^ lm;
-            int32_t ln, rn;
-            if (__popc(lm) == 1) {
-                ln = w.act[__ffs(lm) - 1];
-            } else {
-                ln = avail[used++];
-                qmask[qt] = lm;
-                qnode[qt] = ln;
-                ++qt;
+}
+
+// Reconstruction (reference lbvh_plus_builder.cu:162-245): the treelet's
+// internal node ids in DFS pop order (right child popped first), assigned
+// breadth first from the partition table; boxes are subset unions.
+__device__ void rebuild(const TreeletArgs& a, Slot& sl, int32_t root) {
+    DNode* __restrict__ nodes = a.nodes;
+    const int n = sl.n;
+    const int nexp = sl.nexp;
+    const int full = (1 << n) - 1;
+    int navail = 0, top = 0;
+    sl.stack[top++] = root;
+    while (top > 0) {
+        const int32_t curr = sl.stack[--top];
+        bool is_leaf = false;
+        for (int i = 0; i < n; ++i) is_leaf |= (sl.act[i] == curr);
+        if (is_leaf) continue;
+        sl.avail[navail++] = curr;
+        int32_t cl = -1, cr = -1;
+        for (int e = 0; e < nexp; ++e)
+            if (sl.exn[e] == curr) {
+                cl = sl.exl[e];
+                cr = sl.exr[e];
             }
-            if (__popc(rm) == 1) {
-                rn = w.act[__ffs(rm) - 1];
-            } else {
-                rn = avail[used++];
-                qmask[qt] = rm;
-                qnode[qt] = rn;
-                ++qt;
-            }
-            const Box b = lds_box(w.box[mask]);
-            store_node(nodes, (uint32_t)cur, b, ln, rn);
-            a.parents[ln] = (uint32_t)cur;
-            a.parents[rn] = (uint32_t)cur;
-            int lo = 0x7fffffff, hi = -1;
-            for (int i = 0; i < n; ++i)
-                if ((mask >> i) & 1) {
-                    lo = w.alo[i] < lo ? w.alo[i] : lo;
-                    hi = w.ahi[i] > hi ? w.ahi[i] : hi;
-                }
-            a.ranges[cur] = make_int2(lo, hi);
-        }
+        sl.stack[top++] = cl;
+        sl.stack[top++] = cr;
     }
-    wave_sync();
+    int qh = 0, qt = 0, used = 1;
+    sl.qmask[qt] = full;
+    sl.qnode[qt] = sl.avail[0];
+    ++qt;
+    while (qh < qt) {
+        const int mask = sl.qmask[qh];
+        const int32_t cur = sl.qnode[qh];
+        ++qh;
+        const int lm = sl.part[mask];
+        const int rm = mask ^ lm;
+        int32_t ln, rn;
+        if (__popc(lm) == 1) {
+            ln = sl.act[__ffs(lm) - 1];
+        } else {
+            ln = sl.avail[used++];
+            sl.qmask[qt] = lm;
+            sl.qnode[qt] = ln;
+            ++qt;
+        }
+        if (__popc(rm) == 1) {
+            rn = sl.act[__ffs(rm) - 1];
+        } else {
+            rn = sl.avail[used++];
+            sl.qmask[qt] = rm;
+            sl.qnode[qt] = rn;
+            ++qt;
+        }
+        Box b = empty_box();
+        int lo = 0x7fffffff, hi = -1;
+        for (int i = 0; i < n; ++i)
+            if ((mask >> i) & 1) {
+                b = merge(b, lds_box(sl.lb[i]));
+                lo = sl.lo[i] < lo ? sl.lo[i] : lo;
+                hi = sl.hi[i] > hi ? sl.hi[i] : hi;
+            }
+        store_node(nodes, (uint32_t)cur, b, ln, rn);
+        a.parents[ln] = (uint32_t)cur;
+        a.parents[rn] = (uint32_t)cur;
+        a.ranges[cur] = make_int2(lo, hi);
+    }
 }
 
 __global__ __launch_bounds__(kTBlock) void k_treelet(TreeletArgs a) {
@@ -272,13 +387,32 @@ __global__ __launch_bounds__(kTBlock) void k_treelet(TreeletArgs a) {
                 if (old == tag) pending = true;
                 else active = false;
             }
-            uint64_t mask = __ballot(pending);
             if (__ballot(global && pending) != 0) __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
-            while (mask) {
-                const int owner = __ffsll((unsigned long long)mask) - 1;
-                mask &= mask - 1;
-                const int32_t root = __shfl((int)par, owner, kWave);
-                process_treelet(a, w, root, lane, owner);
+            // Ready treelets, up to eight per round: formed by their owners
+            // into LDS slots, optimised by 8-lane groups, rebuilt by owners.
+            bool todo = pending;
+            while (true) {
+                const uint64_t tmask = __ballot(todo);
+                if (tmask == 0) break;
+                const int rank = todo ? __popcll(tmask & lanemask_lt()) : kGroups;
+                const bool mine = rank < kGroups;
+                Slot& ms = w.slot[mine ? rank : 0];
+                Box rootbox;
+                int tn = 0;
+                if (mine) tn = form(a, (int32_t)par, ms, rootbox);
+                wave_sync();
+                // T treelets this round, each on a group of G lanes.
+                const int T = __popcll(tmask) < kGroups ? __popcll(tmask) : kGroups;
+                const int G = T == 1 ? 64 : (T == 2 ? 32 : (T <= 4 ? 16 : 8));
+                const int gi = lane / G, sub = lane & (G - 1);
+                if (gi < T && w.slot[gi].n >= 2) dp_group(w.slot[gi], sub, G);
+                wave_sync();
+                if (mine) {
+                    if (tn >= 2) rebuild(a, ms, (int32_t)par);
+                    else nodes_refit(a, (int32_t)par, rootbox);  // nothing to optimise
+                    todo = false;
+                }
+                wave_sync();
             }
             if (pending) {
                 cur = par;

@@ -51,6 +51,10 @@ void launch_init_build(uint32_t* seg_scene, uint32_t k, uint32_t* epoch, hipStre
 // Per-triangle boxes (2 x float4 per triangle) and each segment's scene box.
 void launch_tri_bounds(const SegBlocks& sb, float4* tri_boxes, uint32_t* seg_scene, hipStream_t s);
 
+// Onesweep tile: threads x keys per thread.
+constexpr int kSortThreads = 256;
+constexpr int kSortItems = 32;
+
 // Sort plan: digit width, pass count and shifts for a key bit range.
 struct SortPlan {
     int bits;          // digit width (8 or 10)

@@ -130,7 +130,7 @@ SortPlan make_sort_plan(uint32_t n, int begin_bit, int end_bit, int bits, int hi
         w = w < 0 ? 0 : (w > bits ? bits : w);
         p.mask[i] = (1u << w) - 1u;
     }
-    p.tile = 256 * 16;
+    p.tile = kSortThreads * kSortItems;
     p.tiles = (n + p.tile - 1) / p.tile;
     if (p.tiles < 1) p.tiles = 1;
     if (hist_blocks > 0) {

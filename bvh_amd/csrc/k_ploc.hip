@@ -26,10 +26,12 @@
 // its own build gives (a single build is a batch of one).
 //
 // MI355X mapping:
-//   * k_ploc_nn: persistent blocks; each tile stages 256 clusters plus an
-//     r-halo on each side in LDS (SoA floats, conflict-free), every lane scans
-//     its window from LDS and packs (distance bits << 32 | j) into one u64
-//     minimum;
+//   * k_ploc_nn: persistent blocks; each tile stages 768 clusters plus an
+//     r-halo on each side in LDS (SoA floats), every lane owns three
+//     consecutive clusters and scans the union of their windows once, so each
+//     box read from LDS (stride 3 across lanes: conflict-free) feeds three
+//     candidates; candidates pack (distance bits << 32 | j) into one u64
+//     minimum each;
 //   * k_ploc_merge: persistent blocks take tiles of 1024 clusters in order
 //     and run mutual-pair detection, new nodes, and stream compaction of the
 //     survivors behind one decoupled look-back scan whose 64-bit words pack
@@ -46,6 +48,8 @@
 namespace bvhk {
 
 constexpr int kNnBlock = 256;
+constexpr int kNnPer = 3;                     // consecutive clusters per lane (odd: conflict-free LDS)
+constexpr int kNnTile = kNnBlock * kNnPer;
 constexpr int kMergeBlock = 256;
 constexpr int kMergeItems = 4;
 constexpr int kMergeTile = kMergeBlock * kMergeItems;
@@ -63,7 +67,7 @@ constexpr uint64_t kVMask = (1ull << 31) - 1;
 constexpr uint64_t kMMask = ((1ull << 30) - 1) << 31;
 
 int ploc_small() { return kPlocSmall; }
-int ploc_nn_block() { return kNnBlock; }
+int ploc_nn_block() { return kNnTile; }
 int ploc_merge_tile() { return kMergeTile; }
 
 __device__ __forceinline__ uint32_t remap_id(uint32_t id, uint32_t root) {
@@ -141,12 +145,12 @@ __device__ __forceinline__ bool ploc_global_step(const PlocState& st) {
 
 template <bool SEG>
 __global__ __launch_bounds__(kNnBlock) void k_ploc_nn(PlocArgs a, int k) {
-    extern __shared__ float s_nn[];  // 6 SoA arrays of (kNnBlock + 2r) floats
+    extern __shared__ float s_nn[];  // 6 SoA arrays of (kNnTile + 2r) floats
     const PlocState st = a.state[k];
     if (!ploc_global_step(st)) return;
     const uint32_t m = st.m;
     const uint32_t par = st.parity;
-    const uint32_t tiles = (m + kNnBlock - 1) / kNnBlock;
+    const uint32_t tiles = (m + kNnTile - 1) / kNnTile;
     const uint32_t mtiles = (m + kMergeTile - 1) / kMergeTile;
     // Reset the merge kernel's look-back words, tile counter and next state.
     for (uint32_t t = blockIdx.x * kNnBlock + threadIdx.x; t < mtiles; t += gridDim.x * kNnBlock) a.status[t] = 0;
@@ -162,12 +166,12 @@ __global__ __launch_bounds__(kNnBlock) void k_ploc_nn(PlocArgs a, int k) {
         a.state[(k + 1) % kPlocMaxIter] = ns;
     }
     const int r = a.radius;
-    const int span = kNnBlock + 2 * r;
+    const int span = kNnTile + 2 * r;
     const float4* __restrict__ bx = a.box[par];
     const uint4* __restrict__ segs = a.seg[par];
     for (uint32_t tile = blockIdx.x; tile < tiles; tile += gridDim.x) {
         __syncthreads();
-        const int64_t t0 = (int64_t)tile * kNnBlock;
+        const int64_t t0 = (int64_t)tile * kNnTile;
         for (int q = threadIdx.x; q < span; q += kNnBlock) {
             int64_t p = t0 - r + q;
             if (p >= 0 && p < (int64_t)m) {
@@ -181,27 +185,56 @@ __global__ __launch_bounds__(kNnBlock) void k_ploc_nn(PlocArgs a, int k) {
             }
         }
         __syncthreads();
-        const int64_t i = t0 + threadIdx.x;
-        if (i >= (int64_t)m) continue;
-        const uint4 sg = segs[SEG ? a.sid[par][i] : 0u];
-        const int64_t slo = sg.x, shi = (int64_t)sg.x + sg.y - 1;
-        const int qi = threadIdx.x + r;
-        Box bi;
-        bi.mnx = s_nn[0 * span + qi]; bi.mny = s_nn[1 * span + qi]; bi.mnz = s_nn[2 * span + qi];
-        bi.mxx = s_nn[3 * span + qi]; bi.mxy = s_nn[4 * span + qi]; bi.mxz = s_nn[5 * span + qi];
-        const int64_t lo = (i - r > slo) ? i - r : slo;
-        const int64_t hi = (i + r < shi) ? i + r : shi;
-        uint64_t best = ~0ull;
-        for (int64_t j = lo; j <= hi; ++j) {
-            if (j == i) continue;
+        // This lane's clusters i0, i0+1, i0+2 and their windows, clipped to
+        // their meshes.  Window bounds are non-decreasing in i (meshes are in
+        // order), so one pass over [lo[0], hi[kNnPer-1]] covers all three and
+        // every box read from LDS serves up to three candidates.
+        const int64_t i0 = t0 + (int64_t)kNnPer * threadIdx.x;
+        if (i0 >= (int64_t)m) continue;
+        Box bi[kNnPer];
+        int64_t lo[kNnPer], hi[kNnPer];
+        uint64_t best[kNnPer];
+#pragma unroll
+        for (int c = 0; c < kNnPer; ++c) {
+            const int64_t i = i0 + c;
+            best[c] = ~0ull;
+            if (i < (int64_t)m) {
+                const uint4 sg = segs[SEG ? a.sid[par][i] : 0u];
+                const int64_t slo = sg.x, shi = (int64_t)sg.x + sg.y - 1;
+                lo[c] = (i - r > slo) ? i - r : slo;
+                hi[c] = (i + r < shi) ? i + r : shi;
+                const int qi = (int)(i - t0) + r;
+                bi[c].mnx = s_nn[0 * span + qi]; bi[c].mny = s_nn[1 * span + qi]; bi[c].mnz = s_nn[2 * span + qi];
+                bi[c].mxx = s_nn[3 * span + qi]; bi[c].mxy = s_nn[4 * span + qi]; bi[c].mxz = s_nn[5 * span + qi];
+            } else {  // past the end: an empty window inside cluster i0's (c > 0 here)
+                lo[c] = hi[0] + 1;
+                hi[c] = hi[0];
+                bi[c] = bi[0];
+            }
+        }
+        // lo and hi are non-decreasing in c (an empty window included).
+        const int64_t jlo = lo[0];
+        int64_t jhi = hi[0];
+#pragma unroll
+        for (int c = 1; c < kNnPer; ++c) jhi = hi[c] > jhi ? hi[c] : jhi;
+        for (int64_t j = jlo; j <= jhi; ++j) {
             const int q = (int)(j - t0) + r;
             Box bj;
             bj.mnx = s_nn[0 * span + q]; bj.mny = s_nn[1 * span + q]; bj.mnz = s_nn[2 * span + q];
             bj.mxx = s_nn[3 * span + q]; bj.mxy = s_nn[4 * span + q]; bj.mxz = s_nn[5 * span + q];
-            uint64_t key = nn_key(area(merge(bi, bj)), (uint32_t)j);
-            best = key < best ? key : best;
+#pragma unroll
+            for (int c = 0; c < kNnPer; ++c) {
+                if (j >= lo[c] && j <= hi[c] && j != i0 + c) {
+                    const uint64_t key = nn_key(area(merge(bi[c], bj)), (uint32_t)j);
+                    best[c] = key < best[c] ? key : best[c];
+                }
+            }
         }
-        a.nn[i] = (best == ~0ull) ? -1 : (int32_t)(uint32_t)best;
+#pragma unroll
+        for (int c = 0; c < kNnPer; ++c) {
+            const int64_t i = i0 + c;
+            if (i < (int64_t)m) a.nn[i] = (best[c] == ~0ull) ? -1 : (int32_t)(uint32_t)best[c];
+        }
     }
 }
 
@@ -500,9 +533,9 @@ void launch_ploc_init(const PlocArgs& a, const uint64_t* keys, const KeyFormat& 
 }
 
 void launch_ploc_nn(const PlocArgs& a, int k, uint32_t m_bound, hipStream_t s) {
-    uint32_t g = (m_bound + kNnBlock - 1) / kNnBlock;
+    uint32_t g = (m_bound + kNnTile - 1) / kNnTile;
     g = g < 1 ? 1 : (g > kNnGridMax ? kNnGridMax : g);
-    size_t lds = (size_t)6 * (kNnBlock + 2 * a.radius) * sizeof(float);
+    size_t lds = (size_t)6 * (kNnTile + 2 * a.radius) * sizeof(float);
     if (a.sid[0]) hipLaunchKernelGGL(k_ploc_nn<true>, dim3(g), dim3(kNnBlock), lds, s, a, k);
     else hipLaunchKernelGGL(k_ploc_nn<false>, dim3(g), dim3(kNnBlock), lds, s, a, k);
 }

@@ -22,8 +22,7 @@
 
 namespace bvhk {
 
-constexpr int kSortBlock = 256;
-constexpr int kSortItems = 16;
+constexpr int kSortBlock = kSortThreads;
 constexpr uint32_t kFlagA = 1u << 30;
 constexpr uint32_t kFlagP = 2u << 30;
 constexpr uint32_t kValMask = (1u << 30) - 1;

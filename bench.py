@@ -31,6 +31,26 @@ HBM_PEAK_GBS = 8000.0
 STAGE_BYTES = {"Bounds/Centroids": 68, "Morton Codes": 40, "Sort Pass": 16, "Topology+Refit": 172}
 
 
+# HBM bytes per dispatch of the dominant kernels, from two rocprofv3 --pmc runs
+# (FETCH_SIZE, WRITE_SIZE) of this workload, scaled by the same counters on a
+# 256 MiB copy kernel; the summary and the commands that produced it are in
+# profiles/.
+TRAFFIC_FILE_REL = "profiles/traffic_lbvh_4m_blobs.json"
+TRAFFIC_FILE = os.path.join(ROOT, TRAFFIC_FILE_REL)
+
+
+def pmc_traffic(kernel, n):
+    try:
+        with open(TRAFFIC_FILE) as f:
+            d = json.load(f)
+    except (OSError, ValueError):
+        return None
+    if d.get("triangles") != n:
+        return None
+    k = d.get("kernels", {}).get(kernel)
+    return int(k["hbm_bytes_per_dispatch"]) if k else None
+
+
 def stage_unit_bytes(name):
     for k, v in STAGE_BYTES.items():
         if name.startswith(k):
@@ -327,9 +347,11 @@ def main():
     dom_name, dom = max(kernels.items(), key=lambda kv: kv[1]["ms"])
     avg_launch_ms = dom["ms"] / dom["launches"]
     achieved = dom["unit"] * mesh.n / (avg_launch_ms * 1e-3) / 1e9
+    traffic = pmc_traffic(dom_name, mesh.n)
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None, "kernel": dom_name,
-                "avg_launch_ms": round(avg_launch_ms, 4), "bytes_per_launch": dom["unit"] * mesh.n}
+                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic, "kernel": dom_name,
+                "avg_launch_ms": round(avg_launch_ms, 4), "bytes_per_launch": dom["unit"] * mesh.n,
+                "traffic_source": TRAFFIC_FILE_REL if traffic else None}
 
     cpu = None
     parity = None

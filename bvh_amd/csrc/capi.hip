@@ -67,7 +67,7 @@ struct bvh_context {
     uint32_t treelet_epoch = 0;  // LBVH+ arrival-word tags
     Workspace ws;
     SegTables tabs;  // eager segmented builds
-    DevBuf trace_ovf, trace_list, trace_io;  // traversal queue and host-call staging
+    DevBuf trace_ovf, trace_io;  // traversal error word and host-call staging
     // last build's sorted keys (for bvh_debug_sorted_keys)
     uint64_t* last_keys = nullptr;
     uint32_t* last_vals = nullptr;
@@ -497,8 +497,8 @@ int ploc_drive(bvh_ctx_t ctx, const PlocArgs& a, int* slot, uint32_t m_bound, ui
         int batch = ploc_iter_estimate(hs.maxm, 0.80);
         batch = batch < 2 ? 2 : (batch > 48 ? 48 : batch);
         for (int b = 0; b < batch; ++b, k = (k + 1) % kPlocMaxIter) {
-            launch_ploc_nn(a, k, std::min(hs.m, nn_grid * (uint32_t)ploc_nn_block()), ctx->stream);
-            launch_ploc_merge(a, k, std::min(hs.m, merge_grid * (uint32_t)ploc_merge_tile()), ctx->stream);
+            launch_ploc_nn(a, k, hs.m, nn_grid, ctx->stream);
+            launch_ploc_merge(a, k, hs.m, merge_grid, ctx->stream);
         }
         TRY(check_launch(ctx, "k_ploc_nn/merge"));
         HIP_TRY(ctx, hipMemcpyAsync(&hs, a.state + k, sizeof(hs), hipMemcpyDeviceToHost, ctx->stream));
@@ -720,9 +720,8 @@ int batch_enqueue(bvh_batch_t b, const SortPlan& plan) {
     const PlocArgs a = ploc_args(w, j, b->radius);
     launch_ploc_init(a, keys, j.kf, P<float4>(w.tri_boxes), P<uint32_t>(w.scene) + 7, ctx->stream);
     for (int k = 0; k < b->ploc_iters; ++k) {
-        launch_ploc_nn(a, k % kPlocMaxIter, std::min(j.total, b->nn_grid * (uint32_t)ploc_nn_block()), ctx->stream);
-        launch_ploc_merge(a, k % kPlocMaxIter, std::min(j.total, b->merge_grid * (uint32_t)ploc_merge_tile()),
-                          ctx->stream);
+        launch_ploc_nn(a, k % kPlocMaxIter, j.total, b->nn_grid, ctx->stream);
+        launch_ploc_merge(a, k % kPlocMaxIter, j.total, b->merge_grid, ctx->stream);
     }
     launch_ploc_tail(a, b->ploc_iters % kPlocMaxIter, ctx->stream);
     return check_launch(ctx, "k_ploc");
@@ -943,7 +942,6 @@ int bvh_ctx_destroy(bvh_ctx_t ctx) {
     ctx->ws.each([](DevBuf& b) { free_buf(b); });
     ctx->tabs.release();
     free_buf(ctx->trace_ovf);
-    free_buf(ctx->trace_list);
     free_buf(ctx->trace_io);
     for (auto& e : ctx->ev)
         if (e) (void)hipEventDestroy(e);
@@ -1244,7 +1242,6 @@ int bvh_trace(bvh_ctx_t ctx, bvh_tree_t tree, bvh_mesh_t mesh, const float* d_ra
     HIP_TRY(ctx, hipSetDevice(ctx->device));
     int st;
     if ((st = ensure(ctx, ctx->trace_ovf, 64))) return st;
-    if ((st = ensure(ctx, ctx->trace_list, (size_t)nrays * 4))) return st;
     HIP_TRY(ctx, hipEventRecord(ctx->ev[0], ctx->stream));
     if ((st = prepare_trace(ctx, tree))) return st;
     TraceArgs a;
@@ -1259,10 +1256,8 @@ int bvh_trace(bvh_ctx_t ctx, bvh_tree_t tree, bvh_mesh_t mesh, const float* d_ra
     a.prim_out = d_prim;
     a.counters = d_counters;
     a.overflow_count = P<uint32_t>(ctx->trace_ovf);
-    a.overflow_list = P<uint32_t>(ctx->trace_list);
     launch_fill_zero(a.overflow_count, 4, ctx->stream);
     launch_trace(a, ctx->stream);
-    launch_trace_deep(a, ctx->stream);
     if ((st = check_launch(ctx, "k_trace"))) return st;
     HIP_TRY(ctx, hipEventRecord(ctx->ev[1], ctx->stream));
     uint32_t ovf[2] = {0, 0};

@@ -127,14 +127,12 @@ struct TraceArgs {
     float* t_out;
     int32_t* prim_out;
     int32_t* counters;          // optional: 3 per ray
-    uint32_t* overflow_count;   // [0] queued rays, [1] error flag (cleared by the caller)
-    uint32_t* overflow_list;    // nrays entries
+    uint32_t* overflow_count;   // [1] error flag: a ray needed more stack than the limit (cleared by the caller)
 };
 void launch_tri_aos(const MeshView& m, float4* tri, hipStream_t s);
 // Per-internal-node traversal records (64 B each, indexed by node id).
 void launch_trace_layout(const DNode* nodes, uint32_t num_nodes, float4* rec, hipStream_t s);
 void launch_trace(const TraceArgs& a, hipStream_t s);
-void launch_trace_deep(const TraceArgs& a, hipStream_t s);
 
 // ---- LBVH+ treelets (k_treelet.hip) -------------------------------------------
 struct TreeletArgs {
@@ -184,9 +182,10 @@ int ploc_nn_block();
 int ploc_merge_tile();
 void launch_ploc_init(const PlocArgs& a, const uint64_t* keys, const KeyFormat& kf, const float4* tri_boxes,
                       uint32_t* err, hipStream_t s);
-// One global iteration (no-op once every mesh is small); m_bound caps the grid.
-void launch_ploc_nn(const PlocArgs& a, int k, uint32_t m_bound, hipStream_t s);
-void launch_ploc_merge(const PlocArgs& a, int k, uint32_t m_bound, hipStream_t s);
+// One global iteration (no-op once every mesh is small).  The persistent grid
+// is sized for m_bound clusters and capped at max_grid blocks.
+void launch_ploc_nn(const PlocArgs& a, int k, uint32_t m_bound, uint32_t max_grid, hipStream_t s);
+void launch_ploc_merge(const PlocArgs& a, int k, uint32_t m_bound, uint32_t max_grid, hipStream_t s);
 // One workgroup per mesh; reads state slot k.
 void launch_ploc_tail(const PlocArgs& a, int k, hipStream_t s);
 

@@ -159,6 +159,26 @@ __global__ __launch_bounds__(kHierBlock) void k_hierarchy(HierArgs p) {
     const int len = (int)(b - a);
     const int tid = threadIdx.x;
 
+    // Every item's primitive and leaf box first: the random box gathers of all
+    // items are in flight together (and overlap the delta scans below)
+    // instead of one per climb.
+    uint32_t pprim[kHierItems];
+    Box pbox[kHierItems];
+#pragma unroll
+    for (int k = 0; k < kHierItems; ++k) {
+        const uint32_t i = a + k * kHierBlock + tid;
+        uint32_t prim = 0;
+        if (i < b) {
+            prim = K60 ? vals[i] : (uint32_t)(keys[i] & idx_mask);
+            if (prim >= n) {  // cannot happen for a correct sort; never index out of bounds
+                atomicOr(p.err, 1u);
+                prim = 0;
+            }
+            d.indices[i] = prim;
+            pbox[k] = load_box(tboxes + 2 * (size_t)prim);
+        }
+        pprim[k] = prim;
+    }
     for (int q = tid; q <= len; q += kHierBlock) {
         int64_t j = (int64_t)a - 1 + q;
         s_d[q] = (j < 0 || j >= (int64_t)n - 1) ? (int8_t)-1 : (int8_t)key_delta<K60>(keys, vals, (uint32_t)j);
@@ -180,17 +200,12 @@ __global__ __launch_bounds__(kHierBlock) void k_hierarchy(HierArgs p) {
         const int chain = k * kHierBlock + tid;
         const uint32_t i = a + chain;
         if (i >= b) continue;
-        uint32_t prim = K60 ? vals[i] : (uint32_t)(keys[i] & idx_mask);
-        if (prim >= n) {  // cannot happen for a correct sort; never index out of bounds
-            atomicOr(p.err, 1u);
-            prim = 0;
-        }
-        d.indices[i] = prim;
+        const uint32_t prim = pprim[k];
         // The current node: its range, box, children and index.  Its record is
         // written once its handoff kind is known.
         uint32_t l = i, r = i;
         uint32_t my = n - 1 + i;
-        Box box = load_box(tboxes + 2 * (size_t)prim);
+        Box box = pbox[k];
         int32_t lc = (int32_t)(prim | 0x80000000u), rc = -1;
         int dl = dget((int64_t)l - 1);
         int dr = dget((int64_t)r);

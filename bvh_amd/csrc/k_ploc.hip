@@ -50,9 +50,9 @@ namespace bvhk {
 constexpr int kNnBlock = 256;
 constexpr int kNnPer = 3;                     // consecutive clusters per lane (odd: conflict-free LDS)
 constexpr int kNnTile = kNnBlock * kNnPer;
+constexpr uint32_t kNnFill = 1u << 18;        // lanes that keep every CU busy
 constexpr int kMergeBlock = 256;
-constexpr int kMergeItems = 4;
-constexpr int kMergeTile = kMergeBlock * kMergeItems;
+constexpr uint32_t kMergeBigMin = 1u << 18;   // from here on, 4 clusters per thread per merge tile
 constexpr int kTailBlock = 1024;
 constexpr uint32_t kNnGridMax = 2048;
 constexpr uint32_t kMergeGridMax = 1024;
@@ -68,7 +68,22 @@ constexpr uint64_t kMMask = ((1ull << 30) - 1) << 31;
 
 int ploc_small() { return kPlocSmall; }
 int ploc_nn_block() { return kNnTile; }
-int ploc_merge_tile() { return kMergeTile; }
+int ploc_merge_tile() { return kMergeBlock; }  // smallest merge tile (sizes the look-back words)
+
+// Merge tile of an iteration with m live clusters.
+__host__ __device__ __forceinline__ uint32_t merge_tile(uint32_t m) {
+    return m >= kMergeBigMin ? 4u * kMergeBlock : (uint32_t)kMergeBlock;
+}
+
+// Lanes per cluster in the neighbour search: 0 = three clusters per lane
+// (large m), else G lanes split each window (small m: more lanes in flight).
+__device__ __forceinline__ int nn_lanes(uint32_t m) {
+    if (m >= 3u * kNnFill) return 0;
+    if (m >= kNnFill) return 1;
+    if (m >= kNnFill / 2) return 2;
+    if (m >= kNnFill / 4) return 4;
+    return 8;
+}
 
 __device__ __forceinline__ uint32_t remap_id(uint32_t id, uint32_t root) {
     return id == 0 ? root : (id == root ? 0u : id);
@@ -150,8 +165,10 @@ __global__ __launch_bounds__(kNnBlock) void k_ploc_nn(PlocArgs a, int k) {
     if (!ploc_global_step(st)) return;
     const uint32_t m = st.m;
     const uint32_t par = st.parity;
-    const uint32_t tiles = (m + kNnTile - 1) / kNnTile;
-    const uint32_t mtiles = (m + kMergeTile - 1) / kMergeTile;
+    const int G = nn_lanes(m);
+    const uint32_t tc = G == 0 ? (uint32_t)kNnTile : (uint32_t)(kNnBlock / G);  // clusters per tile
+    const uint32_t tiles = (m + tc - 1) / tc;
+    const uint32_t mtiles = (m + merge_tile(m) - 1) / merge_tile(m);
     // Reset the merge kernel's look-back words, tile counter and next state.
     for (uint32_t t = blockIdx.x * kNnBlock + threadIdx.x; t < mtiles; t += gridDim.x * kNnBlock) a.status[t] = 0;
     if (blockIdx.x == 0 && threadIdx.x == 0) {
@@ -166,12 +183,12 @@ __global__ __launch_bounds__(kNnBlock) void k_ploc_nn(PlocArgs a, int k) {
         a.state[(k + 1) % kPlocMaxIter] = ns;
     }
     const int r = a.radius;
-    const int span = kNnTile + 2 * r;
+    const int span = (int)tc + 2 * r;
     const float4* __restrict__ bx = a.box[par];
     const uint4* __restrict__ segs = a.seg[par];
     for (uint32_t tile = blockIdx.x; tile < tiles; tile += gridDim.x) {
         __syncthreads();
-        const int64_t t0 = (int64_t)tile * kNnTile;
+        const int64_t t0 = (int64_t)tile * tc;
         for (int q = threadIdx.x; q < span; q += kNnBlock) {
             int64_t p = t0 - r + q;
             if (p >= 0 && p < (int64_t)m) {
@@ -185,6 +202,38 @@ __global__ __launch_bounds__(kNnBlock) void k_ploc_nn(PlocArgs a, int k) {
             }
         }
         __syncthreads();
+        if (G != 0) {
+            // G lanes per cluster: lane g scans window positions lo+g, lo+g+G, ...
+            // and the group takes the minimum packed key.
+            const int c = threadIdx.x / G, g = threadIdx.x % G;
+            const int64_t i = t0 + c;
+            unsigned long long best = ~0ull;
+            if (i < (int64_t)m) {
+                const uint4 sg = segs[SEG ? a.sid[par][i] : 0u];
+                const int64_t slo = sg.x, shi = (int64_t)sg.x + sg.y - 1;
+                const int64_t lo = (i - r > slo) ? i - r : slo;
+                const int64_t hi = (i + r < shi) ? i + r : shi;
+                const int qi = (int)(i - t0) + r;
+                Box bi;
+                bi.mnx = s_nn[0 * span + qi]; bi.mny = s_nn[1 * span + qi]; bi.mnz = s_nn[2 * span + qi];
+                bi.mxx = s_nn[3 * span + qi]; bi.mxy = s_nn[4 * span + qi]; bi.mxz = s_nn[5 * span + qi];
+                for (int64_t j = lo + g; j <= hi; j += G) {
+                    if (j == i) continue;
+                    const int q = (int)(j - t0) + r;
+                    Box bj;
+                    bj.mnx = s_nn[0 * span + q]; bj.mny = s_nn[1 * span + q]; bj.mnz = s_nn[2 * span + q];
+                    bj.mxx = s_nn[3 * span + q]; bj.mxy = s_nn[4 * span + q]; bj.mxz = s_nn[5 * span + q];
+                    const unsigned long long key = nn_key(area(merge(bi, bj)), (uint32_t)j);
+                    best = key < best ? key : best;
+                }
+            }
+            for (int off = G / 2; off >= 1; off >>= 1) {
+                const unsigned long long o = __shfl_xor(best, off, G);
+                best = o < best ? o : best;
+            }
+            if (g == 0 && i < (int64_t)m) a.nn[i] = (best == ~0ull) ? -1 : (int32_t)(uint32_t)best;
+            continue;
+        }
         // This lane's clusters i0, i0+1, i0+2 and their windows, clipped to
         // their meshes.  Window bounds are non-decreasing in i (meshes are in
         // order), so one pass over [lo[0], hi[kNnPer-1]] covers all three and
@@ -270,17 +319,13 @@ __device__ __forceinline__ uint64_t block_excl_scan_seg(uint64_t v, uint64_t* s_
     return scomb(before, prev);
 }
 
-template <bool SEG>
-__global__ __launch_bounds__(kMergeBlock) void k_ploc_merge(PlocArgs a, int k) {
-    __shared__ uint32_t s_tile;
-    __shared__ uint64_t s_w[kMergeBlock / kWave];
-    __shared__ uint64_t s_prefix;
-    const PlocState st = a.state[k];
-    const int kn = (k + 1) % kPlocMaxIter;
-    if (!ploc_global_step(st)) {
-        if (blockIdx.x == 0 && threadIdx.x == 0) a.state[kn] = st;
-        return;
-    }
+// Merge + compaction over tiles of ITEMS clusters per thread (the tile size
+// is a function of m, shared with the look-back reset in k_ploc_nn).
+template <bool SEG, int ITEMS>
+__device__ void merge_tiles(const PlocArgs& a, const PlocState& st, int kn, uint32_t* s_tile, uint64_t* s_w,
+                            uint64_t* s_prefix) {
+    constexpr int kMergeItems = ITEMS;
+    constexpr uint32_t kMergeTile = kMergeBlock * ITEMS;
     const uint32_t m = st.m;
     const uint32_t par = st.parity;
     const uint32_t tiles = (m + kMergeTile - 1) / kMergeTile;
@@ -293,9 +338,9 @@ __global__ __launch_bounds__(kMergeBlock) void k_ploc_merge(PlocArgs a, int k) {
     uint32_t* __restrict__ iout = a.id[par ^ 1u];
     while (true) {
         __syncthreads();
-        if (threadIdx.x == 0) s_tile = atomicAdd(a.tile_ctr, 1u);
+        if (threadIdx.x == 0) *s_tile = atomicAdd(a.tile_ctr, 1u);
         __syncthreads();
-        const uint32_t tile = s_tile;
+        const uint32_t tile = *s_tile;
         if (tile >= tiles) break;
         const uint32_t base = tile * kMergeTile + threadIdx.x * kMergeItems;
 
@@ -331,7 +376,7 @@ __global__ __launch_bounds__(kMergeBlock) void k_ploc_merge(PlocArgs a, int k) {
             uint64_t* sw = a.status + tile;
             if (tile == 0) {
                 __hip_atomic_store(sw, kStP | tagg, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                s_prefix = 0;
+                *s_prefix = 0;
             } else {
                 __hip_atomic_store(sw, kStA | tagg, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
                 uint64_t pre = 0;
@@ -348,12 +393,12 @@ __global__ __launch_bounds__(kMergeBlock) void k_ploc_merge(PlocArgs a, int k) {
                     --t;
                 }
                 __hip_atomic_store(sw, kStP | scomb(pre, tagg), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                s_prefix = pre;
+                *s_prefix = pre;
             }
-            if (tile == tiles - 1) a.state[kn].m = (uint32_t)(scomb(s_prefix, tagg) & kVMask);
+            if (tile == tiles - 1) a.state[kn].m = (uint32_t)(scomb(*s_prefix, tagg) & kVMask);
         }
         __syncthreads();
-        const uint64_t run0 = scomb(s_prefix, excl);
+        const uint64_t run0 = scomb(*s_prefix, excl);
         uint32_t pos = (uint32_t)(run0 & kVMask);
         uint32_t mrank = (uint32_t)((run0 & kMMask) >> 31);
 #pragma unroll
@@ -399,6 +444,21 @@ __global__ __launch_bounds__(kMergeBlock) void k_ploc_merge(PlocArgs a, int k) {
             }
         }
     }
+}
+
+template <bool SEG>
+__global__ __launch_bounds__(kMergeBlock) void k_ploc_merge(PlocArgs a, int k) {
+    __shared__ uint32_t s_tile;
+    __shared__ uint64_t s_w[kMergeBlock / kWave];
+    __shared__ uint64_t s_prefix;
+    const PlocState st = a.state[k];
+    const int kn = (k + 1) % kPlocMaxIter;
+    if (!ploc_global_step(st)) {
+        if (blockIdx.x == 0 && threadIdx.x == 0) a.state[kn] = st;
+        return;
+    }
+    if (merge_tile(st.m) == 4u * kMergeBlock) merge_tiles<SEG, 4>(a, st, kn, &s_tile, s_w, &s_prefix);
+    else merge_tiles<SEG, 1>(a, st, kn, &s_tile, s_w, &s_prefix);
 }
 
 // Finishes every remaining iteration of one mesh (block = mesh) in LDS.
@@ -532,16 +592,18 @@ void launch_ploc_init(const PlocArgs& a, const uint64_t* keys, const KeyFormat& 
     hipLaunchKernelGGL(k_ploc_init, dim3(a.sb.nblocks), dim3(256), 0, s, a, keys, kf, tri_boxes, err);
 }
 
-void launch_ploc_nn(const PlocArgs& a, int k, uint32_t m_bound, hipStream_t s) {
-    uint32_t g = (m_bound + kNnTile - 1) / kNnTile;
+void launch_ploc_nn(const PlocArgs& a, int k, uint32_t m_bound, uint32_t max_grid, hipStream_t s) {
+    uint32_t g = (m_bound + kNnBlock / 8 - 1) / (kNnBlock / 8);  // enough for the finest split
+    if (g > max_grid) g = max_grid;
     g = g < 1 ? 1 : (g > kNnGridMax ? kNnGridMax : g);
     size_t lds = (size_t)6 * (kNnTile + 2 * a.radius) * sizeof(float);
     if (a.sid[0]) hipLaunchKernelGGL(k_ploc_nn<true>, dim3(g), dim3(kNnBlock), lds, s, a, k);
     else hipLaunchKernelGGL(k_ploc_nn<false>, dim3(g), dim3(kNnBlock), lds, s, a, k);
 }
 
-void launch_ploc_merge(const PlocArgs& a, int k, uint32_t m_bound, hipStream_t s) {
-    uint32_t g = (m_bound + kMergeTile - 1) / kMergeTile;
+void launch_ploc_merge(const PlocArgs& a, int k, uint32_t m_bound, uint32_t max_grid, hipStream_t s) {
+    uint32_t g = (m_bound + kMergeBlock - 1) / kMergeBlock;
+    if (g > max_grid) g = max_grid;
     g = g < 1 ? 1 : (g > kMergeGridMax ? kMergeGridMax : g);
     if (a.sid[0]) hipLaunchKernelGGL(k_ploc_merge<true>, dim3(g), dim3(kMergeBlock), 0, s, a, k);
     else hipLaunchKernelGGL(k_ploc_merge<false>, dim3(g), dim3(kMergeBlock), 0, s, a, k);

@@ -67,7 +67,7 @@ struct bvh_context {
     uint32_t treelet_epoch = 0;  // LBVH+ arrival-word tags
     Workspace ws;
     SegTables tabs;  // eager segmented builds
-    DevBuf trace_ovf, trace_io;  // traversal error word and host-call staging
+    DevBuf trace_ovf, trace_list, trace_io;  // traversal queue and host-call staging
     // last build's sorted keys (for bvh_debug_sorted_keys)
     uint64_t* last_keys = nullptr;
     uint32_t* last_vals = nullptr;
@@ -942,6 +942,7 @@ int bvh_ctx_destroy(bvh_ctx_t ctx) {
     ctx->ws.each([](DevBuf& b) { free_buf(b); });
     ctx->tabs.release();
     free_buf(ctx->trace_ovf);
+    free_buf(ctx->trace_list);
     free_buf(ctx->trace_io);
     for (auto& e : ctx->ev)
         if (e) (void)hipEventDestroy(e);
@@ -1242,6 +1243,7 @@ int bvh_trace(bvh_ctx_t ctx, bvh_tree_t tree, bvh_mesh_t mesh, const float* d_ra
     HIP_TRY(ctx, hipSetDevice(ctx->device));
     int st;
     if ((st = ensure(ctx, ctx->trace_ovf, 64))) return st;
+    if ((st = ensure(ctx, ctx->trace_list, (size_t)nrays * 4))) return st;
     HIP_TRY(ctx, hipEventRecord(ctx->ev[0], ctx->stream));
     if ((st = prepare_trace(ctx, tree))) return st;
     TraceArgs a;
@@ -1256,8 +1258,10 @@ int bvh_trace(bvh_ctx_t ctx, bvh_tree_t tree, bvh_mesh_t mesh, const float* d_ra
     a.prim_out = d_prim;
     a.counters = d_counters;
     a.overflow_count = P<uint32_t>(ctx->trace_ovf);
+    a.overflow_list = P<uint32_t>(ctx->trace_list);
     launch_fill_zero(a.overflow_count, 4, ctx->stream);
     launch_trace(a, ctx->stream);
+    launch_trace_deep(a, ctx->stream);
     if ((st = check_launch(ctx, "k_trace"))) return st;
     HIP_TRY(ctx, hipEventRecord(ctx->ev[1], ctx->stream));
     uint32_t ovf[2] = {0, 0};

@@ -127,12 +127,14 @@ struct TraceArgs {
     float* t_out;
     int32_t* prim_out;
     int32_t* counters;          // optional: 3 per ray
-    uint32_t* overflow_count;   // [1] error flag: a ray needed more stack than the limit (cleared by the caller)
+    uint32_t* overflow_count;   // [0] queued rays, [1] error flag (cleared by the caller)
+    uint32_t* overflow_list;    // nrays entries
 };
 void launch_tri_aos(const MeshView& m, float4* tri, hipStream_t s);
 // Per-internal-node traversal records (64 B each, indexed by node id).
 void launch_trace_layout(const DNode* nodes, uint32_t num_nodes, float4* rec, hipStream_t s);
 void launch_trace(const TraceArgs& a, hipStream_t s);
+void launch_trace_deep(const TraceArgs& a, hipStream_t s);
 
 // ---- LBVH+ treelets (k_treelet.hip) -------------------------------------------
 struct TreeletArgs {

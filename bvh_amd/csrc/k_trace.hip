@@ -19,10 +19,11 @@
 //     from the node array once per tree (k_trace_layout), so expanding a
 //     node costs one memory round trip instead of two dependent ones (node,
 //     then its children's boxes); leaf children carry their primitive;
-//   * the traversal stack keeps its top 24 entries of (node, tenter) per lane
-//     in LDS ([depth][lane], conflict-free) and continues in a private array
-//     up to 256 entries; nothing is truncated (the reference drops pushes
-//     past 128) and a deep ray never restarts;
+//   * the traversal stack is a short per-lane stack in LDS ([depth][lane],
+//     conflict-free), 24 entries of (node, tenter); a ray that would need a
+//     deeper stack is not truncated (the reference drops pushes past 128):
+//     it is queued and re-traced by k_trace_deep with a 256-entry stack (24
+//     entries in LDS, the rest in scratch);
 //   * triangles are read from a 48-byte-per-triangle AoS copy of the mesh
 //     (three 16-byte loads instead of nine scattered 4-byte loads).
 #include "dev_common.hpp"
@@ -154,10 +155,18 @@ __device__ __forceinline__ RayState load_ray(const float* rays, uint32_t i) {
     return r;
 }
 
-// Traversal stack: the top kShortStack entries of every lane live in LDS
-// ([depth][lane], conflict-free); deeper entries continue in a private array
-// (scratch memory), so a deep ray neither restarts nor drops entries.
-struct HybridStack {
+// Stack accessors: LDS short stack or private deep stack.
+struct LdsStack {
+    int32_t (*idx)[kTraceBlock];
+    float (*tent)[kTraceBlock];
+    int lane;
+    static constexpr int kCap = kShortStack;
+    __device__ void put(int sp, int32_t i, float t) { idx[sp][lane] = i; tent[sp][lane] = t; }
+    __device__ void get(int sp, int32_t& i, float& t) const { i = idx[sp][lane]; t = tent[sp][lane]; }
+};
+// Deep stack for rays that outgrew the short stack: the first kShortStack
+// entries in LDS, the rest in a private array (scratch memory).
+struct DeepStack {
     int32_t (*idx)[kTraceBlock];
     float (*tent)[kTraceBlock];
     int lane;
@@ -267,15 +276,37 @@ __global__ __launch_bounds__(kTraceBlock) void k_trace(TraceArgs a) {
     const uint32_t i = blockIdx.x * kTraceBlock + threadIdx.x;
     if (i >= a.nrays) return;
     const RayState r = load_ray(a.rays, i);
-    HybridStack st;
-    st.idx = s_idx;
-    st.tent = s_tent;
-    st.lane = (int)threadIdx.x;
+    LdsStack st{s_idx, s_tent, (int)threadIdx.x};
     float t;
     int32_t p;
     int32_t c[3] = {0, 0, 0};
-    if (traverse(a, r, st, t, p, c)) write_result(a, i, t, p, c);
-    else atomicOr(a.overflow_count + 1, 1u);  // deeper than kDeepStack or not a tree: reported as an error
+    if (traverse(a, r, st, t, p, c)) {
+        write_result(a, i, t, p, c);
+    } else {
+        const uint32_t slot = atomicAdd(a.overflow_count, 1u);
+        a.overflow_list[slot] = i;
+    }
+}
+
+// Rays whose stack outgrew the LDS short stack; grid-stride over the queue
+// (its length is read on the device, so no host round trip is needed).
+__global__ __launch_bounds__(kTraceBlock) void k_trace_deep(TraceArgs a) {
+    __shared__ int32_t s_idx[kShortStack][kTraceBlock];
+    __shared__ float s_tent[kShortStack][kTraceBlock];
+    const uint32_t count = *a.overflow_count;
+    for (uint32_t k = blockIdx.x * kTraceBlock + threadIdx.x; k < count; k += gridDim.x * kTraceBlock) {
+        const uint32_t i = a.overflow_list[k];
+        const RayState r = load_ray(a.rays, i);
+        DeepStack st;
+        st.idx = s_idx;
+        st.tent = s_tent;
+        st.lane = (int)threadIdx.x;
+        float t;
+        int32_t p;
+        int32_t c[3] = {0, 0, 0};
+        if (traverse(a, r, st, t, p, c)) write_result(a, i, t, p, c);
+        else atomicOr(a.overflow_count + 1, 1u);  // deeper than 256 or not a tree: reported as an error
+    }
 }
 
 // Mesh SoA -> 3 x float4 per triangle (v0, v1, v2; w = 0).
@@ -299,5 +330,8 @@ void launch_trace(const TraceArgs& a, hipStream_t s) {
     hipLaunchKernelGGL(k_trace, dim3(g ? g : 1), dim3(kTraceBlock), 0, s, a);
 }
 
+void launch_trace_deep(const TraceArgs& a, hipStream_t s) {
+    hipLaunchKernelGGL(k_trace_deep, dim3(1024), dim3(kTraceBlock), 0, s, a);
+}
 
 }  // namespace bvhk

@@ -67,3 +67,32 @@ def test_trace_external_nodes(ctx):
     ot, op, _ = O.trace(onodes, m.soa, rays)
     np.testing.assert_array_equal(prim, op)
     np.testing.assert_array_equal(t.view(np.uint32), ot.view(np.uint32))
+
+
+def _identical(n):
+    return B.Mesh(np.tile(np.array([[0.1], [0.2], [0.3], [0.4], [0.2], [0.3], [0.1], [0.5], [0.3]], np.float32),
+                          (1, n)))
+
+
+def test_trace_deep_chain_uses_full_stack(ctx):
+    # PLOC on identical triangles merges one pair per iteration: a chain about
+    # 200 deep, so rays through it need far more than the 24 LDS entries.
+    m = _identical(200)
+    rays = B.rays_sphere_to_box(5, m.bounds(), 2000)
+    frac, _ = _trace_both(ctx, m, B.ALGO_PLOC, rays, radius=2)
+    assert frac > 0.1
+
+
+def test_trace_reports_too_deep(ctx):
+    m = _identical(600)
+    dm = ctx.upload(m)
+    tree = ctx.tree()
+    try:
+        ctx.build(dm, tree, algo=B.ALGO_PLOC, radius=2)
+        rays = B.rays_sphere_to_box(6, m.bounds(), 500)
+        with pytest.raises(B.NativeError) as ei:
+            ctx.trace(tree, dm, rays)
+        assert ei.value.status == 8
+    finally:
+        tree.free()
+        dm.free()

@@ -379,34 +379,18 @@ __device__ void merge_tiles(const PlocArgs& a, const PlocState& st, int kn, uint
                 *s_prefix = 0;
             } else {
                 __hip_atomic_store(sw, kStA | tagg, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                // Look-back, kLookBatch predecessors per round trip (their
-                // loads overlap; consumed in order, an unpublished word is
-                // read again).
-                constexpr int kLookBatch = 16;
                 uint64_t pre = 0;
                 int t = (int)tile - 1;
-                bool done = false;
-                while (!done) {
-                    uint64_t w[kLookBatch];
-#pragma unroll
-                    for (int q = 0; q < kLookBatch; ++q)
-                        w[q] = (t - q >= 0) ? __hip_atomic_load(a.status + (t - q), __ATOMIC_RELAXED,
-                                                                __HIP_MEMORY_SCOPE_AGENT)
-                                            : 0ull;
-                    bool stalled = false;
-#pragma unroll
-                    for (int q = 0; q < kLookBatch; ++q) {
-                        if (done || stalled) break;
-                        const uint64_t f = w[q] & kStFlags;
-                        if (f == 0) {  // tile t not published yet (t >= 0: tile 0 always ends a walk)
-                            stalled = true;
-                            break;
-                        }
-                        pre = scomb(w[q] & ~kStFlags, pre);
-                        if (f == kStP) done = true;
-                        else --t;
+                while (true) {
+                    const uint64_t w = __hip_atomic_load(a.status + t, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                    const uint64_t f = w & kStFlags;
+                    if (f == 0) {
+                        __builtin_amdgcn_s_sleep(1);
+                        continue;
                     }
-                    if (stalled) __builtin_amdgcn_s_sleep(1);
+                    pre = scomb(w & ~kStFlags, pre);
+                    if (f == kStP) break;
+                    --t;
                 }
                 __hip_atomic_store(sw, kStP | scomb(pre, tagg), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
                 *s_prefix = pre;

@@ -248,14 +248,8 @@ __global__ __launch_bounds__(kSortBlock) void k_onesweep(const uint64_t* __restr
     }
     uint32_t total;
     uint32_t lrun = block_excl_sum256(tsum, s_w, &total);
-    // Decoupled look-back, LB predecessors per step: a thread issues the
-    // status loads of its BPT bins for tiles tj, tj-1, ..., tj-LB+1 together
-    // and consumes them in order (aggregates add, an inclusive prefix ends the
-    // walk, an unpublished word stops the step and is read again).  When all
-    // tiles are resident at once every walk starts together, and the walk
-    // length, not the load latency, sets the pass time; stepping LB tiles per
-    // round trip divides it by LB.
-    constexpr int LB = BPT == 1 ? 16 : 4;
+    // Decoupled look-back: the thread's BPT bins walk back together, so their
+    // status loads overlap.
     uint32_t g[BPT];
     int tj[BPT];
     uint32_t pending = 0;
@@ -266,36 +260,24 @@ __global__ __launch_bounds__(kSortBlock) void k_onesweep(const uint64_t* __restr
         if (tile > 0) pending |= 1u << j;
     }
     while (pending) {
-        uint32_t w[BPT][LB];
+        uint32_t w[BPT];
 #pragma unroll
         for (int j = 0; j < BPT; ++j)
-#pragma unroll
-            for (int q = 0; q < LB; ++q)
-                w[j][q] = (((pending >> j) & 1u) && tj[j] - q >= 0)
-                              ? __hip_atomic_load(status + (size_t)(tj[j] - q) * BINS + tid * BPT + j,
-                                                  __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)
-                              : 0u;
+            w[j] = ((pending >> j) & 1u) ? __hip_atomic_load(status + (size_t)tj[j] * BINS + tid * BPT + j,
+                                                              __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)
+                                          : 0u;
         bool stalled = false;
 #pragma unroll
         for (int j = 0; j < BPT; ++j) {
-            bool go = (pending >> j) & 1u;
-#pragma unroll
-            for (int q = 0; q < LB; ++q) {
-                if (!go) break;
-                const uint32_t flag = w[j][q] & ~kValMask;
-                if (flag == 0) {  // not published yet (a tile that exists: tj >= 0 while pending)
-                    stalled = true;
-                    go = false;
-                    break;
-                }
-                g[j] += w[j][q] & kValMask;
-                if (flag == kFlagP) {
-                    pending &= ~(1u << j);
-                    go = false;
-                } else {
-                    --tj[j];
-                }
+            if (!((pending >> j) & 1u)) continue;
+            const uint32_t flag = w[j] & ~kValMask;
+            if (flag == 0) {
+                stalled = true;
+                continue;
             }
+            g[j] += w[j] & kValMask;
+            if (flag == kFlagP) pending &= ~(1u << j);
+            else --tj[j];
         }
         if (stalled) __builtin_amdgcn_s_sleep(1);
     }

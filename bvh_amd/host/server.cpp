@@ -69,7 +69,8 @@ struct bvh_server_rec {
     }
     void run_slot(int slot);
     bool form_group(bvh_ctx_t own, std::vector<Req>& group);
-    void finish(int slot, std::vector<Req>& group, const std::vector<int>& status);
+    void finish_one(int slot, Req& q, int status, uint32_t gsize);
+    void publish_residents(int slot, std::vector<Req>& group, std::vector<int>& status);
     void complete_locked(const Req& q, int status, int slot, uint32_t gsize);
 };
 
@@ -241,40 +242,52 @@ bool bvh_server_rec::form_group(bvh_ctx_t own, std::vector<Req>& group) {
     return false;
 }
 
-void bvh_server_rec::finish(int slot, std::vector<Req>& group, const std::vector<int>& status) {
+// One request is done (its rays traced, or it failed): release what it held
+// and publish its completion.
+void bvh_server_rec::finish_one(int slot, Req& q, int status, uint32_t gsize) {
     bvh_ctx_t ctx = ctxs[slot];
-    // Unkeyed meshes end with their request; keyed ones that failed to build are dropped.
+    if (q.build && !q.res) {  // unkeyed meshes end with their request
+        if (q.tree) bvh_tree_free(ctx, q.tree);
+        if (q.mesh) bvh_mesh_free(ctx, q.mesh);
+    }
+    std::lock_guard<std::mutex> lk(mu);
+    in_use -= q.transient;
+    if (q.res) q.res->users--;
+    complete_locked(q, status, slot, gsize);
+    in_flight -= 1;
+    cv_work.notify_all();
+}
+
+// Publishes (or drops, on failure) the resident meshes this group built, so
+// follow-up requests on other slots can start before the group's traces end.
+void bvh_server_rec::publish_residents(int slot, std::vector<Req>& group, std::vector<int>& status) {
+    bvh_ctx_t ctx = ctxs[slot];
     for (size_t i = 0; i < group.size(); ++i) {
         Req& q = group[i];
-        if (q.build && !q.res) {
-            if (q.tree) bvh_tree_free(ctx, q.tree);
-            if (q.mesh) bvh_mesh_free(ctx, q.mesh);
+        if (q.build && q.res && status[i] == BVH_OK) {
+            const int st = bvh_tree_prepare(ctx, q.tree);  // traversal records before other slots use it
+            if (st != BVH_OK) status[i] = st;
         }
     }
     std::lock_guard<std::mutex> lk(mu);
     for (size_t i = 0; i < group.size(); ++i) {
         Req& q = group[i];
-        in_use -= q.transient;
-        if (q.res) {
-            if (q.build) {
-                if (status[i] == BVH_OK) {
-                    q.res->mesh = q.mesh;
-                    q.res->tree = q.tree;
-                    q.res->ready = true;
-                } else {
-                    if (q.tree) bvh_tree_free(ctx, q.tree);
-                    if (q.mesh) bvh_mesh_free(ctx, q.mesh);
-                    in_use -= q.res->bytes;
-                    resident_bytes -= q.res->bytes;
-                    residents.erase(q.res->key);
-                    q.res = nullptr;
-                }
-            }
-            if (q.res) q.res->users--;
+        if (!(q.build && q.res)) continue;
+        if (status[i] == BVH_OK) {
+            q.res->mesh = q.mesh;
+            q.res->tree = q.tree;
+            q.res->ready = true;
+        } else {
+            if (q.tree) bvh_tree_free(ctx, q.tree);
+            if (q.mesh) bvh_mesh_free(ctx, q.mesh);
+            q.mesh = nullptr;
+            q.tree = nullptr;
+            in_use -= q.res->bytes;
+            resident_bytes -= q.res->bytes;
+            residents.erase(q.res->key);
+            q.res = nullptr;
         }
-        complete_locked(q, status[i], slot, (uint32_t)group.size());
     }
-    in_flight -= (uint32_t)group.size();
     cv_work.notify_all();
 }
 
@@ -287,6 +300,7 @@ void bvh_server_rec::run_slot(int slot) {
             while (!stop && !form_group(ctx, group)) cv_work.wait(lk);
             if (group.empty()) return;  // stopping
         }
+        const uint32_t gsize = (uint32_t)group.size();
         std::vector<int> status(group.size(), BVH_OK);
         // 1. upload the meshes this group builds
         std::vector<bvh_mesh_t> bm;
@@ -319,21 +333,18 @@ void bvh_server_rec::run_slot(int slot) {
                 for (size_t i : who) status[i] = st;
             }
         }
-        // Resident trees get their traversal records before other slots may use them.
-        for (size_t i = 0; i < group.size(); ++i) {
+        publish_residents(slot, group, status);
+        // 3. each request's rays against its own tree, fewest rays first; a
+        //    request completes as soon as its own rays are done
+        std::vector<size_t> order(group.size());
+        for (size_t i = 0; i < order.size(); ++i) order[i] = i;
+        std::stable_sort(order.begin(), order.end(),
+                         [&](size_t x, size_t y) { return group[x].r.nrays < group[y].r.nrays; });
+        for (size_t i : order) {
             Req& q = group[i];
-            if (status[i] == BVH_OK && q.build && q.res) {
-                int st = bvh_tree_prepare(ctx, q.tree);
-                if (st != BVH_OK) status[i] = st;
-            }
-        }
-        // 3. each request's rays against its own tree
-        for (size_t i = 0; i < group.size(); ++i) {
-            if (status[i] != BVH_OK) continue;
-            Req& q = group[i];
-            bvh_mesh_t m = q.build ? q.mesh : q.res->mesh;
-            bvh_tree_t t = q.build ? q.tree : q.res->tree;
-            if (q.r.nrays) {
+            if (status[i] == BVH_OK && q.r.nrays) {
+                bvh_mesh_t m = q.build ? q.mesh : q.res->mesh;
+                bvh_tree_t t = q.build ? q.tree : q.res->tree;
                 int st = bvh_trace_host(ctx, t, m, q.r.rays, q.r.nrays, q.r.t_out, q.r.prim_out, nullptr, nullptr);
                 if (st != BVH_OK) {
                     std::lock_guard<std::mutex> lk(mu);
@@ -341,8 +352,8 @@ void bvh_server_rec::run_slot(int slot) {
                     status[i] = st;
                 }
             }
+            finish_one(slot, q, status[i], gsize);
         }
-        finish(slot, group, status);
     }
 }
 

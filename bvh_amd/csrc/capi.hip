@@ -68,6 +68,7 @@ struct bvh_context {
     Workspace ws;
     SegTables tabs;  // eager segmented builds
     DevBuf trace_ovf, trace_list, trace_io;  // traversal queue and host-call staging
+    DevBuf ray_keys[2];                      // ray coherence keys (sorted before tracing)
     // last build's sorted keys (for bvh_debug_sorted_keys)
     uint64_t* last_keys = nullptr;
     uint32_t* last_vals = nullptr;
@@ -472,6 +473,9 @@ PlocArgs ploc_args(Workspace& w, const Job& j, int radius) {
     a.out = P<uint4>(w.ploc_out);
     return a;
 }
+
+// Ray sets from this size on are traced in coherence order.
+constexpr uint32_t kRaySortMin = 16384;
 
 // Most global PLOC iterations captured into a batch graph (more run eagerly).
 constexpr int kPlocCaptureMax = 512;
@@ -943,6 +947,8 @@ int bvh_ctx_destroy(bvh_ctx_t ctx) {
     ctx->tabs.release();
     free_buf(ctx->trace_ovf);
     free_buf(ctx->trace_list);
+    free_buf(ctx->ray_keys[0]);
+    free_buf(ctx->ray_keys[1]);
     free_buf(ctx->trace_io);
     for (auto& e : ctx->ev)
         if (e) (void)hipEventDestroy(e);
@@ -1247,6 +1253,33 @@ int bvh_trace(bvh_ctx_t ctx, bvh_tree_t tree, bvh_mesh_t mesh, const float* d_ra
     HIP_TRY(ctx, hipEventRecord(ctx->ev[0], ctx->stream));
     if ((st = prepare_trace(ctx, tree))) return st;
     TraceArgs a;
+    a.order = nullptr;
+    a.order_mask = 0;
+    // Large ray sets are traced in coherence order (direction bins, origin
+    // Morton code): neighbouring lanes then walk similar paths.
+    const int idx_bits = std::max(1, bit_width(nrays - 1));
+    if (nrays >= kRaySortMin && idx_bits + 39 <= 64) {
+        SortPlan plan = make_sort_plan(nrays, idx_bits, idx_bits + 39, 8);
+        Workspace& w = ctx->ws;
+        const size_t bins = 1u << plan.bits;
+        TRY(ensure(ctx, ctx->ray_keys[0], (size_t)nrays * 8));
+        TRY(ensure(ctx, ctx->ray_keys[1], (size_t)nrays * 8));
+        TRY(ensure(ctx, w.hist_part, (size_t)plan.hist_blocks * plan.passes * bins * 4));
+        TRY(ensure(ctx, w.bin_start, (size_t)plan.passes * bins * 4));
+        TRY(ensure(ctx, w.lookback, zero_bytes_of(plan)));
+        launch_ray_keys(d_rays, nrays, P<DNode>(tree->nodes), (uint32_t)idx_bits, P<uint64_t>(ctx->ray_keys[0]),
+                        ctx->stream);
+        launch_sort_histogram(P<uint64_t>(ctx->ray_keys[0]), nrays, plan, P<uint32_t>(w.hist_part), ctx->stream);
+        launch_hist_scan(P<uint32_t>(w.hist_part), plan, P<uint32_t>(w.bin_start), ctx->stream);
+        launch_fill_zero(P<uint32_t>(w.lookback), zero_bytes_of(plan) / 4, ctx->stream);
+        uint64_t* kout = nullptr;
+        uint32_t* vout = nullptr;
+        launch_onesweep(plan, nrays, P<uint64_t>(ctx->ray_keys[0]), P<uint64_t>(ctx->ray_keys[1]), nullptr, nullptr,
+                        P<uint32_t>(w.bin_start), P<uint32_t>(w.lookback), &kout, &vout, ctx->stream);
+        TRY(check_launch(ctx, "ray ordering"));
+        a.order = kout;
+        a.order_mask = (1ull << idx_bits) - 1;
+    }
     a.nodes = P<DNode>(tree->nodes);
     a.rec = P<float4>(tree->rec);
     a.num_nodes = tree->num_nodes;

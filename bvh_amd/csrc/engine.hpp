@@ -124,6 +124,8 @@ struct TraceArgs {
     const float4* rec;          // 4 x float4 per internal node (k_trace_layout)
     const float* rays;          // 6 floats per ray: origin, direction
     uint32_t nrays;
+    const uint64_t* order;      // optional: sorted coherence keys (ray index in the low bits)
+    uint64_t order_mask;
     float* t_out;
     int32_t* prim_out;
     int32_t* counters;          // optional: 3 per ray
@@ -131,6 +133,9 @@ struct TraceArgs {
     uint32_t* overflow_list;    // nrays entries
 };
 void launch_tri_aos(const MeshView& m, float4* tri, hipStream_t s);
+// Coherence keys of rays (direction bins, origin Morton code, ray index).
+void launch_ray_keys(const float* rays, uint32_t n, const DNode* nodes, uint32_t idx_bits, uint64_t* keys,
+                     hipStream_t s);
 // Per-internal-node traversal records (64 B each, indexed by node id).
 void launch_trace_layout(const DNode* nodes, uint32_t num_nodes, float4* rec, hipStream_t s);
 void launch_trace(const TraceArgs& a, hipStream_t s);

@@ -270,11 +270,18 @@ __device__ __forceinline__ void write_result(const TraceArgs& a, uint32_t i, flo
     }
 }
 
+// Ray processed by slot k: the k-th ray of the coherence order (sorted keys
+// carry the ray index in their low bits), or ray k.
+__device__ __forceinline__ uint32_t ray_at(const TraceArgs& a, uint32_t k) {
+    return a.order ? (uint32_t)(a.order[k] & a.order_mask) : k;
+}
+
 __global__ __launch_bounds__(kTraceBlock) void k_trace(TraceArgs a) {
     __shared__ int32_t s_idx[kShortStack][kTraceBlock];
     __shared__ float s_tent[kShortStack][kTraceBlock];
-    const uint32_t i = blockIdx.x * kTraceBlock + threadIdx.x;
-    if (i >= a.nrays) return;
+    const uint32_t k = blockIdx.x * kTraceBlock + threadIdx.x;
+    if (k >= a.nrays) return;
+    const uint32_t i = ray_at(a, k);
     const RayState r = load_ray(a.rays, i);
     LdsStack st{s_idx, s_tent, (int)threadIdx.x};
     float t;
@@ -307,6 +314,42 @@ __global__ __launch_bounds__(kTraceBlock) void k_trace_deep(TraceArgs a) {
         if (traverse(a, r, st, t, p, c)) write_result(a, i, t, p, c);
         else atomicOr(a.overflow_count + 1, 1u);  // deeper than 256 or not a tree: reported as an error
     }
+}
+
+// Coherence keys: 9 bits of direction (3 per axis, direction scaled by its
+// largest component) above a 30-bit Morton code of the origin in the root box,
+// above the ray index; sorting them makes neighbouring lanes trace similar
+// rays.  Order only: every ray's result is independent of where it runs.
+__global__ void k_ray_keys(const float* __restrict__ rays, uint32_t n, const DNode* __restrict__ nodes,
+                           uint32_t idx_bits, uint64_t* __restrict__ keys) {
+    const float4 mn = nodes[0].mn, mx = nodes[0].mx;
+    const uint32_t stride = gridDim.x * blockDim.x;
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+        const float* r = rays + 6 * (size_t)i;
+        const float ox = r[0], oy = r[1], oz = r[2], dx = r[3], dy = r[4], dz = r[5];
+        const float ex = mx.x - mn.x, ey = mx.y - mn.y, ez = mx.z - mn.z;
+        float x = (ox - mn.x) / (ex > 1e-6f ? ex : 1.0f);
+        float y = (oy - mn.y) / (ey > 1e-6f ? ey : 1.0f);
+        float z = (oz - mn.z) / (ez > 1e-6f ? ez : 1.0f);
+        x = fmin_s(fmax_s(x, 0.0f), 1.0f);
+        y = fmin_s(fmax_s(y, 0.0f), 1.0f);
+        z = fmin_s(fmax_s(z, 0.0f), 1.0f);
+        const uint64_t m = morton30(x, y, z);
+        float dm = fmax_s(fabsf(dx), fmax_s(fabsf(dy), fabsf(dz)));
+        dm = dm > 0.0f ? dm : 1.0f;
+        const uint32_t qx = (uint32_t)fmin_s(fmax_s((dx / dm + 1.0f) * 4.0f, 0.0f), 7.0f);
+        const uint32_t qy = (uint32_t)fmin_s(fmax_s((dy / dm + 1.0f) * 4.0f, 0.0f), 7.0f);
+        const uint32_t qz = (uint32_t)fmin_s(fmax_s((dz / dm + 1.0f) * 4.0f, 0.0f), 7.0f);
+        const uint64_t dk = (uint64_t)((qx << 6) | (qy << 3) | qz);
+        keys[i] = (((dk << 30) | m) << idx_bits) | i;
+    }
+}
+
+void launch_ray_keys(const float* rays, uint32_t n, const DNode* nodes, uint32_t idx_bits, uint64_t* keys,
+                     hipStream_t s) {
+    uint32_t g = (n + 255) / 256;
+    if (g > 4096) g = 4096;
+    hipLaunchKernelGGL(k_ray_keys, dim3(g ? g : 1), dim3(256), 0, s, rays, n, nodes, idx_bits, keys);
 }
 
 // Mesh SoA -> 3 x float4 per triangle (v0, v1, v2; w = 0).

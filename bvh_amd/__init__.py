@@ -75,6 +75,29 @@ class Mesh:
         return np.concatenate([v.min(axis=(0, 2)), v.max(axis=(0, 2))]).astype(np.float32)
 
 
+def tree_file_write(path: str, nodes: np.ndarray, indices: np.ndarray) -> None:
+    """Host-only writer of the tree file format (no device)."""
+    nodes = np.ascontiguousarray(nodes, dtype=NODE_DTYPE)
+    indices = np.ascontiguousarray(indices, dtype=np.uint32)
+    st = N.host().bvh_tree_file_write(path.encode(), nodes.ctypes.data, nodes.size, indices.ctypes.data,
+                                      indices.size)
+    if st != 0:
+        raise NativeError(st, f"tree file write {path}")
+
+
+def tree_file_read(path: str):
+    nn, np_ = C.c_uint32(), C.c_uint32()
+    st = N.host().bvh_tree_file_read(path.encode(), None, C.byref(nn), None, C.byref(np_))
+    if st != 0:
+        raise NativeError(st, f"tree file read {path}")
+    nodes = np.zeros(nn.value, dtype=NODE_DTYPE)
+    idx = np.zeros(np_.value, dtype=np.uint32)
+    st = N.host().bvh_tree_file_read(path.encode(), nodes.ctypes.data, C.byref(nn), idx.ctypes.data, C.byref(np_))
+    if st != 0:
+        raise NativeError(st, f"tree file read {path}")
+    return nodes, idx
+
+
 def batch_counts(seed: int, meshes: int, lo: int, hi: int) -> np.ndarray:
     out = np.zeros(meshes, dtype=np.uint32)
     N.host().bvh_scene_batch_counts(seed, meshes, lo, hi, out.ctypes.data)
@@ -285,6 +308,17 @@ class Tree:
         nodes = np.ascontiguousarray(nodes, dtype=NODE_DTYPE)
         self.ctx._check(self.ctx._lib.bvh_tree_from_nodes(self.ctx.h, nodes.ctypes.data, nodes.size, self.h),
                         "tree_from_nodes")
+
+    def save(self, path: str):
+        """Tree file (bvh_io.h): header, 48-byte BVHNode records, primitive indices."""
+        self.ctx._check(N.host().bvh_tree_save(self.ctx.h, self.h, path.encode()), "tree save")
+
+    def load(self, path: str):
+        self.ctx._check(N.host().bvh_tree_load(self.ctx.h, path.encode(), self.h), "tree load")
+
+    def export_viz(self, path: str):
+        """The reference's 32-byte VizNode export (exportBVHToBinary)."""
+        self.ctx._check(N.host().bvh_tree_export_viz(self.ctx.h, self.h, path.encode()), "tree export")
 
     def free(self):
         if self.h:

@@ -98,6 +98,7 @@ _HIP_SIGS = {
     "bvh_tree_info": (_i32, [_vp, C.POINTER(_u32), C.POINTER(_u32)]),
     "bvh_tree_download": (_i32, [_vp, _vp, _vp, _vp]),
     "bvh_tree_from_nodes": (_i32, [_vp, _vp, _u32, _vp]),
+    "bvh_tree_set_indices": (_i32, [_vp, _vp, _vp, _u32]),
     "bvh_build": (_i32, [_vp, _vp, C.POINTER(BuildParams), _vp, C.POINTER(StageTimes)]),
     "bvh_debug_morton": (_i32, [_vp, _vp, _i32, _vp]),
     "bvh_debug_sorted_keys": (_i32, [_vp, _vp, _vp, _u32]),
@@ -125,6 +126,11 @@ _HOST_SIGS = {
     "bvh_rays_sphere_to_box": (_i32, [_u64, _vp, _u32, _vp]),
     "bvh_rays_uniform_cube": (_i32, [_u64, _u32, _vp]),
     "bvh_fnv1a64": (_u64, [_vp, _sz, _u64]),
+    "bvh_tree_file_write": (_i32, [C.c_char_p, _vp, _u32, _vp, _u32]),
+    "bvh_tree_file_read": (_i32, [C.c_char_p, _vp, C.POINTER(_u32), _vp, C.POINTER(_u32)]),
+    "bvh_tree_save": (_i32, [_vp, _vp, C.c_char_p]),
+    "bvh_tree_load": (_i32, [_vp, C.c_char_p, _vp]),
+    "bvh_tree_export_viz": (_i32, [_vp, _vp, C.c_char_p]),
 }
 
 

@@ -1101,6 +1101,18 @@ int bvh_tree_from_nodes(bvh_ctx_t ctx, const void* nodes, uint32_t num_nodes, bv
     return BVH_OK;
 }
 
+int bvh_tree_set_indices(bvh_ctx_t ctx, bvh_tree_t tree, const uint32_t* indices, uint32_t num_prims) {
+    if (!ctx || !tree || !indices || num_prims == 0) return BVH_E_ARG;
+    if (tree->num_nodes != 2 * num_prims - 1)
+        return set_err(ctx, BVH_E_ARG, "%u indices for a tree of %u nodes", num_prims, tree->num_nodes);
+    int st;
+    if ((st = ensure(ctx, tree->indices, (size_t)num_prims * 4))) return st;
+    HIP_TRY(ctx, hipMemcpyAsync(tree->indices.p, indices, (size_t)num_prims * 4, hipMemcpyHostToDevice, ctx->stream));
+    HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+    tree->num_prims = num_prims;
+    return BVH_OK;
+}
+
 int bvh_build(bvh_ctx_t ctx, bvh_mesh_t mesh, const bvh_build_params* params, bvh_tree_t tree,
               bvh_stage_times* times) {
     if (!ctx || !mesh || !params || !tree) return ctx ? set_err(ctx, BVH_E_ARG, "null argument") : BVH_E_ARG;

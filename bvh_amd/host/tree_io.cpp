@@ -1,0 +1,116 @@
+// tree_io.cpp — tree files and the reference's VizNode export (include/bvh_io.h).
+#include "bvh_io.h"
+
+#include <cstdio>
+#include <cstring>
+#include <memory>
+#include <vector>
+
+namespace {
+
+constexpr char kMagic[8] = {'B', 'V', 'H', 'A', 'M', 'D', '0', '1'};
+constexpr uint32_t kVersion = 1;
+constexpr size_t kNodeBytes = 48;
+
+struct FileCloser {
+    void operator()(FILE* f) const {
+        if (f) std::fclose(f);
+    }
+};
+using File = std::unique_ptr<FILE, FileCloser>;
+
+// The reference's VizNode (src/main.cpp:24-29).
+struct VizNode {
+    float mn[3];
+    float mx[3];
+    int32_t left;
+    int32_t right;
+};
+static_assert(sizeof(VizNode) == 32, "VizNode is 32 bytes");
+
+}  // namespace
+
+extern "C" {
+
+int bvh_tree_file_write(const char* path, const void* nodes, uint32_t num_nodes, const uint32_t* indices,
+                        uint32_t num_prims) {
+    if (!path || !nodes || num_nodes == 0 || (num_prims && !indices)) return BVH_E_ARG;
+    File f(std::fopen(path, "wb"));
+    if (!f) return BVH_E_ARG;
+    const uint32_t head[4] = {kVersion, num_nodes, num_prims, 0u};
+    bool ok = std::fwrite(kMagic, 1, 8, f.get()) == 8 && std::fwrite(head, 4, 4, f.get()) == 4 &&
+              std::fwrite(nodes, kNodeBytes, num_nodes, f.get()) == num_nodes &&
+              (num_prims == 0 || std::fwrite(indices, 4, num_prims, f.get()) == num_prims);
+    ok = ok && std::fflush(f.get()) == 0;
+    return ok ? BVH_OK : BVH_E_STATE;
+}
+
+int bvh_tree_file_read(const char* path, void* nodes, uint32_t* num_nodes, uint32_t* indices, uint32_t* num_prims) {
+    if (!path || !num_nodes || !num_prims) return BVH_E_ARG;
+    File f(std::fopen(path, "rb"));
+    if (!f) return BVH_E_ARG;
+    char magic[8];
+    uint32_t head[4];
+    if (std::fread(magic, 1, 8, f.get()) != 8 || std::memcmp(magic, kMagic, 8) != 0) return BVH_E_STATE;
+    if (std::fread(head, 4, 4, f.get()) != 4 || head[0] != kVersion || head[1] == 0) return BVH_E_STATE;
+    if (!nodes) {  // sizes only
+        *num_nodes = head[1];
+        *num_prims = head[2];
+        return BVH_OK;
+    }
+    if (*num_nodes < head[1] || (head[2] && (!indices || *num_prims < head[2]))) return BVH_E_ARG;
+    if (std::fread(nodes, kNodeBytes, head[1], f.get()) != head[1]) return BVH_E_STATE;
+    if (head[2] && std::fread(indices, 4, head[2], f.get()) != head[2]) return BVH_E_STATE;
+    *num_nodes = head[1];
+    *num_prims = head[2];
+    return BVH_OK;
+}
+
+int bvh_tree_save(bvh_ctx_t ctx, bvh_tree_t tree, const char* path) {
+    if (!ctx || !tree || !path) return BVH_E_ARG;
+    uint32_t nn = 0, np = 0;
+    int st = bvh_tree_info(tree, &nn, &np);
+    if (st != BVH_OK) return st;
+    if (nn == 0) return BVH_E_STATE;
+    std::vector<unsigned char> nodes((size_t)nn * kNodeBytes);
+    std::vector<uint32_t> idx(np);
+    if ((st = bvh_tree_download(ctx, tree, nodes.data(), np ? idx.data() : nullptr)) != BVH_OK) return st;
+    return bvh_tree_file_write(path, nodes.data(), nn, idx.data(), np);
+}
+
+int bvh_tree_load(bvh_ctx_t ctx, const char* path, bvh_tree_t tree) {
+    if (!ctx || !tree || !path) return BVH_E_ARG;
+    uint32_t nn = 0, np = 0;
+    int st = bvh_tree_file_read(path, nullptr, &nn, nullptr, &np);
+    if (st != BVH_OK) return st;
+    std::vector<unsigned char> nodes((size_t)nn * kNodeBytes);
+    std::vector<uint32_t> idx(np);
+    if ((st = bvh_tree_file_read(path, nodes.data(), &nn, idx.data(), &np)) != BVH_OK) return st;
+    if ((st = bvh_tree_from_nodes(ctx, nodes.data(), nn, tree)) != BVH_OK) return st;
+    if (np) st = bvh_tree_set_indices(ctx, tree, idx.data(), np);
+    return st;
+}
+
+int bvh_tree_export_viz(bvh_ctx_t ctx, bvh_tree_t tree, const char* path) {
+    if (!ctx || !tree || !path) return BVH_E_ARG;
+    uint32_t nn = 0, np = 0;
+    int st = bvh_tree_info(tree, &nn, &np);
+    if (st != BVH_OK) return st;
+    if (nn == 0) return BVH_E_STATE;
+    std::vector<unsigned char> raw((size_t)nn * kNodeBytes);
+    if ((st = bvh_tree_download(ctx, tree, raw.data(), nullptr)) != BVH_OK) return st;
+    std::vector<VizNode> out(nn);
+    for (uint32_t i = 0; i < nn; ++i) {
+        const unsigned char* p = raw.data() + (size_t)i * kNodeBytes;
+        std::memcpy(out[i].mn, p + 0, 12);   // bbox.min (x, y, z, pad)
+        std::memcpy(out[i].mx, p + 16, 12);  // bbox.max
+        std::memcpy(&out[i].left, p + 32, 4);
+        std::memcpy(&out[i].right, p + 36, 4);
+    }
+    File f(std::fopen(path, "wb"));
+    if (!f) return BVH_E_ARG;
+    if (std::fwrite(out.data(), sizeof(VizNode), nn, f.get()) != nn) return BVH_E_STATE;
+    return std::fflush(f.get()) == 0 ? BVH_OK : BVH_E_STATE;
+}
+
+}  // extern "C"

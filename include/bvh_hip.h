@@ -115,6 +115,8 @@ int bvh_tree_info(bvh_tree_t tree, uint32_t* num_nodes, uint32_t* num_prims);
 int bvh_tree_download(bvh_ctx_t ctx, bvh_tree_t tree, void* nodes, uint32_t* indices);
 /* Wraps a host node array (e.g. from another builder) into a device tree for tracing. */
 int bvh_tree_from_nodes(bvh_ctx_t ctx, const void* nodes, uint32_t num_nodes, bvh_tree_t tree);
+/* Sets the primitive index array of such a tree (num_nodes must be 2 * num_prims - 1). */
+int bvh_tree_set_indices(bvh_ctx_t ctx, bvh_tree_t tree, const uint32_t* indices, uint32_t num_prims);
 
 /* ---- build -------------------------------------------------------------- */
 /* Synchronous: returns once the tree is complete; times may be NULL. */

@@ -1,0 +1,42 @@
+/*
+ * bvh_io.h — persisting trees (libbvhhost.so, host C++ over the C ABI).
+ *
+ *   bvh_tree_export_viz   the reference's exportBVHToBinary (src/main.cpp:24-29
+ *                         VizNode, :43-64): 32 bytes per node (min xyz, max xyz,
+ *                         left, right), no header, node order as built
+ *   bvh_tree_save/load    a resident tree to/from a file, so a server can keep
+ *                         BVHs across restarts (no reference counterpart)
+ *   bvh_tree_file_*       the same file format on host arrays (no device)
+ *
+ * Tree file: "BVHAMD01" (8 bytes), uint32 version (1), uint32 num_nodes,
+ * uint32 num_prims, uint32 reserved (0), then num_nodes 48-byte BVHNode
+ * records, then num_prims uint32 primitive indices (little-endian).
+ */
+#ifndef BVH_IO_H
+#define BVH_IO_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#include "bvh_hip.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Host arrays <-> file.  file_read with nodes == NULL only reports the sizes. */
+int bvh_tree_file_write(const char* path, const void* nodes, uint32_t num_nodes, const uint32_t* indices,
+                        uint32_t num_prims);
+int bvh_tree_file_read(const char* path, void* nodes, uint32_t* num_nodes, uint32_t* indices, uint32_t* num_prims);
+
+/* Device trees <-> file. */
+int bvh_tree_save(bvh_ctx_t ctx, bvh_tree_t tree, const char* path);
+int bvh_tree_load(bvh_ctx_t ctx, const char* path, bvh_tree_t tree);
+
+/* The reference's 32-byte VizNode export of a device tree. */
+int bvh_tree_export_viz(bvh_ctx_t ctx, bvh_tree_t tree, const char* path);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* BVH_IO_H */

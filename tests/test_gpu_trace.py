@@ -96,3 +96,34 @@ def test_trace_reports_too_deep(ctx):
     finally:
         tree.free()
         dm.free()
+
+
+def test_tree_save_load_and_viz_export(ctx, tmp_path):
+    m = B.Mesh.blobs(12, 30000)
+    dm = ctx.upload(m)
+    t1, t2 = ctx.tree(), ctx.tree()
+    try:
+        ctx.build(dm, t1, algo=B.ALGO_PLOC, radius=16)
+        nodes, idx = t1.download()
+        p = str(tmp_path / "t.bvh")
+        t1.save(p)
+        t2.load(p)
+        n2, i2 = t2.download()
+        assert n2.tobytes() == nodes.tobytes()
+        np.testing.assert_array_equal(i2, idx)
+        rays = B.rays_sphere_to_box(12, m.bounds(), 20000)
+        ta, pa, _ = ctx.trace(t1, dm, rays)
+        tb, pb, _ = ctx.trace(t2, dm, rays)
+        np.testing.assert_array_equal(pa, pb)
+        assert ta.tobytes() == tb.tobytes()
+        v = str(tmp_path / "t.viz")
+        t1.export_viz(v)
+        viz = np.fromfile(v, dtype=np.dtype([("mn", "<f4", (3,)), ("mx", "<f4", (3,)), ("l", "<i4"), ("r", "<i4")]))
+        np.testing.assert_array_equal(viz["mn"], nodes["bmin"])
+        np.testing.assert_array_equal(viz["mx"], nodes["bmax"])
+        np.testing.assert_array_equal(viz["l"], nodes["left"])
+        np.testing.assert_array_equal(viz["r"], nodes["right"])
+    finally:
+        t1.free()
+        t2.free()
+        dm.free()

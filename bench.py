@@ -339,7 +339,7 @@ def main():
         per_stage[name] = {"ms": round(ms, 4), "bytes_per_unit": ub,
                            "GBps": round(gbs, 1) if gbs else None,
                            "frac_hbm": round(gbs / HBM_PEAK_GBS, 4) if gbs else None}
-        kname = {"Bounds/Centroids": "k_tri_bounds", "Morton Codes": "k_morton+k_hist_reduce+k_hist_scan",
+        kname = {"Bounds/Centroids": "k_tri_bounds", "Morton Codes": "k_morton",
                  "Topology+Refit": "k_hierarchy"}.get(name, "k_onesweep" if name.startswith("Sort Pass") else name)
         k = kernels.setdefault(kname, {"ms": 0.0, "launches": 0, "unit": ub})
         k["ms"] += ms

@@ -32,13 +32,13 @@ struct DevBuf {
 
 // Device workspace of one build pipeline.
 struct Workspace {
-    DevBuf tri_boxes, keys_a, keys_b, vals_a, vals_b, hist_part, bin_start, lookback, scene, handoff, epoch;
+    DevBuf tri_boxes, keys_a, keys_b, vals_a, vals_b, hist_tot, lookback, scene, handoff, epoch;
     DevBuf parents, ranges, counters;  // LBVH+
     DevBuf ploc_box[2], ploc_id[2], ploc_sid[2], ploc_seg[2], ploc_nn, ploc_state, ploc_status, ploc_out;
     template <class F>
     void each(F f) {
-        DevBuf* all[] = {&tri_boxes,   &keys_a,      &keys_b,      &vals_a,      &vals_b,     &hist_part,
-                         &bin_start,   &lookback,    &scene,       &handoff,     &epoch,      &parents,
+        DevBuf* all[] = {&tri_boxes,   &keys_a,      &keys_b,      &vals_a,      &vals_b,     &hist_tot,
+                         &lookback,    &scene,       &handoff,     &epoch,       &parents,
                          &ranges,      &counters,    &ploc_box[0], &ploc_box[1], &ploc_id[0], &ploc_id[1],
                          &ploc_sid[0], &ploc_sid[1], &ploc_seg[0], &ploc_seg[1], &ploc_nn,    &ploc_state,
                          &ploc_status, &ploc_out};
@@ -300,8 +300,7 @@ int reserve_front(bvh_ctx_t ctx, Workspace& w, const Job& j, const SortPlan& pla
         TRY(ensure(ctx, w.vals_a, n * 4));
         TRY(ensure(ctx, w.vals_b, n * 4));
     }
-    TRY(ensure(ctx, w.hist_part, (size_t)plan.hist_blocks * plan.passes * bins * 4));
-    TRY(ensure(ctx, w.bin_start, (size_t)plan.passes * bins * 4));
+    TRY(ensure(ctx, w.hist_tot, (size_t)plan.passes * bins * 4));
     TRY(ensure(ctx, w.lookback, zero_bytes_of(plan)));
     TRY(ensure(ctx, w.scene, (size_t)j.k * 32));
     TRY(ensure(ctx, w.handoff, n * 8, true));
@@ -313,14 +312,14 @@ int reserve_front(bvh_ctx_t ctx, Workspace& w, const Job& j, const SortPlan& pla
 int enqueue_front(bvh_ctx_t ctx, Workspace& w, const Job& j, const SortPlan& plan, StageClock* clk,
                   uint64_t** keys_out, uint32_t** vals_out) {
     hipStream_t s = ctx->stream;
-    launch_init_build(P<uint32_t>(w.scene), j.k, P<uint32_t>(w.epoch), s);
+    launch_init_build(P<uint32_t>(w.scene), j.k, P<uint32_t>(w.epoch), s, P<uint32_t>(w.hist_tot),
+                      (uint32_t)plan.passes << plan.bits);
     launch_tri_bounds(j.fsb, P<float4>(w.tri_boxes), P<uint32_t>(w.scene), s);
     TRY(check_launch(ctx, "k_tri_bounds"));
     TRY(mark(clk, "Bounds/Centroids"));
     launch_morton(j.fsb, P<float4>(w.tri_boxes), P<uint32_t>(w.scene), j.m60, j.kf, P<uint64_t>(w.keys_a),
-                  j.m60 ? P<uint32_t>(w.vals_a) : nullptr, plan, P<uint32_t>(w.hist_part), P<uint32_t>(w.lookback),
+                  j.m60 ? P<uint32_t>(w.vals_a) : nullptr, plan, P<uint32_t>(w.hist_tot), P<uint32_t>(w.lookback),
                   zero_bytes_of(plan) / 4, s);
-    launch_hist_scan(P<uint32_t>(w.hist_part), plan, P<uint32_t>(w.bin_start), s);
     TRY(check_launch(ctx, "k_morton"));
     TRY(mark(clk, "Morton Codes"));
     static const char* pass_names[8] = {"Sort Pass 1", "Sort Pass 2", "Sort Pass 3", "Sort Pass 4",
@@ -330,7 +329,7 @@ int enqueue_front(bvh_ctx_t ctx, Workspace& w, const Job& j, const SortPlan& pla
     uint32_t* vin = j.m60 ? P<uint32_t>(w.vals_a) : nullptr;
     uint32_t* vout = j.m60 ? P<uint32_t>(w.vals_b) : nullptr;
     for (int p = 0; p < plan.passes; ++p) {
-        launch_onesweep_pass(plan, p, j.total, kin, kout, vin, vout, P<uint32_t>(w.bin_start),
+        launch_onesweep_pass(plan, p, j.total, kin, kout, vin, vout, P<uint32_t>(w.hist_tot),
                              P<uint32_t>(w.lookback), s);
         TRY(check_launch(ctx, "k_onesweep"));
         TRY(mark(clk, pass_names[p]));
@@ -1276,18 +1275,16 @@ int bvh_trace(bvh_ctx_t ctx, bvh_tree_t tree, bvh_mesh_t mesh, const float* d_ra
         const size_t bins = 1u << plan.bits;
         TRY(ensure(ctx, ctx->ray_keys[0], (size_t)nrays * 8));
         TRY(ensure(ctx, ctx->ray_keys[1], (size_t)nrays * 8));
-        TRY(ensure(ctx, w.hist_part, (size_t)plan.hist_blocks * plan.passes * bins * 4));
-        TRY(ensure(ctx, w.bin_start, (size_t)plan.passes * bins * 4));
+        TRY(ensure(ctx, w.hist_tot, (size_t)plan.passes * bins * 4));
         TRY(ensure(ctx, w.lookback, zero_bytes_of(plan)));
         launch_ray_keys(d_rays, nrays, P<DNode>(tree->nodes), (uint32_t)idx_bits, P<uint64_t>(ctx->ray_keys[0]),
                         ctx->stream);
-        launch_sort_histogram(P<uint64_t>(ctx->ray_keys[0]), nrays, plan, P<uint32_t>(w.hist_part), ctx->stream);
-        launch_hist_scan(P<uint32_t>(w.hist_part), plan, P<uint32_t>(w.bin_start), ctx->stream);
+        launch_sort_histogram(P<uint64_t>(ctx->ray_keys[0]), nrays, plan, P<uint32_t>(w.hist_tot), ctx->stream);
         launch_fill_zero(P<uint32_t>(w.lookback), zero_bytes_of(plan) / 4, ctx->stream);
         uint64_t* kout = nullptr;
         uint32_t* vout = nullptr;
         launch_onesweep(plan, nrays, P<uint64_t>(ctx->ray_keys[0]), P<uint64_t>(ctx->ray_keys[1]), nullptr, nullptr,
-                        P<uint32_t>(w.bin_start), P<uint32_t>(w.lookback), &kout, &vout, ctx->stream);
+                        P<uint32_t>(w.hist_tot), P<uint32_t>(w.lookback), &kout, &vout, ctx->stream);
         TRY(check_launch(ctx, "ray ordering"));
         a.order = kout;
         a.order_mask = (1ull << idx_bits) - 1;
@@ -1367,17 +1364,15 @@ int bvh_sort_pairs_u64(bvh_ctx_t ctx, uint64_t* d_keys, uint32_t* d_vals, uint32
     HIP_TRY(ctx, hipSetDevice(ctx->device));
     TRY(ensure(ctx, w.keys_b, (size_t)n * 8));
     if (d_vals) TRY(ensure(ctx, w.vals_b, (size_t)n * 4));
-    TRY(ensure(ctx, w.hist_part, (size_t)plan.hist_blocks * plan.passes * bins * 4));
-    TRY(ensure(ctx, w.bin_start, (size_t)plan.passes * bins * 4));
+    TRY(ensure(ctx, w.hist_tot, (size_t)plan.passes * bins * 4));
     TRY(ensure(ctx, w.lookback, zero_bytes));
     HIP_TRY(ctx, hipEventRecord(ctx->ev[0], ctx->stream));
-    launch_sort_histogram(d_keys, n, plan, P<uint32_t>(w.hist_part), ctx->stream);
-    launch_hist_scan(P<uint32_t>(w.hist_part), plan, P<uint32_t>(w.bin_start), ctx->stream);
+    launch_sort_histogram(d_keys, n, plan, P<uint32_t>(w.hist_tot), ctx->stream);
     launch_fill_zero(P<uint32_t>(w.lookback), zero_bytes / 4, ctx->stream);
     uint64_t* kout = nullptr;
     uint32_t* vout = nullptr;
     launch_onesweep(plan, n, d_keys, P<uint64_t>(w.keys_b), d_vals, d_vals ? P<uint32_t>(w.vals_b) : nullptr,
-                    P<uint32_t>(w.bin_start), P<uint32_t>(w.lookback), &kout, &vout, ctx->stream);
+                    P<uint32_t>(w.hist_tot), P<uint32_t>(w.lookback), &kout, &vout, ctx->stream);
     TRY(check_launch(ctx, "k_onesweep"));
     HIP_TRY(ctx, hipEventRecord(ctx->ev[1], ctx->stream));
     if (kout != d_keys) {

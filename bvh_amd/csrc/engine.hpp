@@ -46,8 +46,10 @@ struct KeyFormat {
 // ---- LBVH/LBVH+/PLOC front end (k_front.hip) ----------------------------
 constexpr uint32_t kFrontChunk = 4096;   // triangles per block of the front kernels
 
-// Resets the per-segment scene words (8 per segment) and advances *epoch.
-void launch_init_build(uint32_t* seg_scene, uint32_t k, uint32_t* epoch, hipStream_t s);
+// Resets the per-segment scene words (8 per segment), clears hist_words words
+// of the digit-total table (when given) and advances *epoch.
+void launch_init_build(uint32_t* seg_scene, uint32_t k, uint32_t* epoch, hipStream_t s, uint32_t* hist_tot = nullptr,
+                       uint32_t hist_words = 0);
 // Per-triangle boxes (2 x float4 per triangle) and each segment's scene box.
 void launch_tri_bounds(const SegBlocks& sb, float4* tri_boxes, uint32_t* seg_scene, hipStream_t s);
 
@@ -70,33 +72,31 @@ SortPlan make_sort_plan(uint32_t n, int begin_bit, int end_bit, int bits, int hi
 // Bytes of the zero-initialised lookback area a plan needs (status words + tile counters).
 size_t sort_zero_bytes(const SortPlan& p);
 
-// Morton keys + fused all-pass digit histograms (one row per block of sb) +
-// zeroing of `zero_words` words.
+// Morton keys + all-pass digit totals added into hist_tot (cleared by
+// launch_init_build) + zeroing of `zero_words` words.
 //   m60 = false: keys = packed per KeyFormat (vals unused)
 //   m60 = true : keys = morton60, vals = index (single segment only)
 void launch_morton(const SegBlocks& sb, const float4* tri_boxes, const uint32_t* seg_scene, bool m60,
-                   const KeyFormat& kf, uint64_t* keys, uint32_t* vals, const SortPlan& plan, uint32_t* hist_part,
+                   const KeyFormat& kf, uint64_t* keys, uint32_t* vals, const SortPlan& plan, uint32_t* hist_tot,
                    uint32_t* zero_ptr, size_t zero_words, hipStream_t s);
 // Debug: Morton codes only (no histograms), in input order.
 void launch_morton_codes(const SegBlocks& sb, const float4* tri_boxes, const uint32_t* seg_scene, bool m60,
                          uint64_t* codes, hipStream_t s);
 
 // ---- radix sort (k_sort.hip) ---------------------------------------------
-// Histogram of keys already in memory (used by the standalone sort).
-void launch_sort_histogram(const uint64_t* keys, uint32_t n, const SortPlan& plan, uint32_t* hist_part,
+// Digit totals (passes x bins) of keys already in memory (standalone and ray
+// sorts); clears the table first.
+void launch_sort_histogram(const uint64_t* keys, uint32_t n, const SortPlan& plan, uint32_t* hist_tot,
                            hipStream_t s);
-// Reduces the per-block histograms and scans them into per-pass bin starts.
-// (Row 0 of hist_part is overwritten with the column totals.)
-void launch_hist_scan(uint32_t* hist_part, const SortPlan& plan, uint32_t* bin_start, hipStream_t s);
 void launch_fill_zero(uint32_t* p, size_t words, hipStream_t s);
 void launch_stream_copy(const void* src, void* dst, size_t bytes, hipStream_t s);
 // All onesweep passes.  Result ends in (*keys_out, *vals_out) which is one of
 // the two buffers passed in.
 void launch_onesweep_pass(const SortPlan& plan, int pass, uint32_t n, const uint64_t* kin, uint64_t* kout,
-                          const uint32_t* vin, uint32_t* vout, const uint32_t* bin_start, uint32_t* lookback,
+                          const uint32_t* vin, uint32_t* vout, const uint32_t* hist_tot, uint32_t* lookback,
                           hipStream_t s);
 void launch_onesweep(const SortPlan& plan, uint32_t n, uint64_t* keys_a, uint64_t* keys_b, uint32_t* vals_a,
-                     uint32_t* vals_b, const uint32_t* bin_start, uint32_t* lookback, uint64_t** keys_out,
+                     uint32_t* vals_b, const uint32_t* hist_tot, uint32_t* lookback, uint64_t** keys_out,
                      uint32_t** vals_out, hipStream_t s);
 
 // ---- hierarchy (k_hier.hip) ------------------------------------------------

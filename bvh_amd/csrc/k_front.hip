@@ -40,19 +40,25 @@ __device__ __forceinline__ void block_segment(const SegBlocks& sb, uint32_t& seg
 }
 
 // Per-segment scene block: words 0-2 order-mapped min, 3-5 max, 6 non-finite
-// flag, 7 internal-error flag.  Also advances the build epoch.
-__global__ void k_init_build(uint32_t* seg_scene, uint32_t k, uint32_t* epoch) {
+// flag, 7 internal-error flag.  Also clears the sort's digit-total table and
+// advances the build epoch.
+__global__ void k_init_build(uint32_t* seg_scene, uint32_t k, uint32_t* epoch, uint32_t* hist_tot,
+                             uint32_t hist_words) {
     const uint32_t stride = gridDim.x * blockDim.x;
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < k * 8; i += stride) {
         const uint32_t w = i & 7;
         seg_scene[i] = (w < 3) ? 0xFFFFFFFFu : 0u;
     }
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < hist_words; i += stride) hist_tot[i] = 0u;
     if (blockIdx.x == 0 && threadIdx.x == 0 && epoch) *epoch += 1u;
 }
 
-void launch_init_build(uint32_t* seg_scene, uint32_t k, uint32_t* epoch, hipStream_t s) {
-    uint32_t g = (k * 8 + 255) / 256;
-    hipLaunchKernelGGL(k_init_build, dim3(g ? g : 1), dim3(256), 0, s, seg_scene, k, epoch);
+void launch_init_build(uint32_t* seg_scene, uint32_t k, uint32_t* epoch, hipStream_t s, uint32_t* hist_tot,
+                       uint32_t hist_words) {
+    const uint32_t words = (k * 8 > hist_words) ? k * 8 : hist_words;
+    uint32_t g = (words + 255) / 256;
+    hipLaunchKernelGGL(k_init_build, dim3(g ? g : 1), dim3(256), 0, s, seg_scene, k, epoch, hist_tot,
+                       hist_tot ? hist_words : 0u);
 }
 
 __device__ __forceinline__ uint32_t nonfinite(float f) {
@@ -153,12 +159,13 @@ struct Shifts {
     uint32_t m[8];
 };
 
-// Morton keys + per-block digit histograms for all passes + zeroing.
+// Morton keys + digit totals of all passes (added into a table cleared by
+// k_init_build) + zeroing of the look-back words.
 template <int BITS, bool M60>
 __global__ __launch_bounds__(kFrontBlock) void k_morton(SegBlocks sb, const float4* __restrict__ tri_boxes,
                                                         const uint32_t* __restrict__ seg_scene, KeyFormat kf,
                                                         uint64_t* __restrict__ keys, uint32_t* __restrict__ vals,
-                                                        int passes, Shifts sh, uint32_t* __restrict__ hist_part,
+                                                        int passes, Shifts sh, uint32_t* __restrict__ hist_tot,
                                                         uint4* __restrict__ zero_ptr, size_t zero_vec4) {
     constexpr int BINS = 1 << BITS;
     extern __shared__ uint32_t s_hist[];  // passes * BINS
@@ -199,13 +206,15 @@ __global__ __launch_bounds__(kFrontBlock) void k_morton(SegBlocks sb, const floa
         }
     }
     __syncthreads();
-    uint32_t* out = hist_part + (size_t)blockIdx.x * passes * BINS;
-    for (int i = threadIdx.x; i < passes * BINS; i += kFrontBlock) out[i] = s_hist[i];
+    for (int i = threadIdx.x; i < passes * BINS; i += kFrontBlock) {
+        const uint32_t c = s_hist[i];
+        if (c) atomicAdd(hist_tot + i, c);
+    }
 }
 
 template <int BITS, bool M60>
 static void morton_launch(const SegBlocks& sb, const float4* tri_boxes, const uint32_t* seg_scene, const KeyFormat& kf,
-                          uint64_t* keys, uint32_t* vals, const SortPlan& plan, uint32_t* hist_part,
+                          uint64_t* keys, uint32_t* vals, const SortPlan& plan, uint32_t* hist_tot,
                           uint32_t* zero_ptr, size_t zero_words, hipStream_t s) {
     Shifts sh;
     for (int i = 0; i < 8; ++i) {
@@ -214,18 +223,18 @@ static void morton_launch(const SegBlocks& sb, const float4* tri_boxes, const ui
     }
     size_t lds = (size_t)plan.passes * (1u << BITS) * 4;
     hipLaunchKernelGGL((k_morton<BITS, M60>), dim3(sb.nblocks), dim3(kFrontBlock), lds, s, sb, tri_boxes, seg_scene, kf,
-                       keys, vals, plan.passes, sh, hist_part, reinterpret_cast<uint4*>(zero_ptr), zero_words / 4);
+                       keys, vals, plan.passes, sh, hist_tot, reinterpret_cast<uint4*>(zero_ptr), zero_words / 4);
 }
 
 void launch_morton(const SegBlocks& sb, const float4* tri_boxes, const uint32_t* seg_scene, bool m60,
-                   const KeyFormat& kf, uint64_t* keys, uint32_t* vals, const SortPlan& plan, uint32_t* hist_part,
+                   const KeyFormat& kf, uint64_t* keys, uint32_t* vals, const SortPlan& plan, uint32_t* hist_tot,
                    uint32_t* zero_ptr, size_t zero_words, hipStream_t s) {
     if (plan.bits == 8) {
-        if (m60) morton_launch<8, true>(sb, tri_boxes, seg_scene, kf, keys, vals, plan, hist_part, zero_ptr, zero_words, s);
-        else morton_launch<8, false>(sb, tri_boxes, seg_scene, kf, keys, vals, plan, hist_part, zero_ptr, zero_words, s);
+        if (m60) morton_launch<8, true>(sb, tri_boxes, seg_scene, kf, keys, vals, plan, hist_tot, zero_ptr, zero_words, s);
+        else morton_launch<8, false>(sb, tri_boxes, seg_scene, kf, keys, vals, plan, hist_tot, zero_ptr, zero_words, s);
     } else {
-        if (m60) morton_launch<10, true>(sb, tri_boxes, seg_scene, kf, keys, vals, plan, hist_part, zero_ptr, zero_words, s);
-        else morton_launch<10, false>(sb, tri_boxes, seg_scene, kf, keys, vals, plan, hist_part, zero_ptr, zero_words, s);
+        if (m60) morton_launch<10, true>(sb, tri_boxes, seg_scene, kf, keys, vals, plan, hist_tot, zero_ptr, zero_words, s);
+        else morton_launch<10, false>(sb, tri_boxes, seg_scene, kf, keys, vals, plan, hist_tot, zero_ptr, zero_words, s);
     }
 }
 

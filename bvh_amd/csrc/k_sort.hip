@@ -5,9 +5,11 @@
 // and thrust::sort_by_key (lbvh_plus_builder.cu:465, ploc_builder.cu:459-461).
 //
 // Design for gfx950:
-//   * digit histograms of all passes come precomputed from the Morton pass
-//     (k_front.hip) or from k_sort_histogram; k_hist_scan turns them into
-//     per-pass bin starts;
+//   * digit totals of all passes come precomputed from the Morton pass
+//     (k_front.hip) or from k_sort_histogram: every producer block adds its
+//     LDS histogram into one zeroed table with global atomics, and each
+//     onesweep tile turns its pass's row into bin starts with one block scan
+//     (no separate reduce or scan launch);
 //   * each onesweep tile (256 threads x 16 keys) takes a tile id from an
 //     atomic counter (ids follow start order, so every look-back target is
 //     already resident), ranks keys with wave64 ballot matching into
@@ -59,7 +61,7 @@ struct ShiftArr {
 
 template <int BITS>
 __global__ __launch_bounds__(256) void k_sort_histogram(const uint64_t* __restrict__ keys, uint32_t n, int passes,
-                                                         ShiftArr sh, uint32_t* __restrict__ hist_part) {
+                                                         ShiftArr sh, uint32_t* __restrict__ hist_tot) {
     constexpr int BINS = 1 << BITS;
     extern __shared__ uint32_t s_hist[];
     for (int i = threadIdx.x; i < passes * BINS; i += 256) s_hist[i] = 0;
@@ -70,12 +72,15 @@ __global__ __launch_bounds__(256) void k_sort_histogram(const uint64_t* __restri
         for (int p = 0; p < passes; ++p) atomicAdd(&s_hist[p * BINS + ((uint32_t)(k >> sh.s[p]) & sh.m[p])], 1u);
     }
     __syncthreads();
-    uint32_t* out = hist_part + (size_t)blockIdx.x * passes * BINS;
-    for (int i = threadIdx.x; i < passes * BINS; i += 256) out[i] = s_hist[i];
+    for (int i = threadIdx.x; i < passes * BINS; i += 256) {
+        const uint32_t c = s_hist[i];
+        if (c) atomicAdd(hist_tot + i, c);
+    }
 }
 
-void launch_sort_histogram(const uint64_t* keys, uint32_t n, const SortPlan& plan, uint32_t* hist_part,
+void launch_sort_histogram(const uint64_t* keys, uint32_t n, const SortPlan& plan, uint32_t* hist_tot,
                            hipStream_t s) {
+    launch_fill_zero(hist_tot, (size_t)plan.passes * (1u << plan.bits), s);
     ShiftArr sh;
     for (int i = 0; i < 8; ++i) {
         sh.s[i] = plan.shift[i];
@@ -84,10 +89,10 @@ void launch_sort_histogram(const uint64_t* keys, uint32_t n, const SortPlan& pla
     size_t lds = (size_t)plan.passes * (1u << plan.bits) * 4;
     if (plan.bits == 8)
         hipLaunchKernelGGL(k_sort_histogram<8>, dim3(plan.hist_blocks), dim3(256), lds, s, keys, n, plan.passes, sh,
-                           hist_part);
+                           hist_tot);
     else
         hipLaunchKernelGGL(k_sort_histogram<10>, dim3(plan.hist_blocks), dim3(256), lds, s, keys, n, plan.passes, sh,
-                           hist_part);
+                           hist_tot);
 }
 
 // Exclusive sum over a 256-thread block; returns the prefix, *total the sum.
@@ -108,67 +113,11 @@ __device__ __forceinline__ uint32_t block_excl_sum256(uint32_t v, uint32_t* s_w,
     return before + incl - v;
 }
 
-// Column sums of the per-block histograms: each block owns 64 (pass, bin)
-// columns and its 4 waves split the producer blocks; partial sums meet in LDS.
-// The totals overwrite row 0 of hist_part.
-__global__ __launch_bounds__(256) void k_hist_reduce(uint32_t* __restrict__ hist_part, int blocks, int cols) {
-    __shared__ uint32_t s_part[4][64];
-    const int wave = threadIdx.x / kWave, lane = threadIdx.x % kWave;
-    const int col = blockIdx.x * 64 + lane;
-    uint32_t acc = 0;
-    if (col < cols) {
-        int blk = wave;
-        for (; blk + 12 < blocks; blk += 16) {
-            uint32_t a0 = hist_part[(size_t)blk * cols + col];
-            uint32_t a1 = hist_part[(size_t)(blk + 4) * cols + col];
-            uint32_t a2 = hist_part[(size_t)(blk + 8) * cols + col];
-            uint32_t a3 = hist_part[(size_t)(blk + 12) * cols + col];
-            acc += (a0 + a1) + (a2 + a3);
-        }
-        for (; blk < blocks; blk += 4) acc += hist_part[(size_t)blk * cols + col];
-    }
-    s_part[wave][lane] = acc;
-    __syncthreads();
-    if (wave == 0 && col < cols) hist_part[col] = s_part[0][lane] + s_part[1][lane] + s_part[2][lane] + s_part[3][lane];
-}
-
-// grid = passes; exclusive scan of the per-pass totals (row 0 of hist_part).
-template <int BITS>
-__global__ __launch_bounds__(256) void k_hist_scan(const uint32_t* __restrict__ totals, uint32_t* __restrict__ bin_start) {
-    constexpr int BINS = 1 << BITS;
-    constexpr int BPT = BINS / 256;
-    __shared__ uint32_t s_w[4];
-    const int p = blockIdx.x;
-    uint32_t c[BPT];
-    uint32_t sum = 0;
-#pragma unroll
-    for (int j = 0; j < BPT; ++j) {
-        c[j] = totals[p * BINS + threadIdx.x * BPT + j];
-        sum += c[j];
-    }
-    uint32_t tot;
-    uint32_t run = block_excl_sum256(sum, s_w, &tot);
-#pragma unroll
-    for (int j = 0; j < BPT; ++j) {
-        bin_start[p * BINS + threadIdx.x * BPT + j] = run;
-        run += c[j];
-    }
-}
-
-void launch_hist_scan(uint32_t* hist_part, const SortPlan& plan, uint32_t* bin_start, hipStream_t s) {
-    const int cols = plan.passes * (1 << plan.bits);
-    hipLaunchKernelGGL(k_hist_reduce, dim3((cols + 63) / 64), dim3(256), 0, s, hist_part, plan.hist_blocks, cols);
-    if (plan.bits == 8)
-        hipLaunchKernelGGL(k_hist_scan<8>, dim3(plan.passes), dim3(256), 0, s, hist_part, bin_start);
-    else
-        hipLaunchKernelGGL(k_hist_scan<10>, dim3(plan.passes), dim3(256), 0, s, hist_part, bin_start);
-}
-
 template <int BITS, bool VALS>
 __global__ __launch_bounds__(kSortBlock) void k_onesweep(const uint64_t* __restrict__ kin, uint64_t* __restrict__ kout,
                                                         const uint32_t* __restrict__ vin, uint32_t* __restrict__ vout,
                                                         uint32_t n, int shift, uint32_t MASK,
-                                                        const uint32_t* __restrict__ bin_start, uint32_t* status,
+                                                        const uint32_t* __restrict__ hist_tot, uint32_t* status,
                                                         uint32_t* tile_ctr) {
     constexpr int BINS = 1 << BITS;
     constexpr int WAVES = kSortBlock / kWave;
@@ -203,6 +152,17 @@ __global__ __launch_bounds__(kSortBlock) void k_onesweep(const uint64_t* __restr
         key[it] = valid ? kin[idx] : ~0ull;
         if (VALS) val[it] = valid ? vin[idx] : 0u;
     }
+    // Global start of this thread's bins: exclusive scan of the pass's digit
+    // totals (its loads overlap the key loads above).
+    uint32_t tot[BPT];
+    uint32_t tot_sum = 0;
+#pragma unroll
+    for (int j = 0; j < BPT; ++j) {
+        tot[j] = hist_tot[tid * BPT + j];
+        tot_sum += tot[j];
+    }
+    uint32_t all_keys;
+    uint32_t gstart = block_excl_sum256(tot_sum, s_w, &all_keys);
 
     uint32_t* wh = s_whist + wave * BINS;
     const uint64_t lt_mask = lanemask_lt();
@@ -288,8 +248,9 @@ __global__ __launch_bounds__(kSortBlock) void k_onesweep(const uint64_t* __restr
             __hip_atomic_store(status + (size_t)tile * BINS + b, kFlagP | (g[j] + cnt[j]), __ATOMIC_RELAXED,
                                __HIP_MEMORY_SCOPE_AGENT);
         s_local[b] = lrun;
-        s_global[b] = bin_start[b] + g[j] - lrun;
+        s_global[b] = gstart + g[j] - lrun;
         lrun += cnt[j];
+        gstart += tot[j];
     }
     __syncthreads();
 
@@ -319,38 +280,38 @@ __global__ __launch_bounds__(kSortBlock) void k_onesweep(const uint64_t* __restr
 
 template <int BITS>
 static void onesweep_pass(bool vals, const uint64_t* kin, uint64_t* kout, const uint32_t* vin, uint32_t* vout,
-                          uint32_t n, int shift, uint32_t mask, uint32_t tiles, const uint32_t* bin_start, uint32_t* status,
+                          uint32_t n, int shift, uint32_t mask, uint32_t tiles, const uint32_t* hist_tot, uint32_t* status,
                           uint32_t* tile_ctr, hipStream_t s) {
     if (vals)
         hipLaunchKernelGGL((k_onesweep<BITS, true>), dim3(tiles), dim3(kSortBlock), 0, s, kin, kout, vin, vout, n,
-                           shift, mask, bin_start, status, tile_ctr);
+                           shift, mask, hist_tot, status, tile_ctr);
     else
         hipLaunchKernelGGL((k_onesweep<BITS, false>), dim3(tiles), dim3(kSortBlock), 0, s, kin, kout, vin, vout, n,
-                           shift, mask, bin_start, status, tile_ctr);
+                           shift, mask, hist_tot, status, tile_ctr);
 }
 
 void launch_onesweep_pass(const SortPlan& plan, int p, uint32_t n, const uint64_t* kin, uint64_t* kout,
-                          const uint32_t* vin, uint32_t* vout, const uint32_t* bin_start, uint32_t* lookback,
+                          const uint32_t* vin, uint32_t* vout, const uint32_t* hist_tot, uint32_t* lookback,
                           hipStream_t s) {
     const uint32_t bins = 1u << plan.bits;
     const bool vals = vin != nullptr;
     uint32_t* st = lookback + (size_t)p * plan.tiles * bins;
     uint32_t* ctr = lookback + (size_t)plan.passes * plan.tiles * bins + p;
     if (plan.bits == 8)
-        onesweep_pass<8>(vals, kin, kout, vin, vout, n, plan.shift[p], plan.mask[p], plan.tiles, bin_start + p * bins, st, ctr, s);
+        onesweep_pass<8>(vals, kin, kout, vin, vout, n, plan.shift[p], plan.mask[p], plan.tiles, hist_tot + p * bins, st, ctr, s);
     else
-        onesweep_pass<10>(vals, kin, kout, vin, vout, n, plan.shift[p], plan.mask[p], plan.tiles, bin_start + p * bins, st, ctr, s);
+        onesweep_pass<10>(vals, kin, kout, vin, vout, n, plan.shift[p], plan.mask[p], plan.tiles, hist_tot + p * bins, st, ctr, s);
 }
 
 void launch_onesweep(const SortPlan& plan, uint32_t n, uint64_t* keys_a, uint64_t* keys_b, uint32_t* vals_a,
-                     uint32_t* vals_b, const uint32_t* bin_start, uint32_t* lookback, uint64_t** keys_out,
+                     uint32_t* vals_b, const uint32_t* hist_tot, uint32_t* lookback, uint64_t** keys_out,
                      uint32_t** vals_out, hipStream_t s) {
     uint64_t* kin = keys_a;
     uint64_t* kout = keys_b;
     uint32_t* vin = vals_a;
     uint32_t* vout = vals_b;
     for (int p = 0; p < plan.passes; ++p) {
-        launch_onesweep_pass(plan, p, n, kin, kout, vin, vout, bin_start, lookback, s);
+        launch_onesweep_pass(plan, p, n, kin, kout, vin, vout, hist_tot, lookback, s);
         uint64_t* tk = kin; kin = kout; kout = tk;
         uint32_t* tv = vin; vin = vout; vout = tv;
     }

@@ -137,7 +137,7 @@ typedef struct bvh_batch_info {
     int32_t algo;
     int32_t radix_bits;                /* digit width in use */
     int32_t sort_passes;
-    int32_t key_bits;                  /* sorted key bits: index | morton30 | mesh */
+    int32_t key_bits;                  /* sorted key bits: morton30 + mesh id (index bits below are not sorted) */
     int32_t ploc_captured_iterations;  /* global PLOC iterations in the graph */
     int32_t ploc_iterations;           /* global PLOC iterations the last run needed */
     int32_t ploc_nn_grid;              /* persistent grid of the neighbour search */

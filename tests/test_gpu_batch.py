@@ -155,3 +155,14 @@ def test_build_batched_eager(ctx, algo):
         for d in dms:
             d.free()
     _assert_identical([got], ref)
+
+
+@pytest.mark.parametrize("algo", [B.ALGO_LBVH, B.ALGO_PLOC])
+def test_batch_many_tiny_meshes(ctx, algo):
+    # 3000 meshes of 1-40 triangles: 12 mesh-id bits in the keys, thousands of
+    # one-chunk segments, every mesh finished in the per-mesh LDS tail (PLOC).
+    rng = np.random.default_rng(8)
+    meshes = [B.Mesh.uniform(9000 + i, int(n)) for i, n in enumerate(rng.integers(1, 41, 3000))]
+    results, ref, info = _run_batch(ctx, meshes, algo, radius=4, runs=1)
+    _assert_identical(results, ref)
+    assert info["meshes"] == 3000 and info["key_bits"] == 30 + 12   # sorted bits: morton + mesh id

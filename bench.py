@@ -28,7 +28,7 @@ HBM_PEAK_GBS = 8000.0
 # sort pass).  Bounds: 9 fp32 vertex words in + one 32-B box out.  Morton: box
 # in + 8-B key out.  Sort pass: 8-B key in + out.  Topology+refit: key 8 + leaf
 # box gather 32 + leaf node 48 + internal node 48 + index 4 + sibling box 32.
-STAGE_BYTES = {"Bounds/Centroids": 68, "Morton Codes": 40, "Sort Pass": 16, "Topology+Refit": 172}
+STAGE_BYTES = {"Bounds/Centroids": 68, "Morton Codes": 40, "Sort Pass": 16, "Topology+Refit": 140}
 
 
 # HBM bytes per dispatch of the dominant kernels, from two rocprofv3 --pmc runs
@@ -340,7 +340,7 @@ def main():
                            "GBps": round(gbs, 1) if gbs else None,
                            "frac_hbm": round(gbs / HBM_PEAK_GBS, 4) if gbs else None}
         kname = {"Bounds/Centroids": "k_tri_bounds", "Morton Codes": "k_morton",
-                 "Topology+Refit": "k_hierarchy"}.get(name, "k_onesweep" if name.startswith("Sort Pass") else name)
+                 "Topology+Refit": "k_hier_level"}.get(name, "k_onesweep" if name.startswith("Sort Pass") else name)
         k = kernels.setdefault(kname, {"ms": 0.0, "launches": 0, "unit": ub})
         k["ms"] += ms
         k["launches"] += 1

@@ -33,12 +33,14 @@ struct DevBuf {
 // Device workspace of one build pipeline.
 struct Workspace {
     DevBuf tri_boxes, keys_a, keys_b, vals_a, vals_b, hist_tot, lookback, scene, handoff, epoch;
+    DevBuf hunits[2], hstatus, hctl;   // LBVH/LBVH+ hierarchy levels
     DevBuf parents, ranges, counters;  // LBVH+
     DevBuf ploc_box[2], ploc_id[2], ploc_sid[2], ploc_seg[2], ploc_nn, ploc_state, ploc_status, ploc_out;
     template <class F>
     void each(F f) {
         DevBuf* all[] = {&tri_boxes,   &keys_a,      &keys_b,      &vals_a,      &vals_b,     &hist_tot,
-                         &lookback,    &scene,       &handoff,     &epoch,       &parents,
+                         &lookback,    &scene,       &handoff,     &epoch,       &hunits[0],
+                         &hunits[1],   &hstatus,     &hctl,        &parents,
                          &ranges,      &counters,    &ploc_box[0], &ploc_box[1], &ploc_id[0], &ploc_id[1],
                          &ploc_sid[0], &ploc_sid[1], &ploc_seg[0], &ploc_seg[1], &ploc_nn,    &ploc_state,
                          &ploc_status, &ploc_out};
@@ -290,7 +292,7 @@ SortPlan plan_for(const Job& j, int radix_bits) {
 size_t zero_bytes_of(const SortPlan& plan) { return (sort_zero_bytes(plan) + 15) & ~(size_t)15; }
 
 // Allocations of the front end and the hierarchy (outside any capture).
-int reserve_front(bvh_ctx_t ctx, Workspace& w, const Job& j, const SortPlan& plan) {
+int reserve_front(bvh_ctx_t ctx, Workspace& w, const Job& j, const SortPlan& plan, bool hier) {
     const size_t n = j.total;
     const size_t bins = 1u << plan.bits;
     TRY(ensure(ctx, w.tri_boxes, n * 32));
@@ -305,6 +307,12 @@ int reserve_front(bvh_ctx_t ctx, Workspace& w, const Job& j, const SortPlan& pla
     TRY(ensure(ctx, w.scene, (size_t)j.k * 32));
     TRY(ensure(ctx, w.handoff, n * 8, true));
     TRY(ensure(ctx, w.epoch, 16, true));
+    TRY(ensure(ctx, w.hctl, (size_t)2 * hier_max_levels() * 4));
+    if (hier) {
+        TRY(ensure(ctx, w.hunits[0], n * (size_t)hier_unit_bytes()));
+        TRY(ensure(ctx, w.hunits[1], n * (size_t)hier_unit_bytes()));
+        TRY(ensure(ctx, w.hstatus, (size_t)hier_max_levels() * (j.hsb.nblocks + 1) * 8, true));
+    }
     return BVH_OK;
 }
 
@@ -313,7 +321,7 @@ int enqueue_front(bvh_ctx_t ctx, Workspace& w, const Job& j, const SortPlan& pla
                   uint64_t** keys_out, uint32_t** vals_out) {
     hipStream_t s = ctx->stream;
     launch_init_build(P<uint32_t>(w.scene), j.k, P<uint32_t>(w.epoch), s, P<uint32_t>(w.hist_tot),
-                      (uint32_t)plan.passes << plan.bits);
+                      (uint32_t)plan.passes << plan.bits, P<uint32_t>(w.hctl), 2u * hier_max_levels());
     launch_tri_bounds(j.fsb, P<float4>(w.tri_boxes), P<uint32_t>(w.scene), s);
     TRY(check_launch(ctx, "k_tri_bounds"));
     TRY(mark(clk, "Bounds/Centroids"));
@@ -353,6 +361,14 @@ HierArgs hier_args(Workspace& w, const Job& j, const uint64_t* keys, const uint3
     ha.handoff = P<uint64_t>(w.handoff);
     ha.epoch = P<uint32_t>(w.epoch);
     ha.err = P<uint32_t>(w.scene) + 7;
+    ha.units[0] = P<HUnit>(w.hunits[0]);
+    ha.units[1] = P<HUnit>(w.hunits[1]);
+    ha.units_in = nullptr;
+    ha.units_out = nullptr;
+    ha.status = P<uint64_t>(w.hstatus);
+    ha.status_stride = j.hsb.nblocks + 1;
+    ha.ctl = P<uint32_t>(w.hctl);
+    ha.levels = hier_levels(j.maxn, j.k);
     return ha;
 }
 
@@ -376,7 +392,7 @@ int build_lbvh(bvh_ctx_t ctx, bvh_mesh_t mesh, const bvh_build_params* prm, bvh_
     Job j = single_job(mesh, tree, false);
     SortPlan plan = plan_for(j, prm->radix_bits);
     Workspace& w = ctx->ws;
-    TRY(reserve_front(ctx, w, j, plan));
+    TRY(reserve_front(ctx, w, j, plan, true));
     StageClock clk{ctx};
     TRY(clk.begin());
     uint64_t* keys = nullptr;
@@ -401,7 +417,7 @@ int build_lbvh_plus(bvh_ctx_t ctx, bvh_mesh_t mesh, const bvh_build_params* prm,
     Job j = single_job(mesh, tree, true);
     SortPlan plan = plan_for(j, prm->radix_bits);
     Workspace& w = ctx->ws;
-    TRY(reserve_front(ctx, w, j, plan));
+    TRY(reserve_front(ctx, w, j, plan, true));
     TRY(ensure(ctx, w.parents, (size_t)(2 * (size_t)n) * 4));
     TRY(ensure(ctx, w.ranges, (size_t)n * 8));
     TRY(ensure(ctx, w.counters, (size_t)n * 4, true));
@@ -547,7 +563,7 @@ int build_ploc(bvh_ctx_t ctx, bvh_mesh_t mesh, const bvh_build_params* prm, bvh_
     Job j = single_job(mesh, tree, false);
     SortPlan plan = plan_for(j, prm->radix_bits);
     Workspace& w = ctx->ws;
-    TRY(reserve_front(ctx, w, j, plan));
+    TRY(reserve_front(ctx, w, j, plan, false));
     TRY(reserve_ploc(ctx, w, j));
     StageClock clk{ctx};
     TRY(clk.begin());
@@ -660,7 +676,7 @@ int build_batched_eager(bvh_ctx_t ctx, const bvh_mesh_t* meshes, const bvh_tree_
     SortPlan plan = plan_for(j, prm->radix_bits);
     if (plan.passes > 8) return set_err(ctx, BVH_E_UNSUPPORTED, "batch keys need more than 8 sort passes");
     Workspace& w = ctx->ws;
-    TRY(reserve_front(ctx, w, j, plan));
+    TRY(reserve_front(ctx, w, j, plan, prm->algo == BVH_ALGO_LBVH));
     if (prm->algo == BVH_ALGO_PLOC) TRY(reserve_ploc(ctx, w, j));
     StageClock clk{ctx};
     TRY(clk.begin());
@@ -736,7 +752,7 @@ int batch_capture(bvh_batch_t b) {
     SortPlan plan = make_sort_plan(b->job.total, b->job.begin_bit, b->job.end_bit, b->radix_bits,
                                    (int)b->job.fsb.nblocks);
     if (plan.passes > 8) return set_err(ctx, BVH_E_UNSUPPORTED, "batch keys need more than 8 sort passes");
-    TRY(reserve_front(ctx, b->ws, b->job, plan));
+    TRY(reserve_front(ctx, b->ws, b->job, plan, b->algo == BVH_ALGO_LBVH));
     if (b->algo == BVH_ALGO_PLOC) TRY(reserve_ploc(ctx, b->ws, b->job));
     HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
     HIP_TRY(ctx, hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));

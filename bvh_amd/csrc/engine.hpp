@@ -47,9 +47,10 @@ struct KeyFormat {
 constexpr uint32_t kFrontChunk = 4096;   // triangles per block of the front kernels
 
 // Resets the per-segment scene words (8 per segment), clears hist_words words
-// of the digit-total table (when given) and advances *epoch.
+// of the digit-total table and ctl_words hierarchy control words (when given)
+// and advances *epoch.
 void launch_init_build(uint32_t* seg_scene, uint32_t k, uint32_t* epoch, hipStream_t s, uint32_t* hist_tot = nullptr,
-                       uint32_t hist_words = 0);
+                       uint32_t hist_words = 0, uint32_t* ctl = nullptr, uint32_t ctl_words = 0);
 // Per-triangle boxes (2 x float4 per triangle) and each segment's scene box.
 void launch_tri_bounds(const SegBlocks& sb, float4* tri_boxes, uint32_t* seg_scene, hipStream_t s);
 
@@ -100,19 +101,37 @@ void launch_onesweep(const SortPlan& plan, uint32_t n, uint64_t* keys_a, uint64_
                      uint32_t** vals_out, hipStream_t s);
 
 // ---- hierarchy (k_hier.hip) ------------------------------------------------
+// A finished subtree handed from one hierarchy level to the next (48 B):
+// leaf range, mesh, packed delimiting deltas, box.
+struct __align__(16) HUnit {
+    uint32_t l, r, seg, beta;
+    float mnx, mny, mnz, mxx;
+    float mxy, mxz, pad0, pad1;
+};
 struct HierArgs {
-    SegBlocks sb;           // blocks over chunks of hier_chunk() leaves
+    SegBlocks sb;           // level-1 blocks over chunks of hier_chunk() leaves
     const uint64_t* keys;   // sorted keys (batch-wide)
     const uint32_t* vals;   // sorted indices (60-bit keys) or nullptr (packed keys)
     KeyFormat kf;
     const float4* tri_boxes;
     uint32_t* parents;      // optional, 2n-1 (single segment)
     int2* ranges;           // optional, n-1 (single segment)
-    uint64_t* handoff;      // batch-wide, epoch-tagged
+    uint64_t* handoff;      // batch-wide, epoch-tagged (final climbing pass)
     const uint32_t* epoch;  // device word advanced by launch_init_build
     uint32_t* err;          // set to nonzero when a sorted key names no primitive
+    HUnit* units[2];        // level ping-pong buffers, one entry per triangle each
+    const HUnit* units_in;  // set per launch
+    HUnit* units_out;
+    uint64_t* status;       // hier_max_levels() x status_stride look-back words, epoch-tagged
+    uint32_t status_stride; // >= level-1 chunks + 1
+    uint32_t* ctl;          // 2 x hier_max_levels() words cleared by launch_init_build:
+                            // per-level chunk tickets, then per-level unit counts
+    int levels;             // chunked levels (hier_levels); a climbing pass finishes the rest
 };
 int hier_chunk();
+int hier_unit_bytes();
+int hier_max_levels();
+int hier_levels(uint32_t maxn, uint32_t meshes);
 void launch_hierarchy(const HierArgs& a, hipStream_t s);
 
 // ---- traversal (k_trace.hip) --------------------------------------------------

@@ -41,24 +41,26 @@ __device__ __forceinline__ void block_segment(const SegBlocks& sb, uint32_t& seg
 
 // Per-segment scene block: words 0-2 order-mapped min, 3-5 max, 6 non-finite
 // flag, 7 internal-error flag.  Also clears the sort's digit-total table and
-// advances the build epoch.
+// the hierarchy's control words, and advances the build epoch.
 __global__ void k_init_build(uint32_t* seg_scene, uint32_t k, uint32_t* epoch, uint32_t* hist_tot,
-                             uint32_t hist_words) {
+                             uint32_t hist_words, uint32_t* ctl, uint32_t ctl_words) {
     const uint32_t stride = gridDim.x * blockDim.x;
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < k * 8; i += stride) {
         const uint32_t w = i & 7;
         seg_scene[i] = (w < 3) ? 0xFFFFFFFFu : 0u;
     }
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < hist_words; i += stride) hist_tot[i] = 0u;
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < ctl_words; i += stride) ctl[i] = 0u;
     if (blockIdx.x == 0 && threadIdx.x == 0 && epoch) *epoch += 1u;
 }
 
 void launch_init_build(uint32_t* seg_scene, uint32_t k, uint32_t* epoch, hipStream_t s, uint32_t* hist_tot,
-                       uint32_t hist_words) {
-    const uint32_t words = (k * 8 > hist_words) ? k * 8 : hist_words;
+                       uint32_t hist_words, uint32_t* ctl, uint32_t ctl_words) {
+    uint32_t words = (k * 8 > hist_words) ? k * 8 : hist_words;
+    if (ctl_words > words) words = ctl_words;
     uint32_t g = (words + 255) / 256;
     hipLaunchKernelGGL(k_init_build, dim3(g ? g : 1), dim3(256), 0, s, seg_scene, k, epoch, hist_tot,
-                       hist_tot ? hist_words : 0u);
+                       hist_tot ? hist_words : 0u, ctl, ctl ? ctl_words : 0u);
 }
 
 __device__ __forceinline__ uint32_t nonfinite(float f) {

@@ -1,4 +1,4 @@
-// k_hier.hip — fused topology, leaf initialisation and bottom-up refit.
+// k_hier.hip — LBVH topology, leaf initialisation and node boxes.
 //
 // Replaces kBuildInternalNodes_nt/_P + kInitLeafNodes_nt/_P + kRefitHierarchy_nt
 // (reference lbvh_builder.cu:247-338, lbvh_plus_builder.cu:292-335,341-366 with
@@ -7,47 +7,60 @@
 // The reference finds each internal node top-down (Karras 2012: direction,
 // exponential + binary range search, binary split search; O(log n) scattered
 // key reads per node), then refits bottom-up in a second kernel through
-// per-node atomic counters and __threadfence.  Here one pass climbs from the
-// leaves: a node covering sorted leaves [l, r] is the left child of its
-// parent iff delta(r) > delta(l-1) (delta = common-prefix length of adjacent
-// keys, -1 past either end); the parent's split is then r (left child) or
-// l-1 (right child).  The radix tree of unique keys is unique, so this
-// yields exactly the reference's topology, and its node numbering follows
-// from the same rule: an internal node is numbered by its right end when it
-// is a left child and by its left end otherwise (the root is 0), and leaf k
-// is node n-1+k, as in Karras' layout.  The two children of a split meet in a
-// handoff word: the first arrival deposits its far range end and stops, the
-// second merges the boxes (left box first, as the reference merges
-// nodes[left] with nodes[right]) and climbs on.
+// per-node atomic counters and __threadfence.  The tree is the Cartesian tree
+// of the boundary deltas (delta(j) = common-prefix length of keys j and j+1,
+// -1 past either end of a mesh): the node whose split is boundary s covers
+// the leaves between the nearest boundary on each side with a smaller delta
+// (the minimum inside any node's range is unique), and it is the left child
+// of its parent iff its right delimiter's delta is the larger one.  Karras'
+// numbering follows: an internal node is numbered by its right end when it
+// is a left child and by its left end otherwise (the root is 0); leaf k is
+// node n-1+k.  A node's box is the reference's left-then-right merge of its
+// children, i.e. the left fold of its leaves' boxes with select min/max;
+// that fold is associative (it keeps the leftmost of equal values, +0/-0
+// included), so any evaluation that keeps operands in leaf order gives the
+// reference's bits.
 //
-// MI355X mapping: a workgroup of 512 threads owns a chunk of 1024 sorted
-// leaves (two per thread).  Phase 1: every split whose parent lies entirely
-// inside the chunk (decided once per boundary from prefix/suffix minima of
-// the chunk's delta values, so both children agree, and packed beside the
-// delta) hands off through an LDS word with LDS-only release/acquire; the
-// word names the depositing chain, whose subtree box waits in LDS, so the
-// second arrival never re-reads a node record from memory and no climb step
-// waits on HBM.  Chains whose parent straddles the chunk boundary park in an
-// exit list.  Phase 2 runs the exits, one per thread: those nodes (the top of
-// the tree, roughly 2n/1024 of them) hand off through an epoch-tagged 64-bit
-// word in HBM; their records are written with agent-scope (sc1) stores and
-// read with agent-scope loads, with a full wait before the exchange, so no L2
-// writeback or invalidate is ever needed.  Epoch tags mean the handoff array
-// is never cleared.  Blocks map onto (mesh, chunk) pairs, so a batch of
-// meshes builds in one launch with chunk and root boundaries at every mesh
-// boundary.
+// MI355X mapping, level by level.  A level sees a sequence of units (level 1:
+// the sorted leaves; later levels: finished subtrees) and cuts it into chunks
+// of 1024 units, one 1024-thread workgroup each, with everything in LDS:
+//   * a sparse table of the chunk's boundary deltas: each inner boundary finds
+//     its nearest smaller delta on both sides by binary lifting (11 LDS reads
+//     per side, all threads at once, no climbing chains);
+//   * a segment tree of the units' boxes in heap order: a node's box is a
+//     left-to-right query over its unit range;
+//   * a boundary whose nearest smaller deltas both lie in the chunk yields a
+//     finished node (record written once, plain stores); the others (and the
+//     mesh walls, delta -1) cut the chunk into segments, each a finished
+//     subtree whose parent lies outside the chunk.  Those segments are the
+//     next level's units, compacted in order through decoupled look-back over
+//     epoch-tagged status words (no clearing, so a build is graph-capturable).
+// Each level shrinks the unit count ~50-100x on Morton-ordered data (4M blob
+// triangles: 41,822 units after level 1, 623 after level 2, none after level
+// 3).  Whatever remains after the last level (pathological delta patterns,
+// e.g. a caterpillar) is finished by a climbing pass: one thread per unit,
+// epoch-tagged 64-bit handoff words in HBM, records written with agent-scope
+// (sc1) stores and read with agent-scope loads after a full wait, so no L2
+// writeback or invalidate is needed.  Level-1 chunks never span meshes; later
+// chunks may, with the mesh walls (delta -1) keeping meshes apart, so a batch
+// of meshes builds with the same launches.
 #include "dev_common.hpp"
 #include "engine.hpp"
 
 namespace bvhk {
 
-constexpr int kHierBlock = 512;
-constexpr int kHierItems = 2;
-constexpr int kChunk = kHierBlock * kHierItems;
-// s_waitcnt encoding (gfx9 family) that waits for vmcnt(0) only.
-constexpr int kWaitVmcnt0 = 0x0F70;
+constexpr int kHierThreads = 1024;       // one unit per thread
+constexpr int kChunk = kHierThreads;     // units per chunk
+constexpr int kLog = 10;                 // log2(kChunk)
+constexpr int kHierMaxLevels = 6;
+constexpr uint32_t kFlagA = 1u << 30;    // look-back: aggregate published
+constexpr uint32_t kFlagP = 2u << 30;    // look-back: inclusive prefix published
+constexpr uint32_t kValMask = (1u << 30) - 1;
+constexpr float kBig = 3.402823466e+38f;
 
 int hier_chunk() { return kChunk; }
+int hier_unit_bytes() { return (int)sizeof(HUnit); }
+int hier_max_levels() { return kHierMaxLevels; }
 
 template <bool K60>
 __device__ __forceinline__ int key_delta(const uint64_t* __restrict__ keys, const uint32_t* __restrict__ vals,
@@ -60,44 +73,6 @@ __device__ __forceinline__ int key_delta(const uint64_t* __restrict__ keys, cons
     }
     uint64_t z = x ^ y;
     return z == 0 ? 64 : __clzll((long long)z);
-}
-
-__device__ __forceinline__ int wave_incl_min(int v) {
-    uint32_t l = __lane_id();
-#pragma unroll
-    for (int o = 1; o < 64; o <<= 1) {
-        int w = __shfl_up(v, o, 64);
-        if (l >= (uint32_t)o) v = (w < v) ? w : v;
-    }
-    return v;
-}
-
-// Inclusive min-scan of src[0..cnt) into dst (forward, or backward when REV).
-template <bool REV>
-__device__ void block_min_scan(const int8_t* src, int8_t* dst, int cnt, int* s_w) {
-    const int tid = threadIdx.x, wave = tid / kWave, lane = tid % kWave;
-    const int per = (cnt + kHierBlock - 1) / kHierBlock;
-    const int lo = tid * per;
-    const int hi = (lo + per < cnt) ? lo + per : cnt;
-    int m = 127;
-    for (int q = lo; q < hi; ++q) {
-        int v = src[REV ? cnt - 1 - q : q];
-        m = v < m ? v : m;
-    }
-    int incl = wave_incl_min(m);
-    int ex = __shfl_up(incl, 1, 64);
-    if (lane == 0) ex = 127;
-    if (lane == kWave - 1) s_w[wave] = incl;
-    __syncthreads();
-    for (int w = 0; w < wave; ++w) ex = s_w[w] < ex ? s_w[w] : ex;
-    int run = ex;
-    for (int q = lo; q < hi; ++q) {
-        int qq = REV ? cnt - 1 - q : q;
-        int v = src[qq];
-        run = v < run ? v : run;
-        dst[qq] = (int8_t)run;
-    }
-    __syncthreads();
 }
 
 // Node record written with agent-scope atomic stores (coherent across XCDs
@@ -131,231 +106,324 @@ __device__ __forceinline__ Box load_box_coherent(const DNode* nodes, uint32_t i)
     return b;
 }
 
-template <bool K60>
-__global__ __launch_bounds__(kHierBlock) void k_hierarchy(HierArgs p) {
-    __shared__ int8_t s_d[kChunk + 1];    // delta of boundaries a-1 .. b-1
-    // Prefix and suffix minima of s_d; once read they are replaced by the
-    // packed per-boundary words s_dq: delta + 1 (low byte), and bit 8 set when
-    // the split at that boundary has its parent inside the chunk.
-    __shared__ union {
-        struct {
-            int8_t pre[kChunk + 1];
-            int8_t suf[kChunk + 1];
-        } m;
-        uint16_t dq[kChunk + 1];
-    } s_u;
-    int8_t* const s_pre = s_u.m.pre;
-    int8_t* const s_suf = s_u.m.suf;
-    uint16_t* const s_dq = s_u.dq;
-    __shared__ uint32_t s_slot[kChunk];   // LDS handoff words: (far end - a + 1) | depositing chain << 16
-    __shared__ float s_box[6][kChunk];    // each chain's current box, read by its partner
-    __shared__ int s_w[kHierBlock / kWave];
-    __shared__ uint32_t s_nexit;
-    static_assert(kChunk <= 1024, "exit entries pack three 10-bit chunk offsets");
+__device__ __forceinline__ SegDesc seg_desc(const SegBlocks& sb, uint32_t seg) {
+    return sb.segs ? sb.segs[seg] : sb.single;
+}
 
-    // This block's mesh (segment) and chunk; everything below is in the
-    // mesh's local coordinates.
-    SegDesc d;
-    uint32_t chunk;
-    if (p.sb.blocks) {
-        const uint2 e = p.sb.blocks[blockIdx.x];
-        d = p.sb.segs[e.x];
-        chunk = e.y;
-    } else {
-        d = p.sb.single;
-        chunk = blockIdx.x;
+// Packed deltas of a unit: (delta before it + 1) | (delta after it + 1) << 8.
+__device__ __forceinline__ uint32_t pack_beta(int bl, int br) {
+    return (uint32_t)(bl + 1) | ((uint32_t)(br + 1) << 8);
+}
+__device__ __forceinline__ int beta_left(uint32_t w) { return (int)(w & 0xFFu) - 1; }
+__device__ __forceinline__ int beta_right(uint32_t w) { return (int)((w >> 8) & 0xFFu) - 1; }
+
+// Node index of a finished subtree covering leaves [l, r] with delimiting
+// deltas bl (boundary l-1) and br (boundary r); never the root.
+__device__ __forceinline__ uint32_t subtree_index(uint32_t n, uint32_t l, uint32_t r, int bl, int br) {
+    if (l == r) return n - 1 + l;
+    return (br > bl) ? r : l;
+}
+
+struct HierSmem {
+    float box[6][2 * kChunk];           // segment tree of unit boxes, heap order, units at kChunk + u
+    int8_t st[kLog + 1][kChunk + 1];    // st[k][q] = min of boundary deltas q .. q + 2^k - 1
+    uint32_t ul[kChunk], ur[kChunk];    // unit leaf ranges
+    uint32_t us[kChunk];                // unit mesh
+    uint32_t start[kChunk / 32];        // bit u: a segment starts at unit u
+    uint32_t wsum[kHierThreads / kWave];
+    uint32_t ticket, out_base;
+};
+
+__device__ __forceinline__ Box tree_box(const HierSmem& s, int h) {
+    Box b;
+    b.mnx = s.box[0][h]; b.mny = s.box[1][h]; b.mnz = s.box[2][h];
+    b.mxx = s.box[3][h]; b.mxy = s.box[4][h]; b.mxz = s.box[5][h];
+    return b;
+}
+
+// Box of units [x, y], merged left to right (the reference's fold order).
+__device__ Box range_box(const HierSmem& s, int x, int y) {
+    Box lo_acc, hi_acc;
+    lo_acc.mnx = lo_acc.mny = lo_acc.mnz = kBig;
+    lo_acc.mxx = lo_acc.mxy = lo_acc.mxz = -kBig;
+    hi_acc = lo_acc;
+    int lo = x + kChunk, hi = y + kChunk + 1;
+    while (lo < hi) {
+        if (lo & 1) lo_acc = merge(lo_acc, tree_box(s, lo++));
+        if (hi & 1) hi_acc = merge(tree_box(s, --hi), hi_acc);
+        lo >>= 1;
+        hi >>= 1;
     }
-    const uint32_t n = d.n;
-    const uint64_t* __restrict__ keys = p.keys + d.start;
-    const uint32_t* __restrict__ vals = K60 ? p.vals + d.start : nullptr;
-    const float4* __restrict__ tboxes = p.tri_boxes + 2 * (size_t)d.start;
-    uint64_t* __restrict__ handoff = p.handoff + d.start;
-    const uint32_t epoch = *p.epoch;
-    const uint64_t idx_mask = (p.kf.idx_bits >= 32) ? 0xFFFFFFFFull : ((1ull << p.kf.idx_bits) - 1);
-    const uint32_t a = chunk * kChunk;
-    const uint32_t b = (a + kChunk < n) ? a + kChunk : n;
-    const int len = (int)(b - a);
-    const int tid = threadIdx.x;
+    return merge(lo_acc, hi_acc);
+}
 
-    // Every item's primitive and leaf box first: the random box gathers of all
-    // items are in flight together (and overlap the delta scans below)
-    // instead of one per climb.
-    uint32_t pprim[kHierItems];
-    Box pbox[kHierItems];
-#pragma unroll
-    for (int k = 0; k < kHierItems; ++k) {
-        const uint32_t i = a + k * kHierBlock + tid;
-        uint32_t prim = 0;
-        if (i < b) {
-            prim = K60 ? vals[i] : (uint32_t)(keys[i] & idx_mask);
+// One chunk of one level; the block's ticket picks the chunk.  LEAF: level 1
+// (units are the sorted leaves of one mesh chunk).  Otherwise units come from
+// p.units_in.
+template <bool LEAF, bool K60>
+__device__ void hier_chunk_body(const HierArgs& p, HierSmem& s, uint32_t t, uint32_t nchunks, int level) {
+    const int tid = threadIdx.x;
+    const int u = tid;
+    const uint32_t epoch = *p.epoch;
+
+    // ---- units: leaf range, mesh, box, delimiting deltas --------------------
+    uint32_t m;            // units in this chunk
+    uint32_t seg0 = 0;     // level 1: the chunk's mesh
+    SegDesc d0;            // level 1: its descriptor
+    if (tid < kChunk / 32) s.start[tid] = 0;
+    if (LEAF) {
+        uint32_t chunk;
+        if (p.sb.blocks) {
+            const uint2 e = p.sb.blocks[t];
+            seg0 = e.x;
+            chunk = e.y;
+        } else {
+            chunk = t;
+        }
+        d0 = seg_desc(p.sb, seg0);
+        const uint32_t n = d0.n;
+        const uint32_t a = chunk * kChunk;
+        m = (n - a < (uint32_t)kChunk) ? n - a : (uint32_t)kChunk;
+        const uint64_t* __restrict__ keys = p.keys + d0.start;
+        const uint32_t* __restrict__ vals = K60 ? p.vals + d0.start : nullptr;
+        const uint64_t idx_mask = (p.kf.idx_bits >= 32) ? 0xFFFFFFFFull : ((1ull << p.kf.idx_bits) - 1);
+        Box b;
+        b.mnx = b.mny = b.mnz = kBig;
+        b.mxx = b.mxy = b.mxz = -kBig;
+        int dr = -1;
+        if ((uint32_t)u < m) {
+            const uint32_t i = a + u;
+            uint32_t prim = K60 ? vals[i] : (uint32_t)(keys[i] & idx_mask);
             if (prim >= n) {  // cannot happen for a correct sort; never index out of bounds
                 atomicOr(p.err, 1u);
                 prim = 0;
             }
-            d.indices[i] = prim;
-            pbox[k] = load_box(tboxes + 2 * (size_t)prim);
+            d0.indices[i] = prim;
+            b = load_box(p.tri_boxes + 2 * ((size_t)d0.start + prim));
+            store_node(d0.nodes, n - 1 + i, b, (int32_t)(prim | 0x80000000u), -1);
+            dr = (i + 1 < n) ? key_delta<K60>(keys, vals, i) : -1;
+            s.ul[u] = i;
+            s.ur[u] = i;
+            s.us[u] = seg0;
         }
-        pprim[k] = prim;
-    }
-    for (int q = tid; q <= len; q += kHierBlock) {
-        int64_t j = (int64_t)a - 1 + q;
-        s_d[q] = (j < 0 || j >= (int64_t)n - 1) ? (int8_t)-1 : (int8_t)key_delta<K60>(keys, vals, (uint32_t)j);
-    }
-    for (int q = tid; q < len; q += kHierBlock) s_slot[q] = 0;
-    if (tid == 0) s_nexit = 0;
-    __syncthreads();
-    block_min_scan<false>(s_d, s_pre, len + 1, s_w);
-    block_min_scan<true>(s_d, s_suf, len + 1, s_w);
-    // A split's parent lies inside the chunk iff a smaller delta exists on
-    // both sides within the chunk (prefix/suffix minima); both children of a
-    // split evaluate this identically.
-    constexpr int kPerThread = (kChunk + 1 + kHierBlock - 1) / kHierBlock;
-    uint16_t dqv[kPerThread];
-#pragma unroll
-    for (int t = 0; t < kPerThread; ++t) {
-        const int q = tid + t * kHierBlock;
-        if (q <= len) {
-            const int dv = s_d[q];
-            const bool in = q >= 1 && q <= len - 1 && s_pre[q - 1] < dv && s_suf[q + 1] < dv;
-            dqv[t] = (uint16_t)((dv + 1) | (in ? 0x100 : 0));
+        s.st[0][u + 1] = (int8_t)dr;
+        if (tid == 0) s.st[0][0] = (int8_t)((a > 0) ? key_delta<K60>(keys, vals, a - 1) : -1);
+        s.box[0][kChunk + u] = b.mnx; s.box[1][kChunk + u] = b.mny; s.box[2][kChunk + u] = b.mnz;
+        s.box[3][kChunk + u] = b.mxx; s.box[4][kChunk + u] = b.mxy; s.box[5][kChunk + u] = b.mxz;
+    } else {
+        const uint32_t cnt = p.ctl[kHierMaxLevels + level - 1];
+        const uint32_t base = t * kChunk;
+        m = (cnt - base < (uint32_t)kChunk) ? cnt - base : (uint32_t)kChunk;
+        float4 w0 = make_float4(kBig, kBig, kBig, -kBig), w1 = make_float4(-kBig, -kBig, 0.0f, 0.0f);
+        int dr = -1;
+        if ((uint32_t)u < m) {
+            const HUnit* src = p.units_in + base + u;
+            const uint4 h = *reinterpret_cast<const uint4*>(src);
+            w0 = reinterpret_cast<const float4*>(src)[1];
+            w1 = reinterpret_cast<const float4*>(src)[2];
+            s.ul[u] = h.x;
+            s.ur[u] = h.y;
+            s.us[u] = h.z;
+            dr = beta_right(h.w);
+            if (u == 0) s.st[0][0] = (int8_t)beta_left(h.w);
         }
-    }
-    __syncthreads();
-#pragma unroll
-    for (int t = 0; t < kPerThread; ++t) {
-        const int q = tid + t * kHierBlock;
-        if (q <= len) s_dq[q] = dqv[t];
+        s.st[0][u + 1] = (int8_t)dr;
+        s.box[0][kChunk + u] = w0.x; s.box[1][kChunk + u] = w0.y; s.box[2][kChunk + u] = w0.z;
+        s.box[3][kChunk + u] = w0.w; s.box[4][kChunk + u] = w1.x; s.box[5][kChunk + u] = w1.y;
     }
     __syncthreads();
 
-    // delta at boundary j (-1 beyond either end), from LDS inside the chunk.
-    auto dget = [&](int64_t j) -> int {
-        if (j < 0 || j >= (int64_t)n - 1) return -1;
-        if (j >= (int64_t)a - 1 && j <= (int64_t)b - 1) return s_d[j - ((int64_t)a - 1)];
-        return key_delta<K60>(keys, vals, (uint32_t)j);
+    // ---- sparse table of deltas and segment tree of boxes, 10 rounds ---------
+#pragma unroll 1
+    for (int k = 1; k <= kLog; ++k) {
+        const int w = 1 << (k - 1);
+        for (int q = tid; q + w <= kChunk; q += kHierThreads) {
+            const int8_t x = s.st[k - 1][q], y = s.st[k - 1][q + w];
+            s.st[k][q] = (y < x) ? y : x;
+        }
+        const int lvl_n = kChunk >> k;
+        if (tid < lvl_n) {
+            const int h = lvl_n + tid;
+            const Box c = merge(tree_box(s, 2 * h), tree_box(s, 2 * h + 1));
+            s.box[0][h] = c.mnx; s.box[1][h] = c.mny; s.box[2][h] = c.mnz;
+            s.box[3][h] = c.mxx; s.box[4][h] = c.mxy; s.box[5][h] = c.mxz;
+        }
+        __syncthreads();
+    }
+
+    // ---- inner boundary j = u (between units u and u+1) -----------------------
+    // Nearest boundary with a smaller delta on each side, by binary lifting:
+    // qL in [0, u] (or -1), qR in [u+2, m] (or m+1).  Boundary q lies before
+    // unit q; boundary 0 is the chunk's left edge, m its right edge.
+    const int p1 = u + 1;
+    const bool inner = (uint32_t)u + 1 < m;
+    const int v = inner ? s.st[0][p1] : -1;
+    int qL = -1, qR = (int)m + 1;
+    bool node = false;
+    if (inner && v >= 0) {
+        int pl = p1, pr = p1 + 1;
+#pragma unroll
+        for (int k = kLog; k >= 0; --k) {
+            const int sz = 1 << k;
+            if (pl >= sz && s.st[k][pl - sz] >= v) pl -= sz;
+            if (pr + sz - 1 <= (int)m && s.st[k][pr] >= v) pr += sz;
+        }
+        qL = pl - 1;
+        qR = pr;
+        node = (qL >= 0 && qR <= (int)m);
+    }
+    // Segment starts: unit 0, and the unit after every boundary that is not a
+    // node here (mesh walls included).
+    if (inner && !node) atomicOr(&s.start[(u + 1) >> 5], 1u << ((u + 1) & 31));
+    if (tid == 0) atomicOr(&s.start[0], 1u);
+    __syncthreads();
+
+    auto unit_index = [&](int x, uint32_t n) -> uint32_t {
+        return subtree_index(n, s.ul[x], s.ur[x], s.st[0][x], s.st[0][x + 1]);
     };
+    if (node) {
+        const uint32_t seg = s.us[u];
+        const SegDesc d = LEAF ? d0 : seg_desc(p.sb, seg);
+        const uint32_t n = d.n;
+        const uint32_t l = s.ul[qL], r = s.ur[qR - 1];
+        const int bL = s.st[0][qL], bR = s.st[0][qR];
+        const uint32_t idx = (bL < 0 && bR < 0) ? 0u : ((bR > bL) ? r : l);
+        const uint32_t lc = (qL == u) ? unit_index(u, n) : s.ur[u];
+        const uint32_t rc = (u + 1 == qR - 1) ? unit_index(u + 1, n) : s.ul[u + 1];
+        const Box b = range_box(s, qL, qR - 1);
+        store_node(d.nodes, idx, b, (int32_t)lc, (int32_t)rc);
+        if (p.parents) {
+            p.parents[lc] = idx;
+            p.parents[rc] = idx;
+        }
+        if (p.ranges) p.ranges[idx] = make_int2((int)l, (int)r);
+    }
 
-    DNode* __restrict__ nodes = d.nodes;
-    // delta at boundary j1 - 1, for a <= j1 <= b (-1 beyond the mesh).
-    auto dloc = [&](uint32_t j1) -> int { return s_d[j1 - a]; };
-    // Every prefetched box is in registers before the climbs start.  gfx950
-    // counts stores and loads in one vmcnt, so a load still pending inside
-    // the climb loop would make each step wait for its own node store.
-    __builtin_amdgcn_s_waitcnt(kWaitVmcnt0);
-
-    // Phase 1, inside the chunk: splits whose parent lies entirely in the
-    // chunk (decided from the prefix/suffix minima, which both children
-    // evaluate identically) meet in LDS.  These climbs touch no global memory
-    // but their node stores and never wait for them.  A chain whose parent
-    // straddles the chunk boundary stores its node record coherently, parks
-    // its box in LDS and leaves an exit entry (range and chain, 10 bits each);
-    // no wave waits on HBM here.
-    uint32_t exitw[kHierItems];
-    for (int k = 0; k < kHierItems; ++k) {
-        exitw[k] = ~0u;
-        const int chain = k * kHierBlock + tid;
-        const uint32_t i = a + chain;
-        if (i >= b) continue;
-        uint32_t l = i, r = i;
-        uint32_t my = n - 1 + i;
-        Box box = pbox[k];
-        int32_t lc = (int32_t)(pprim[k] | 0x80000000u), rc = -1;
-        // Packed deltas (delta + 1 | in-chunk bit) of boundaries l - 1 and r.
-        uint32_t dl = s_dq[l - a];
-        uint32_t dr = s_dq[r + 1 - a];
-        bool done = false;
+    // ---- segments: the next level's units ------------------------------------
+    bool exit_unit = false;
+    int y = 0;
+    if ((uint32_t)u < m && ((s.start[u >> 5] >> (u & 31)) & 1u)) {
+        // Last unit of the segment: before the next start, or the chunk's last.
+        y = (int)m - 1;
+        uint32_t wbits = s.start[u >> 5] & ~((2u << (u & 31)) - 1u);  // starts after u in this word
+        int wi = u >> 5;
         while (true) {
-            const bool isLeft = (dr & 0xFFu) > (dl & 0xFFu);
-            // The split's own in-chunk bit (never set at the chunk's end
-            // boundaries, so the root and straddling parents leave here).
-            if (!((isLeft ? dr : dl) & 0x100u)) break;
-            const uint32_t s = isLeft ? r : l - 1;
-            const uint32_t far = isLeft ? l : r;
-            store_node(nodes, my, box, lc, rc);
-            // The partner reads this chain's box from LDS, not from the node
-            // record: LDS-only release/acquire around the handoff word.
-            s_box[0][chain] = box.mnx; s_box[1][chain] = box.mny; s_box[2][chain] = box.mnz;
-            s_box[3][chain] = box.mxx; s_box[4][chain] = box.mxy; s_box[5][chain] = box.mxz;
-            const uint32_t word = (far - a + 1u) | ((uint32_t)chain << 16);
-            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
-            const uint32_t old = __hip_atomic_exchange(&s_slot[s - a], word, __ATOMIC_RELAXED,
-                                                       __HIP_MEMORY_SCOPE_WORKGROUP);
-            if (old == 0) {
-                done = true;
+            if (wbits) {
+                const int nxt = wi * 32 + __ffs((int)wbits) - 1;
+                if (nxt - 1 < y) y = nxt - 1;
                 break;
             }
-            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
-            const uint32_t other = a + (old & 0xFFFFu) - 1u;
-            const int pc = (int)(old >> 16);
-            // Only the far boundary moves; its delta and the partner's box are
-            // read together.
-            const uint32_t nv = s_dq[(isLeft ? other + 1u : other) - a];
-            Box sbox;
-            sbox.mnx = s_box[0][pc]; sbox.mny = s_box[1][pc]; sbox.mnz = s_box[2][pc];
-            sbox.mxx = s_box[3][pc]; sbox.mxy = s_box[4][pc]; sbox.mxz = s_box[5][pc];
-            const uint32_t sib = isLeft ? ((s + 1 == other) ? (n - 1 + other) : (s + 1))
-                                        : ((other == s) ? (n - 1 + s) : s);
-            lc = (int32_t)(isLeft ? my : sib);
-            rc = (int32_t)(isLeft ? sib : my);
-            if (isLeft) {
-                r = other;
-                dr = nv;
-            } else {
-                l = other;
-                dl = nv;
-            }
-            box = isLeft ? merge(box, sbox) : merge(sbox, box);
-            const uint32_t pidx = (l == 0 && r == n - 1) ? 0u : (((dr & 0xFFu) > (dl & 0xFFu)) ? r : l);
-            if (p.parents) {
-                p.parents[lc] = pidx;
-                p.parents[rc] = pidx;
-            }
-            if (p.ranges) p.ranges[pidx] = make_int2((int)l, (int)r);
-            my = pidx;
+            if (++wi >= kChunk / 32) break;
+            wbits = s.start[wi];
         }
-        if (done) continue;
-        if (l == 0 && r == n - 1) {  // the chunk is the whole mesh: this is the root
-            store_node(nodes, my, box, lc, rc);
-            continue;
-        }
-        store_node_coherent(nodes, my, box, lc, rc);
-        s_box[0][chain] = box.mnx; s_box[1][chain] = box.mny; s_box[2][chain] = box.mnz;
-        s_box[3][chain] = box.mxx; s_box[4][chain] = box.mxy; s_box[5][chain] = box.mxz;
-        exitw[k] = (l - a) | ((r - a) << 10) | ((uint32_t)chain << 20);
+        const int bl = s.st[0][u], br = s.st[0][y + 1];
+        exit_unit = !(bl < 0 && br < 0);  // both walls: a finished mesh
     }
-    // The exit records are complete in memory before any thread publishes a
-    // handoff word for them; then the exits are compacted into s_slot.
-    __builtin_amdgcn_s_waitcnt(kWaitVmcnt0);
+    // Rank among the chunk's exits (block exclusive scan of the flags).
+    const uint64_t bal = __ballot(exit_unit);
+    const int lane = tid % kWave, wave = tid / kWave;
+    if (lane == 0) s.wsum[wave] = (uint32_t)__popcll(bal);
     __syncthreads();
+    uint32_t before = 0, total = 0;
 #pragma unroll
-    for (int k = 0; k < kHierItems; ++k)
-        if (exitw[k] != ~0u) s_slot[atomicAdd(&s_nexit, 1u)] = exitw[k];
-    __syncthreads();
-    const uint32_t nexit = s_nexit;
+    for (int w = 0; w < kHierThreads / kWave; ++w) {
+        const uint32_t c = s.wsum[w];
+        before += (w < wave) ? c : 0u;
+        total += c;
+    }
+    const uint32_t rank = before + (uint32_t)__popcll(bal & lanemask_lt());
 
-    // Phase 2, one exit per thread: splits whose parent straddles a chunk
-    // boundary.  The partner may run on another XCD, so every record is
-    // stored with agent-coherent stores, complete before the epoch-tagged
-    // handoff word in HBM is swapped.
-    for (uint32_t e = tid; e < nexit; e += kHierBlock) {
-        const uint32_t w = s_slot[e];
-        uint32_t l = a + (w & 1023u), r = a + ((w >> 10) & 1023u);
-        const uint32_t chain = w >> 20;
+    // Decoupled look-back over the level's chunks (ticket order = unit order):
+    // epoch-tagged words, so stale words of earlier builds read as not ready.
+    if (tid == 0) {
+        uint64_t* st = p.status + (size_t)level * p.status_stride;
+        const uint64_t tag = (uint64_t)epoch << 32;
+        uint32_t prefix = 0;
+        if (t == 0) {
+            __hip_atomic_store(st + t, tag | kFlagP | total, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        } else {
+            __hip_atomic_store(st + t, tag | kFlagA | total, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            int j = (int)t - 1;
+            while (true) {
+                const uint64_t w = __hip_atomic_load(st + j, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                const uint32_t lo = (uint32_t)w;
+                if ((uint32_t)(w >> 32) != epoch || (lo & ~kValMask) == 0) {
+                    __builtin_amdgcn_s_sleep(1);
+                    continue;
+                }
+                prefix += lo & kValMask;
+                if ((lo & ~kValMask) == kFlagP) break;
+                --j;
+            }
+            __hip_atomic_store(st + t, tag | kFlagP | (prefix + total), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        }
+        s.out_base = prefix;
+        if (t == nchunks - 1) p.ctl[kHierMaxLevels + level] = prefix + total;
+    }
+    __syncthreads();
+    if (exit_unit) {
+        const Box b = range_box(s, u, y);
+        HUnit* dst = p.units_out + s.out_base + rank;
+        reinterpret_cast<uint4*>(dst)[0] =
+            make_uint4(s.ul[u], s.ur[y], s.us[u], pack_beta(s.st[0][u], s.st[0][y + 1]));
+        reinterpret_cast<float4*>(dst)[1] = make_float4(b.mnx, b.mny, b.mnz, b.mxx);
+        reinterpret_cast<float4*>(dst)[2] = make_float4(b.mxy, b.mxz, 0.0f, 0.0f);
+    }
+}
+
+template <bool LEAF, bool K60>
+__global__ __launch_bounds__(kHierThreads) void k_hier_level(HierArgs p, int level) {
+    __shared__ HierSmem s;
+    const uint32_t nchunks =
+        LEAF ? p.sb.nblocks : (p.ctl[kHierMaxLevels + level - 1] + kChunk - 1) / (uint32_t)kChunk;
+    while (true) {
+        if (threadIdx.x == 0) s.ticket = atomicAdd(&p.ctl[level], 1u);
+        __syncthreads();
+        const uint32_t t = s.ticket;
+        if (t >= nchunks) break;
+        hier_chunk_body<LEAF, K60>(p, s, t, nchunks, level);
+        if (LEAF) break;  // level 1 launches exactly one block per chunk
+        __syncthreads();
+    }
+}
+
+// Finishes whatever the chunked levels left: each unit climbs with
+// epoch-tagged handoff words in HBM; the first arrival at a parent stops, the
+// second merges (left box first) and climbs on.
+template <bool K60>
+__global__ __launch_bounds__(256) void k_hier_climb(HierArgs p, int level) {
+    const uint32_t cnt = p.ctl[kHierMaxLevels + level];
+    const uint32_t epoch = *p.epoch;
+    for (uint32_t e = blockIdx.x * blockDim.x + threadIdx.x; e < cnt; e += gridDim.x * blockDim.x) {
+        const HUnit* src = p.units_in + e;
+        const uint4 h = *reinterpret_cast<const uint4*>(src);
+        const float4 w0 = reinterpret_cast<const float4*>(src)[1];
+        const float4 w1 = reinterpret_cast<const float4*>(src)[2];
+        const SegDesc d = seg_desc(p.sb, h.z);
+        const uint32_t n = d.n;
+        const uint64_t* __restrict__ keys = p.keys + d.start;
+        const uint32_t* __restrict__ vals = K60 ? p.vals + d.start : nullptr;
+        uint64_t* __restrict__ handoff = p.handoff + d.start;
+        DNode* __restrict__ nodes = d.nodes;
+        auto dget = [&](int64_t j) -> int {
+            if (j < 0 || j >= (int64_t)n - 1) return -1;
+            return key_delta<K60>(keys, vals, (uint32_t)j);
+        };
+        uint32_t l = h.x, r = h.y;
+        int dl = beta_left(h.w), dr = beta_right(h.w);
         Box box;
-        box.mnx = s_box[0][chain]; box.mny = s_box[1][chain]; box.mnz = s_box[2][chain];
-        box.mxx = s_box[3][chain]; box.mxy = s_box[4][chain]; box.mxz = s_box[5][chain];
-        int dl = dloc(l);
-        int dr = dloc(r + 1);
-        uint32_t my = (l == r) ? n - 1 + l : ((dr > dl) ? r : l);
+        box.mnx = w0.x; box.mny = w0.y; box.mnz = w0.z;
+        box.mxx = w0.w; box.mxy = w1.x; box.mxz = w1.y;
+        uint32_t my = subtree_index(n, l, r, dl, dr);
         int32_t lc = 0, rc = 0;
-        bool stored = true;  // the exit record was stored in phase 1
+        bool stored = true;  // the unit's record was written by an earlier launch
         while (true) {
             if (l == 0 && r == n - 1) {
                 store_node(nodes, my, box, lc, rc);
                 break;
             }
             const bool isLeft = dr > dl;
-            const uint32_t s = isLeft ? r : l - 1;
+            const uint32_t sp = isLeft ? r : l - 1;
             const uint32_t far = isLeft ? l : r;
             if (!stored) {
                 store_node_coherent(nodes, my, box, lc, rc);
@@ -363,11 +431,11 @@ __global__ __launch_bounds__(kHierBlock) void k_hierarchy(HierArgs p) {
             }
             stored = false;
             const uint64_t hw = ((uint64_t)epoch << 32) | far;
-            const uint64_t old = __hip_atomic_exchange(&handoff[s], hw, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            const uint64_t old = __hip_atomic_exchange(&handoff[sp], hw, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
             if ((uint32_t)(old >> 32) != epoch) break;
             const uint32_t other = (uint32_t)old;
-            const uint32_t sib = isLeft ? ((s + 1 == other) ? (n - 1 + other) : (s + 1))
-                                        : ((other == s) ? (n - 1 + s) : s);
+            const uint32_t sib = isLeft ? ((sp + 1 == other) ? (n - 1 + other) : (sp + 1))
+                                        : ((other == sp) ? (n - 1 + sp) : sp);
             const Box sbox = load_box_coherent(nodes, sib);
             const Box pb = isLeft ? merge(box, sbox) : merge(sbox, box);
             lc = (int32_t)(isLeft ? my : sib);
@@ -388,9 +456,39 @@ __global__ __launch_bounds__(kHierBlock) void k_hierarchy(HierArgs p) {
     }
 }
 
-void launch_hierarchy(const HierArgs& a, hipStream_t s) {
-    if (a.vals) hipLaunchKernelGGL(k_hierarchy<true>, dim3(a.sb.nblocks), dim3(kHierBlock), 0, s, a);
-    else hipLaunchKernelGGL(k_hierarchy<false>, dim3(a.sb.nblocks), dim3(kHierBlock), 0, s, a);
+int hier_levels(uint32_t maxn, uint32_t meshes) {
+    // Each level shrinks the units >= ~64x on Morton-ordered data; meshes that
+    // straddle a chunk boundary of a later level may need one more.
+    int levels = 1;
+    uint64_t chunks = (maxn + kChunk - 1) / kChunk;
+    while (chunks > 1 && levels < kHierMaxLevels - 1) {
+        ++levels;
+        chunks = (chunks + 63) / 64;
+    }
+    if (meshes > 1 && maxn > (uint32_t)kChunk && levels < kHierMaxLevels) ++levels;
+    return levels;
+}
+
+void launch_hierarchy(const HierArgs& a0, hipStream_t s) {
+    HierArgs a = a0;
+    const bool k60 = a.vals != nullptr;
+    a.units_out = a.units[0];
+    a.units_in = nullptr;
+    if (k60) hipLaunchKernelGGL((k_hier_level<true, true>), dim3(a.sb.nblocks), dim3(kHierThreads), 0, s, a, 0);
+    else hipLaunchKernelGGL((k_hier_level<true, false>), dim3(a.sb.nblocks), dim3(kHierThreads), 0, s, a, 0);
+    uint32_t grid = a.sb.nblocks;
+    for (int lv = 1; lv < a.levels; ++lv) {
+        a.units_in = a.units[(lv - 1) & 1];
+        a.units_out = a.units[lv & 1];
+        grid = (grid + 31) / 32;
+        if (grid < 1) grid = 1;
+        if (k60) hipLaunchKernelGGL((k_hier_level<false, true>), dim3(grid), dim3(kHierThreads), 0, s, a, lv);
+        else hipLaunchKernelGGL((k_hier_level<false, false>), dim3(grid), dim3(kHierThreads), 0, s, a, lv);
+    }
+    a.units_in = a.units[(a.levels - 1) & 1];
+    const uint32_t cgrid = (a.sb.nblocks + 63) / 64 < 1 ? 1 : ((a.sb.nblocks + 63) / 64 > 512 ? 512 : (a.sb.nblocks + 63) / 64);
+    if (k60) hipLaunchKernelGGL(k_hier_climb<true>, dim3(cgrid), dim3(256), 0, s, a, a.levels - 1);
+    else hipLaunchKernelGGL(k_hier_climb<false>, dim3(cgrid), dim3(256), 0, s, a, a.levels - 1);
 }
 
 }  // namespace bvhk

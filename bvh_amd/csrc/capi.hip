@@ -33,14 +33,14 @@ struct DevBuf {
 // Device workspace of one build pipeline.
 struct Workspace {
     DevBuf tri_boxes, keys_a, keys_b, vals_a, vals_b, hist_tot, lookback, scene, handoff, epoch;
-    DevBuf hunits[2], hstatus, hctl;   // LBVH/LBVH+ hierarchy levels
+    DevBuf hunits[2], hstatus, hctl, hcounts;  // LBVH/LBVH+ hierarchy levels
     DevBuf parents, ranges, counters;  // LBVH+
     DevBuf ploc_box[2], ploc_id[2], ploc_sid[2], ploc_seg[2], ploc_nn, ploc_state, ploc_status, ploc_out;
     template <class F>
     void each(F f) {
         DevBuf* all[] = {&tri_boxes,   &keys_a,      &keys_b,      &vals_a,      &vals_b,     &hist_tot,
                          &lookback,    &scene,       &handoff,     &epoch,       &hunits[0],
-                         &hunits[1],   &hstatus,     &hctl,        &parents,
+                         &hunits[1],   &hstatus,     &hctl,        &hcounts,     &parents,
                          &ranges,      &counters,    &ploc_box[0], &ploc_box[1], &ploc_id[0], &ploc_id[1],
                          &ploc_sid[0], &ploc_sid[1], &ploc_seg[0], &ploc_seg[1], &ploc_nn,    &ploc_state,
                          &ploc_status, &ploc_out};
@@ -309,8 +309,9 @@ int reserve_front(bvh_ctx_t ctx, Workspace& w, const Job& j, const SortPlan& pla
     TRY(ensure(ctx, w.epoch, 16, true));
     TRY(ensure(ctx, w.hctl, (size_t)2 * hier_max_levels() * 4));
     if (hier) {
-        TRY(ensure(ctx, w.hunits[0], n * (size_t)hier_unit_bytes()));
+        TRY(ensure(ctx, w.hunits[0], (size_t)j.hsb.nblocks * hier_chunk() * hier_unit_bytes()));
         TRY(ensure(ctx, w.hunits[1], n * (size_t)hier_unit_bytes()));
+        TRY(ensure(ctx, w.hcounts, (size_t)j.hsb.nblocks * 4));
         TRY(ensure(ctx, w.hstatus, (size_t)hier_max_levels() * (j.hsb.nblocks + 1) * 8, true));
     }
     return BVH_OK;
@@ -365,6 +366,7 @@ HierArgs hier_args(Workspace& w, const Job& j, const uint64_t* keys, const uint3
     ha.units[1] = P<HUnit>(w.hunits[1]);
     ha.units_in = nullptr;
     ha.units_out = nullptr;
+    ha.counts = P<uint32_t>(w.hcounts);
     ha.status = P<uint64_t>(w.hstatus);
     ha.status_stride = j.hsb.nblocks + 1;
     ha.ctl = P<uint32_t>(w.hctl);

@@ -119,7 +119,9 @@ struct HierArgs {
     uint64_t* handoff;      // batch-wide, epoch-tagged (final climbing pass)
     const uint32_t* epoch;  // device word advanced by launch_init_build
     uint32_t* err;          // set to nonzero when a sorted key names no primitive
-    HUnit* units[2];        // level ping-pong buffers, one entry per triangle each
+    HUnit* units[2];        // [0]: level-1 chunk slots (chunks x hier_chunk()), later levels;
+                            // [1]: one entry per triangle
+    uint32_t* counts;       // level-1 exits per chunk
     const HUnit* units_in;  // set per launch
     HUnit* units_out;
     uint64_t* status;       // hier_max_levels() x status_stride look-back words, epoch-tagged

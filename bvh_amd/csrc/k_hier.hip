@@ -333,34 +333,59 @@ __device__ void hier_chunk_body(const HierArgs& p, HierSmem& s, uint32_t t, uint
     }
     const uint32_t rank = before + (uint32_t)__popcll(bal & lanemask_lt());
 
-    // Decoupled look-back over the level's chunks (ticket order = unit order):
-    // epoch-tagged words, so stale words of earlier builds read as not ready.
-    if (tid == 0) {
-        uint64_t* st = p.status + (size_t)level * p.status_stride;
-        const uint64_t tag = (uint64_t)epoch << 32;
-        uint32_t prefix = 0;
-        if (t == 0) {
-            __hip_atomic_store(st + t, tag | kFlagP | total, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        } else {
-            __hip_atomic_store(st + t, tag | kFlagA | total, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            int j = (int)t - 1;
-            while (true) {
-                const uint64_t w = __hip_atomic_load(st + j, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                const uint32_t lo = (uint32_t)w;
-                if ((uint32_t)(w >> 32) != epoch || (lo & ~kValMask) == 0) {
+    if (LEAF) {
+        // Level 1: exits go to the chunk's own slots; k_hier_gather compacts
+        // them, so no level-1 block waits for another.
+        if (tid == 0) {
+            p.counts[t] = total;
+            s.out_base = t * kChunk;
+        }
+        __syncthreads();
+    } else {
+        // Decoupled look-back over the level's chunks (ticket order = unit order),
+        // one wave reading 64 predecessors' words per step: epoch-tagged words, so
+        // stale words of earlier builds read as not ready.
+        if (wave == 0) {
+            uint64_t* st = p.status + (size_t)level * p.status_stride;
+            const uint64_t tag = (uint64_t)epoch << 32;
+            if (lane == 0)
+                __hip_atomic_store(st + t, tag | (t == 0 ? kFlagP : kFlagA) | total, __ATOMIC_RELAXED,
+                                   __HIP_MEMORY_SCOPE_AGENT);
+            uint32_t prefix = 0;
+            int top = (int)t - 1;  // nearest predecessor not yet accumulated
+            while (top >= 0) {
+                const int j = top - lane;
+                uint32_t lo = kFlagP;  // before chunk 0: an inclusive prefix of 0
+                if (j >= 0) {
+                    const uint64_t w = __hip_atomic_load(st + j, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                    lo = ((uint32_t)(w >> 32) == epoch) ? (uint32_t)w : 0u;
+                }
+                const uint64_t pmask = __ballot((lo & ~kValMask) == kFlagP);
+                const uint64_t nready = __ballot((lo & ~kValMask) == 0);
+                // Lanes 0 .. first P (or all 64) must be ready.
+                const int first_p = pmask ? __ffsll((long long)pmask) - 1 : kWave - 1;
+                const uint64_t need = (first_p == kWave - 1) ? ~0ull : ((2ull << first_p) - 1);
+                if (nready & need) {
                     __builtin_amdgcn_s_sleep(1);
                     continue;
                 }
-                prefix += lo & kValMask;
-                if ((lo & ~kValMask) == kFlagP) break;
-                --j;
+                uint32_t v = (lane <= first_p) ? (lo & kValMask) : 0u;
+    #pragma unroll
+                for (int o = 32; o >= 1; o >>= 1) v += __shfl_xor(v, o, kWave);
+                prefix += v;
+                if (pmask) break;
+                top -= kWave;
             }
-            __hip_atomic_store(st + t, tag | kFlagP | (prefix + total), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            if (lane == 0) {
+                if (t > 0)
+                    __hip_atomic_store(st + t, tag | kFlagP | (prefix + total), __ATOMIC_RELAXED,
+                                       __HIP_MEMORY_SCOPE_AGENT);
+                s.out_base = prefix;
+                if (t == nchunks - 1) p.ctl[kHierMaxLevels + level] = prefix + total;
+            }
         }
-        s.out_base = prefix;
-        if (t == nchunks - 1) p.ctl[kHierMaxLevels + level] = prefix + total;
+        __syncthreads();
     }
-    __syncthreads();
     if (exit_unit) {
         const Box b = range_box(s, u, y);
         HUnit* dst = p.units_out + s.out_base + rank;
@@ -371,18 +396,71 @@ __device__ void hier_chunk_body(const HierArgs& p, HierSmem& s, uint32_t t, uint
     }
 }
 
+// Compacts level 1's exits (p.counts[c] units in the slots of chunk c) into
+// one ordered sequence.  Block b owns chunks [b*kGather, (b+1)*kGather): it
+// sums the counts of all earlier chunks, scans its own, and copies.
+constexpr int kGather = 64;
+__global__ __launch_bounds__(kHierThreads) void k_hier_gather(HierArgs p) {
+    __shared__ uint32_t s_off[kGather + 1];
+    __shared__ uint32_t s_w[kHierThreads / kWave];
+    const int tid = threadIdx.x, lane = tid % kWave, wave = tid / kWave;
+    const uint32_t nch = p.sb.nblocks;
+    const uint32_t c0 = blockIdx.x * kGather;
+    const uint32_t c1 = (c0 + kGather < nch) ? c0 + kGather : nch;
+    // Units before chunk c0.
+    uint32_t acc = 0;
+    for (uint32_t c = tid; c < c0; c += kHierThreads) acc += p.counts[c];
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1) acc += __shfl_xor(acc, o, kWave);
+    if (lane == 0) s_w[wave] = acc;
+    __syncthreads();
+    if (tid == 0) {
+        uint32_t base = 0;
+        for (int w = 0; w < kHierThreads / kWave; ++w) base += s_w[w];
+        uint32_t run = base;
+        for (uint32_t c = c0; c < c1; ++c) {
+            s_off[c - c0] = run;
+            run += p.counts[c];
+        }
+        s_off[c1 - c0] = run;
+        if (c1 == nch) p.ctl[kHierMaxLevels + 0] = run;
+    }
+    __syncthreads();
+    const uint32_t first = s_off[0], last = s_off[c1 - c0];
+    for (uint32_t g = first + tid; g < last; g += kHierThreads) {
+        // Chunk of output unit g: the last c with s_off[c - c0] <= g.
+        int lo = 0, hi = (int)(c1 - c0) - 1;
+        while (lo < hi) {
+            const int mid = (lo + hi + 1) >> 1;
+            if (s_off[mid] <= g) lo = mid;
+            else hi = mid - 1;
+        }
+        const uint4* src = reinterpret_cast<const uint4*>(p.units[0] + (size_t)(c0 + lo) * kChunk + (g - s_off[lo]));
+        uint4* dst = reinterpret_cast<uint4*>(p.units[1] + g);
+        const uint4 a = src[0], b = src[1], c = src[2];
+        dst[0] = a;
+        dst[1] = b;
+        dst[2] = c;
+    }
+}
+
 template <bool LEAF, bool K60>
 __global__ __launch_bounds__(kHierThreads) void k_hier_level(HierArgs p, int level) {
     __shared__ HierSmem s;
     const uint32_t nchunks =
         LEAF ? p.sb.nblocks : (p.ctl[kHierMaxLevels + level - 1] + kChunk - 1) / (uint32_t)kChunk;
+    if (LEAF) {  // one block per chunk, no ordering needed
+        hier_chunk_body<LEAF, K60>(p, s, blockIdx.x, nchunks, level);
+        return;
+    }
+    // Later levels take chunks in ticket order (the look-back needs every
+    // smaller ticket to be running or done).
     while (true) {
         if (threadIdx.x == 0) s.ticket = atomicAdd(&p.ctl[level], 1u);
         __syncthreads();
         const uint32_t t = s.ticket;
         if (t >= nchunks) break;
         hier_chunk_body<LEAF, K60>(p, s, t, nchunks, level);
-        if (LEAF) break;  // level 1 launches exactly one block per chunk
         __syncthreads();
     }
 }
@@ -472,21 +550,25 @@ int hier_levels(uint32_t maxn, uint32_t meshes) {
 void launch_hierarchy(const HierArgs& a0, hipStream_t s) {
     HierArgs a = a0;
     const bool k60 = a.vals != nullptr;
-    a.units_out = a.units[0];
+    // Level 1 writes per-chunk slots of units[0]; k_hier_gather compacts them
+    // into units[1]; level lv >= 1 reads units[lv & 1] and writes the other.
     a.units_in = nullptr;
+    a.units_out = a.units[0];
     if (k60) hipLaunchKernelGGL((k_hier_level<true, true>), dim3(a.sb.nblocks), dim3(kHierThreads), 0, s, a, 0);
     else hipLaunchKernelGGL((k_hier_level<true, false>), dim3(a.sb.nblocks), dim3(kHierThreads), 0, s, a, 0);
+    hipLaunchKernelGGL(k_hier_gather, dim3((a.sb.nblocks + kGather - 1) / kGather), dim3(kHierThreads), 0, s, a);
     uint32_t grid = a.sb.nblocks;
     for (int lv = 1; lv < a.levels; ++lv) {
-        a.units_in = a.units[(lv - 1) & 1];
-        a.units_out = a.units[lv & 1];
+        a.units_in = a.units[lv & 1];
+        a.units_out = a.units[(lv + 1) & 1];
         grid = (grid + 31) / 32;
         if (grid < 1) grid = 1;
         if (k60) hipLaunchKernelGGL((k_hier_level<false, true>), dim3(grid), dim3(kHierThreads), 0, s, a, lv);
         else hipLaunchKernelGGL((k_hier_level<false, false>), dim3(grid), dim3(kHierThreads), 0, s, a, lv);
     }
-    a.units_in = a.units[(a.levels - 1) & 1];
-    const uint32_t cgrid = (a.sb.nblocks + 63) / 64 < 1 ? 1 : ((a.sb.nblocks + 63) / 64 > 512 ? 512 : (a.sb.nblocks + 63) / 64);
+    a.units_in = a.units[a.levels & 1];
+    uint32_t cgrid = (a.sb.nblocks + 63) / 64;
+    cgrid = cgrid < 1 ? 1 : (cgrid > 512 ? 512 : cgrid);
     if (k60) hipLaunchKernelGGL(k_hier_climb<true>, dim3(cgrid), dim3(256), 0, s, a, a.levels - 1);
     else hipLaunchKernelGGL(k_hier_climb<false>, dim3(cgrid), dim3(256), 0, s, a, a.levels - 1);
 }

@@ -45,6 +45,7 @@ struct KeyFormat {
 
 // ---- LBVH/LBVH+/PLOC front end (k_front.hip) ----------------------------
 constexpr uint32_t kFrontChunk = 4096;   // triangles per block of the front kernels
+constexpr int kFrontUnroll = 4;          // triangles per thread whose loads are in flight together
 
 // Resets the per-segment scene words (8 per segment), clears hist_words words
 // of the digit-total table and ctl_words hierarchy control words (when given)
@@ -55,8 +56,8 @@ void launch_init_build(uint32_t* seg_scene, uint32_t k, uint32_t* epoch, hipStre
 void launch_tri_bounds(const SegBlocks& sb, float4* tri_boxes, uint32_t* seg_scene, hipStream_t s);
 
 // Onesweep tile: threads x keys per thread.
-constexpr int kSortThreads = 256;
-constexpr int kSortItems = 32;
+constexpr int kSortThreads = 512;
+constexpr int kSortItems = 16;
 
 // Sort plan: digit width, pass count and shifts for a key bit range.
 struct SortPlan {

@@ -78,27 +78,44 @@ __global__ __launch_bounds__(kFrontBlock) void k_tri_bounds(SegBlocks sb, float4
     const uint32_t lo = chunk * kFrontChunk;
     const uint32_t hi = (lo + kFrontChunk < d.n) ? lo + kFrontChunk : d.n;
     float4* __restrict__ out = tri_boxes + 2 * (size_t)d.start;
-    for (uint32_t i = lo + threadIdx.x; i < hi; i += kFrontBlock) {
-        float a0 = d.v[0][i], a1 = d.v[1][i], a2 = d.v[2][i];
-        float b0 = d.v[3][i], b1 = d.v[4][i], b2 = d.v[5][i];
-        float c0 = d.v[6][i], c1 = d.v[7][i], c2 = d.v[8][i];
-        bad |= nonfinite(a0) | nonfinite(a1) | nonfinite(a2) | nonfinite(b0) | nonfinite(b1) | nonfinite(b2) |
-               nonfinite(c0) | nonfinite(c1) | nonfinite(c2);
-        Box b;
-        b.mnx = fmin_s(a0, fmin_s(b0, c0));
-        b.mny = fmin_s(a1, fmin_s(b1, c1));
-        b.mnz = fmin_s(a2, fmin_s(b2, c2));
-        b.mxx = fmax_s(a0, fmax_s(b0, c0));
-        b.mxy = fmax_s(a1, fmax_s(b1, c1));
-        b.mxz = fmax_s(a2, fmax_s(b2, c2));
-        store_box(out + 2 * (size_t)i, b);
-        uint32_t o;
-        o = f2ord(b.mnx); omn[0] = o < omn[0] ? o : omn[0];
-        o = f2ord(b.mny); omn[1] = o < omn[1] ? o : omn[1];
-        o = f2ord(b.mnz); omn[2] = o < omn[2] ? o : omn[2];
-        o = f2ord(b.mxx); omx[0] = o > omx[0] ? o : omx[0];
-        o = f2ord(b.mxy); omx[1] = o > omx[1] ? o : omx[1];
-        o = f2ord(b.mxz); omx[2] = o > omx[2] ? o : omx[2];
+    const float* __restrict__ v[9];
+#pragma unroll
+    for (int k = 0; k < 9; ++k) v[k] = d.v[k];
+    // kFrontUnroll triangles per thread per step: all their loads are issued
+    // before any box is computed, so a thread waits on HBM once per step.
+    for (uint32_t i0 = lo + threadIdx.x; i0 < hi; i0 += kFrontUnroll * kFrontBlock) {
+        float c[kFrontUnroll][9];
+#pragma unroll
+        for (int q = 0; q < kFrontUnroll; ++q) {
+            const uint32_t i = i0 + q * kFrontBlock;
+#pragma unroll
+            for (int k = 0; k < 9; ++k) c[q][k] = (i < hi) ? v[k][i] : 0.0f;
+        }
+#pragma unroll
+        for (int q = 0; q < kFrontUnroll; ++q) {
+            const uint32_t i = i0 + q * kFrontBlock;
+            if (i >= hi) continue;
+            const float a0 = c[q][0], a1 = c[q][1], a2 = c[q][2];
+            const float b0 = c[q][3], b1 = c[q][4], b2 = c[q][5];
+            const float c0 = c[q][6], c1 = c[q][7], c2 = c[q][8];
+            bad |= nonfinite(a0) | nonfinite(a1) | nonfinite(a2) | nonfinite(b0) | nonfinite(b1) | nonfinite(b2) |
+                   nonfinite(c0) | nonfinite(c1) | nonfinite(c2);
+            Box b;
+            b.mnx = fmin_s(a0, fmin_s(b0, c0));
+            b.mny = fmin_s(a1, fmin_s(b1, c1));
+            b.mnz = fmin_s(a2, fmin_s(b2, c2));
+            b.mxx = fmax_s(a0, fmax_s(b0, c0));
+            b.mxy = fmax_s(a1, fmax_s(b1, c1));
+            b.mxz = fmax_s(a2, fmax_s(b2, c2));
+            store_box(out + 2 * (size_t)i, b);
+            uint32_t o;
+            o = f2ord(b.mnx); omn[0] = o < omn[0] ? o : omn[0];
+            o = f2ord(b.mny); omn[1] = o < omn[1] ? o : omn[1];
+            o = f2ord(b.mnz); omn[2] = o < omn[2] ? o : omn[2];
+            o = f2ord(b.mxx); omx[0] = o > omx[0] ? o : omx[0];
+            o = f2ord(b.mxy); omx[1] = o > omx[1] ? o : omx[1];
+            o = f2ord(b.mxz); omx[2] = o > omx[2] ? o : omx[2];
+        }
     }
     __shared__ uint32_t red[kFrontBlock / kWave][6];
     const int wave = threadIdx.x / kWave, lane = threadIdx.x % kWave;
@@ -188,23 +205,34 @@ __global__ __launch_bounds__(kFrontBlock) void k_morton(SegBlocks sb, const floa
     const uint32_t lo = chunk * kFrontChunk;
     const uint32_t hi = (lo + kFrontChunk < d.n) ? lo + kFrontChunk : d.n;
     const float4* __restrict__ bx = tri_boxes + 2 * (size_t)d.start;
-    for (uint32_t i = lo + threadIdx.x; i < hi; i += kFrontBlock) {
-        Box b = load_box(bx + 2 * (size_t)i);
-        float x = norm_coord(b.mnx, b.mxx, smnx, smxx);
-        float y = norm_coord(b.mny, b.mxy, smny, smxy);
-        float z = norm_coord(b.mnz, b.mxz, smnz, smxz);
-        uint64_t key;
-        const size_t g = (size_t)d.start + i;
-        if (M60) {
-            key = morton60(x, y, z);
-            vals[g] = i;
-        } else {
-            key = segbits | ((uint64_t)morton30(x, y, z) << kf.idx_bits) | (uint64_t)i;
+    for (uint32_t i0 = lo + threadIdx.x; i0 < hi; i0 += kFrontUnroll * kFrontBlock) {
+        Box bq[kFrontUnroll];
+#pragma unroll
+        for (int q = 0; q < kFrontUnroll; ++q) {
+            const uint32_t i = i0 + q * kFrontBlock;
+            if (i < hi) bq[q] = load_box(bx + 2 * (size_t)i);
         }
-        keys[g] = key;
-        for (int p = 0; p < passes; ++p) {
-            uint32_t dg = (uint32_t)(key >> sh.s[p]) & sh.m[p];
-            atomicAdd(&s_hist[p * BINS + dg], 1u);
+#pragma unroll
+        for (int q = 0; q < kFrontUnroll; ++q) {
+            const uint32_t i = i0 + q * kFrontBlock;
+            if (i >= hi) continue;
+            const Box b = bq[q];
+            float x = norm_coord(b.mnx, b.mxx, smnx, smxx);
+            float y = norm_coord(b.mny, b.mxy, smny, smxy);
+            float z = norm_coord(b.mnz, b.mxz, smnz, smxz);
+            uint64_t key;
+            const size_t g = (size_t)d.start + i;
+            if (M60) {
+                key = morton60(x, y, z);
+                vals[g] = i;
+            } else {
+                key = segbits | ((uint64_t)morton30(x, y, z) << kf.idx_bits) | (uint64_t)i;
+            }
+            keys[g] = key;
+            for (int p = 0; p < passes; ++p) {
+                uint32_t dg = (uint32_t)(key >> sh.s[p]) & sh.m[p];
+                atomicAdd(&s_hist[p * BINS + dg], 1u);
+            }
         }
     }
     __syncthreads();

@@ -10,11 +10,12 @@
 //     LDS histogram into one zeroed table with global atomics, and each
 //     onesweep tile turns its pass's row into bin starts with one block scan
 //     (no separate reduce or scan launch);
-//   * each onesweep tile (256 threads x 16 keys) takes a tile id from an
+//   * each onesweep tile (512 threads x 16 keys) takes a tile id from an
 //     atomic counter (ids follow start order, so every look-back target is
-//     already resident), ranks keys with wave64 ballot matching into
-//     per-wave LDS histograms (stable: items are consumed in input order),
-//     publishes its per-digit counts, resolves its global digit offsets by
+//     already resident), counts its digits and publishes them at once (so
+//     later tiles' look-backs pass it while it ranks), ranks keys with wave64
+//     ballot matching into per-wave LDS histograms (stable: items are
+//     consumed in input order), resolves its global digit offsets by
 //     decoupled look-back over agent-scope status words (flag | count in one
 //     32-bit word, so no payload fence is needed), reorders the tile in LDS
 //     and writes each digit run contiguously.
@@ -25,6 +26,9 @@
 namespace bvhk {
 
 constexpr int kSortBlock = kSortThreads;
+// Count and publish a tile's digits before ranking it (LDS atomics per key)
+// rather than after (from the ranking's per-wave histograms).
+constexpr bool kSortEarlyCounts = false;
 constexpr uint32_t kFlagA = 1u << 30;
 constexpr uint32_t kFlagP = 2u << 30;
 constexpr uint32_t kValMask = (1u << 30) - 1;
@@ -95,15 +99,16 @@ void launch_sort_histogram(const uint64_t* keys, uint32_t n, const SortPlan& pla
                            hist_tot);
 }
 
-// Exclusive sum over a 256-thread block; returns the prefix, *total the sum.
-__device__ __forceinline__ uint32_t block_excl_sum256(uint32_t v, uint32_t* s_w, uint32_t* total) {
+// Exclusive sum over a block of T threads; returns the prefix, *total the sum.
+template <int T>
+__device__ __forceinline__ uint32_t block_excl_sum(uint32_t v, uint32_t* s_w, uint32_t* total) {
     const int wave = threadIdx.x / kWave, lane = threadIdx.x % kWave;
     uint32_t incl = wave_incl_sum(v);
     if (lane == kWave - 1) s_w[wave] = incl;
     __syncthreads();
     uint32_t before = 0, all = 0;
 #pragma unroll
-    for (int w = 0; w < 256 / kWave; ++w) {
+    for (int w = 0; w < T / kWave; ++w) {
         uint32_t x = s_w[w];
         before += (w < wave) ? x : 0u;
         all += x;
@@ -122,8 +127,10 @@ __global__ __launch_bounds__(kSortBlock) void k_onesweep(const uint64_t* __restr
     constexpr int BINS = 1 << BITS;
     constexpr int WAVES = kSortBlock / kWave;
     constexpr int TILE = kSortBlock * kSortItems;
-    constexpr int BPT = BINS / kSortBlock;
+    // Bins per thread; with more threads than bins the first BINS threads own one each.
+    constexpr int BPT = (BINS + kSortBlock - 1) / kSortBlock;
     __shared__ uint32_t s_whist[WAVES * BINS];
+    __shared__ uint32_t s_thist[BINS];
     __shared__ uint32_t s_local[BINS];
     __shared__ uint32_t s_global[BINS];
     __shared__ uint64_t s_keys[TILE];
@@ -134,8 +141,10 @@ __global__ __launch_bounds__(kSortBlock) void k_onesweep(const uint64_t* __restr
     const int tid = threadIdx.x;
     const int wave = tid / kWave;
     const int lane = tid % kWave;
+    const bool owner = tid * BPT < BINS;
     if (tid == 0) s_tile = atomicAdd(tile_ctr, 1u);
     for (int i = tid; i < WAVES * BINS; i += kSortBlock) s_whist[i] = 0;
+    for (int i = tid; i < BINS; i += kSortBlock) s_thist[i] = 0;
     __syncthreads();
     const uint32_t tile = s_tile;
     if (tile >= gridDim.x) return;  // tile counter not cleared: publish nothing, touch nothing
@@ -152,18 +161,44 @@ __global__ __launch_bounds__(kSortBlock) void k_onesweep(const uint64_t* __restr
         key[it] = valid ? kin[idx] : ~0ull;
         if (VALS) val[it] = valid ? vin[idx] : 0u;
     }
+    // The tile's digit counts first, published before the ranking so that
+    // later tiles' look-backs can pass this tile while it ranks.
+    if (kSortEarlyCounts) {
+#pragma unroll
+        for (int it = 0; it < kSortItems; ++it) {
+            uint32_t idx = seg + it * kWave + lane;
+            if (idx < n) atomicAdd(&s_thist[(uint32_t)(key[it] >> shift) & MASK], 1u);
+        }
+    }
     // Global start of this thread's bins: exclusive scan of the pass's digit
-    // totals (its loads overlap the key loads above).
+    // totals.
     uint32_t tot[BPT];
-    uint32_t tot_sum = 0;
+    uint32_t cnt[BPT];
+    uint32_t tot_sum = 0, tsum = 0;
+#pragma unroll
+    for (int j = 0; j < BPT; ++j) cnt[j] = 0;
+    __syncthreads();
 #pragma unroll
     for (int j = 0; j < BPT; ++j) {
-        tot[j] = hist_tot[tid * BPT + j];
+        const int b = tid * BPT + j;
+        tot[j] = owner ? hist_tot[b] : 0u;
         tot_sum += tot[j];
+        if (kSortEarlyCounts) {
+            cnt[j] = owner ? s_thist[b] : 0u;
+            tsum += cnt[j];
+            if (owner)
+                __hip_atomic_store(status + (size_t)tile * BINS + b, (tile == 0 ? kFlagP : kFlagA) | cnt[j],
+                                   __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        }
     }
     uint32_t all_keys;
-    uint32_t gstart = block_excl_sum256(tot_sum, s_w, &all_keys);
+    uint32_t gstart = block_excl_sum<kSortBlock>(tot_sum, s_w, &all_keys);
+    uint32_t total;
+    uint32_t lrun = 0;
+    if (kSortEarlyCounts) lrun = block_excl_sum<kSortBlock>(tsum, s_w, &total);
 
+    // Stable ranking inside each wave (items consumed in input order) with
+    // wave64 ballot matching into per-wave LDS histograms.
     uint32_t* wh = s_whist + wave * BINS;
     const uint64_t lt_mask = lanemask_lt();
 #pragma unroll
@@ -188,12 +223,12 @@ __global__ __launch_bounds__(kSortBlock) void k_onesweep(const uint64_t* __restr
         rk[it] = r;
     }
     __syncthreads();
-
-    uint32_t cnt[BPT];
-    uint32_t tsum = 0;
+    // Per bin: exclusive prefix over the waves (in place); without early
+    // counts the tile's counts are published here.
 #pragma unroll
     for (int j = 0; j < BPT; ++j) {
-        int b = tid * BPT + j;
+        const int b = tid * BPT + j;
+        if (!owner) continue;
         uint32_t run = 0;
 #pragma unroll
         for (int w = 0; w < WAVES; ++w) {
@@ -201,13 +236,14 @@ __global__ __launch_bounds__(kSortBlock) void k_onesweep(const uint64_t* __restr
             s_whist[w * BINS + b] = run;
             run += c;
         }
-        cnt[j] = run;
-        tsum += run;
-        uint32_t word = (tile == 0 ? kFlagP : kFlagA) | run;
-        __hip_atomic_store(status + (size_t)tile * BINS + b, word, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if (!kSortEarlyCounts) {
+            cnt[j] = run;
+            tsum += run;
+            __hip_atomic_store(status + (size_t)tile * BINS + b, (tile == 0 ? kFlagP : kFlagA) | run,
+                               __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        }
     }
-    uint32_t total;
-    uint32_t lrun = block_excl_sum256(tsum, s_w, &total);
+    if (!kSortEarlyCounts) lrun = block_excl_sum<kSortBlock>(tsum, s_w, &total);
     // Decoupled look-back: the thread's BPT bins walk back together, so their
     // status loads overlap.
     uint32_t g[BPT];
@@ -217,7 +253,7 @@ __global__ __launch_bounds__(kSortBlock) void k_onesweep(const uint64_t* __restr
     for (int j = 0; j < BPT; ++j) {
         g[j] = 0;
         tj[j] = (int)tile - 1;
-        if (tile > 0) pending |= 1u << j;
+        if (tile > 0 && owner) pending |= 1u << j;
     }
     while (pending) {
         uint32_t w[BPT];
@@ -243,7 +279,8 @@ __global__ __launch_bounds__(kSortBlock) void k_onesweep(const uint64_t* __restr
     }
 #pragma unroll
     for (int j = 0; j < BPT; ++j) {
-        int b = tid * BPT + j;
+        const int b = tid * BPT + j;
+        if (!owner) continue;
         if (tile > 0)
             __hip_atomic_store(status + (size_t)tile * BINS + b, kFlagP | (g[j] + cnt[j]), __ATOMIC_RELAXED,
                                __HIP_MEMORY_SCOPE_AGENT);

@@ -233,18 +233,34 @@ __device__ void hier_chunk_body(const HierArgs& p, HierSmem& s, uint32_t t, uint
     }
     __syncthreads();
 
-    // ---- sparse table of deltas and segment tree of boxes, 10 rounds ---------
+    // ---- sparse table of deltas and segment tree of boxes: two levels per
+    // round, 5 rounds.  Table entries whose window runs past boundary kChunk
+    // are never read (the lifting below stays inside [0, m]); their inputs are
+    // clamped to the row.
+    static_assert(kLog % 2 == 0, "levels are built in pairs");
 #pragma unroll 1
-    for (int k = 1; k <= kLog; ++k) {
+    for (int k = 1; k <= kLog; k += 2) {
         const int w = 1 << (k - 1);
-        for (int q = tid; q + w <= kChunk; q += kHierThreads) {
-            const int8_t x = s.st[k - 1][q], y = s.st[k - 1][q + w];
-            s.st[k][q] = (y < x) ? y : x;
+        {
+            const int q = tid;
+            auto at = [&](int j) -> int8_t { return s.st[k - 1][(j < kChunk) ? j : kChunk]; };
+            const int8_t x0 = at(q), x1 = at(q + w), x2 = at(q + 2 * w), x3 = at(q + 3 * w);
+            const int8_t lo = (x1 < x0) ? x1 : x0;
+            const int8_t hi = (x3 < x2) ? x3 : x2;
+            s.st[k][q] = lo;
+            s.st[k + 1][q] = (hi < lo) ? hi : lo;
         }
-        const int lvl_n = kChunk >> k;
+        const int lvl_n = kChunk >> (k + 1);  // nodes on level k + 1
         if (tid < lvl_n) {
             const int h = lvl_n + tid;
-            const Box c = merge(tree_box(s, 2 * h), tree_box(s, 2 * h + 1));
+            const Box c0 = merge(tree_box(s, 4 * h), tree_box(s, 4 * h + 1));
+            const Box c1 = merge(tree_box(s, 4 * h + 2), tree_box(s, 4 * h + 3));
+            const Box c = merge(c0, c1);
+            const int h0 = 2 * h, h1 = 2 * h + 1;
+            s.box[0][h0] = c0.mnx; s.box[1][h0] = c0.mny; s.box[2][h0] = c0.mnz;
+            s.box[3][h0] = c0.mxx; s.box[4][h0] = c0.mxy; s.box[5][h0] = c0.mxz;
+            s.box[0][h1] = c1.mnx; s.box[1][h1] = c1.mny; s.box[2][h1] = c1.mnz;
+            s.box[3][h1] = c1.mxx; s.box[4][h1] = c1.mxy; s.box[5][h1] = c1.mxz;
             s.box[0][h] = c.mnx; s.box[1][h] = c.mny; s.box[2][h] = c.mnz;
             s.box[3][h] = c.mxx; s.box[4][h] = c.mxy; s.box[5][h] = c.mxz;
         }

@@ -174,8 +174,13 @@ class Context:
         return Tree(self, h)
 
     def build(self, dmesh: "DeviceMesh", tree: "Tree", algo=ALGO_LBVH, radius=25, rounds=1, radix_bits=0,
-              keep_keys=False) -> dict:
+              keep_keys=False, timing=True) -> dict:
+        """Builds `tree` from `dmesh`.  timing=False records no per-stage events
+        (the returned times are then empty)."""
         p = N.BuildParams(algo, radius, rounds, radix_bits, N.BUILD_KEEP_KEYS if keep_keys else 0)
+        if not timing:
+            self._check(self._lib.bvh_build(self.h, dmesh.h, C.byref(p), tree.h, None), "build")
+            return {"total_ms": None, "stages": {}, "ploc_iterations": 0}
         t = N.StageTimes()
         self._check(self._lib.bvh_build(self.h, dmesh.h, C.byref(p), tree.h, C.byref(t)), "build")
         return {"total_ms": float(t.total_ms), "stages": t.as_dict(), "ploc_iterations": int(t.ploc_iterations)}

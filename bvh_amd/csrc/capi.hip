@@ -194,24 +194,30 @@ int check_launch(bvh_ctx_t ctx, const char* what) {
     return BVH_OK;
 }
 
+// Per-stage HIP events, recorded only when the caller asked for times (a
+// build without them enqueues its kernels and nothing else; every builder
+// ends with check_input_flags, which synchronises the stream).
 struct StageClock {
     bvh_ctx_t ctx;
+    bool on = true;
     int k = 0;
     const char* names[BVH_MAX_STAGES];
     int begin() {
         k = 0;
-        HIP_TRY(ctx, hipEventRecord(ctx->ev[0], ctx->stream));
+        if (on) HIP_TRY(ctx, hipEventRecord(ctx->ev[0], ctx->stream));
         return BVH_OK;
     }
     // Past the last slot, the final stage absorbs the remaining work (the
     // total stays exact).
     int mark(const char* name) {
+        if (!on) return BVH_OK;
         if (k < BVH_MAX_STAGES) ++k;
         names[k - 1] = name;
         HIP_TRY(ctx, hipEventRecord(ctx->ev[k], ctx->stream));
         return BVH_OK;
     }
     int finish(bvh_stage_times* t) {
+        if (!on) return BVH_OK;
         HIP_TRY(ctx, hipEventSynchronize(ctx->ev[k]));
         if (!t) return BVH_OK;
         std::memset(t, 0, sizeof(*t));
@@ -395,7 +401,7 @@ int build_lbvh(bvh_ctx_t ctx, bvh_mesh_t mesh, const bvh_build_params* prm, bvh_
     SortPlan plan = plan_for(j, prm->radix_bits);
     Workspace& w = ctx->ws;
     TRY(reserve_front(ctx, w, j, plan, true));
-    StageClock clk{ctx};
+    StageClock clk{ctx, times != nullptr};
     TRY(clk.begin());
     uint64_t* keys = nullptr;
     uint32_t* vals = nullptr;
@@ -423,7 +429,7 @@ int build_lbvh_plus(bvh_ctx_t ctx, bvh_mesh_t mesh, const bvh_build_params* prm,
     TRY(ensure(ctx, w.parents, (size_t)(2 * (size_t)n) * 4));
     TRY(ensure(ctx, w.ranges, (size_t)n * 8));
     TRY(ensure(ctx, w.counters, (size_t)n * 4, true));
-    StageClock clk{ctx};
+    StageClock clk{ctx, times != nullptr};
     TRY(clk.begin());
     uint64_t* keys = nullptr;
     uint32_t* vals = nullptr;
@@ -567,7 +573,7 @@ int build_ploc(bvh_ctx_t ctx, bvh_mesh_t mesh, const bvh_build_params* prm, bvh_
     Workspace& w = ctx->ws;
     TRY(reserve_front(ctx, w, j, plan, false));
     TRY(reserve_ploc(ctx, w, j));
-    StageClock clk{ctx};
+    StageClock clk{ctx, times != nullptr};
     TRY(clk.begin());
     uint64_t* keys = nullptr;
     uint32_t* vals = nullptr;
@@ -680,7 +686,7 @@ int build_batched_eager(bvh_ctx_t ctx, const bvh_mesh_t* meshes, const bvh_tree_
     Workspace& w = ctx->ws;
     TRY(reserve_front(ctx, w, j, plan, prm->algo == BVH_ALGO_LBVH));
     if (prm->algo == BVH_ALGO_PLOC) TRY(reserve_ploc(ctx, w, j));
-    StageClock clk{ctx};
+    StageClock clk{ctx, times != nullptr};
     TRY(clk.begin());
     uint64_t* keys = nullptr;
     uint32_t* vals = nullptr;

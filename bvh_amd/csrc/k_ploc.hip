@@ -372,30 +372,43 @@ __device__ void merge_tiles(const PlocArgs& a, const PlocState& st, int kn, uint
         }
         uint64_t tagg;
         const uint64_t excl = block_excl_scan_seg<kMergeBlock>(agg, s_w, &tagg);
-        if (threadIdx.x == 0) {
+        // Decoupled look-back by wave 0, 64 predecessors' words per step; the
+        // window is combined in tile order (farther first) under scomb.
+        if (threadIdx.x < kWave) {
+            const int lane = threadIdx.x;
             uint64_t* sw = a.status + tile;
-            if (tile == 0) {
-                __hip_atomic_store(sw, kStP | tagg, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                *s_prefix = 0;
-            } else {
-                __hip_atomic_store(sw, kStA | tagg, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                uint64_t pre = 0;
-                int t = (int)tile - 1;
-                while (true) {
-                    const uint64_t w = __hip_atomic_load(a.status + t, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                    const uint64_t f = w & kStFlags;
-                    if (f == 0) {
-                        __builtin_amdgcn_s_sleep(1);
-                        continue;
-                    }
-                    pre = scomb(w & ~kStFlags, pre);
-                    if (f == kStP) break;
-                    --t;
+            if (lane == 0)
+                __hip_atomic_store(sw, (tile == 0 ? kStP : kStA) | tagg, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            uint64_t pre = 0;
+            int top = (int)tile - 1;  // nearest predecessor not yet combined
+            while (top >= 0) {
+                const int j = top - lane;
+                uint64_t w = kStP;  // before tile 0: an inclusive prefix of 0 (never combined past a P)
+                if (j >= 0) w = __hip_atomic_load(a.status + j, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                const uint64_t f = w & kStFlags;
+                const uint64_t pmask = __ballot(f == kStP);
+                const uint64_t nready = __ballot(f == 0);
+                const int first_p = pmask ? __ffsll((long long)pmask) - 1 : kWave - 1;
+                const uint64_t need = (first_p == kWave - 1) ? ~0ull : ((2ull << first_p) - 1);
+                if (nready & need) {
+                    __builtin_amdgcn_s_sleep(1);
+                    continue;
                 }
-                __hip_atomic_store(sw, kStP | scomb(pre, tagg), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                *s_prefix = pre;
+                uint64_t v = (lane <= first_p) ? (w & ~kStFlags) : 0ull;
+#pragma unroll
+                for (int o = 1; o < kWave; o <<= 1) {
+                    const uint64_t other = __shfl_down(v, o, kWave);
+                    if (lane + o < kWave) v = scomb(other, v);
+                }
+                pre = scomb(__shfl(v, 0, kWave), pre);
+                if (pmask) break;
+                top -= kWave;
             }
-            if (tile == tiles - 1) a.state[kn].m = (uint32_t)(scomb(*s_prefix, tagg) & kVMask);
+            if (lane == 0) {
+                if (tile > 0) __hip_atomic_store(sw, kStP | scomb(pre, tagg), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                *s_prefix = pre;
+                if (tile == tiles - 1) a.state[kn].m = (uint32_t)(scomb(pre, tagg) & kVMask);
+            }
         }
         __syncthreads();
         const uint64_t run0 = scomb(*s_prefix, excl);

@@ -506,14 +506,20 @@ __global__ __launch_bounds__(kTailBlock) void k_ploc_tail(PlocArgs a, int k) {
     }
     __syncthreads();
     while (m > 1) {
-        for (uint32_t i = threadIdx.x; i < m; i += kTailBlock) {
+        // Lanes per cluster: as m shrinks, each window is split over up to 64
+        // lanes (strided candidates, then a shuffle min over the lane group;
+        // the packed key's minimum is unique, so the split is exact).
+        uint32_t L = 1;
+        while (L < (uint32_t)kWave && m * (2 * L) <= (uint32_t)kTailBlock) L <<= 1;
+        const uint32_t sub = threadIdx.x & (L - 1);
+        for (uint32_t i = threadIdx.x / L; i < m; i += kTailBlock / L) {
             Box bi;
             bi.mnx = s_b[0][i]; bi.mny = s_b[1][i]; bi.mnz = s_b[2][i];
             bi.mxx = s_b[3][i]; bi.mxy = s_b[4][i]; bi.mxz = s_b[5][i];
             const int lo = ((int)i - r > 0) ? (int)i - r : 0;
             const int hi = ((int)i + r < (int)m - 1) ? (int)i + r : (int)m - 1;
             uint64_t best = ~0ull;
-            for (int j = lo; j <= hi; ++j) {
+            for (int j = lo + (int)sub; j <= hi; j += (int)L) {
                 if (j == (int)i) continue;
                 Box bj;
                 bj.mnx = s_b[0][j]; bj.mny = s_b[1][j]; bj.mnz = s_b[2][j];
@@ -521,7 +527,11 @@ __global__ __launch_bounds__(kTailBlock) void k_ploc_tail(PlocArgs a, int k) {
                 uint64_t key = nn_key(area(merge(bi, bj)), (uint32_t)j);
                 best = key < best ? key : best;
             }
-            s_nn[i] = (best == ~0ull) ? -1 : (int32_t)(uint32_t)best;
+            for (uint32_t o = L >> 1; o >= 1; o >>= 1) {
+                const uint64_t other = __shfl_xor(best, (int)o, kWave);
+                best = other < best ? other : best;
+            }
+            if (sub == 0) s_nn[i] = (best == ~0ull) ? -1 : (int32_t)(uint32_t)best;
         }
         __syncthreads();
         // Per thread PER consecutive slots; scan of packed (merges, survivors).

@@ -87,6 +87,77 @@ __device__ __forceinline__ Box node_box(const DNode* nodes, uint32_t i) {
     return load_box(reinterpret_cast<const float4*>(nodes + i));
 }
 
+// Node record written with agent-scope atomic stores (coherent across XCDs
+// without flushing the L2), and the matching loads.
+__device__ __forceinline__ void store_node_coherent(DNode* nodes, uint32_t i, const Box& b, int32_t l, int32_t r) {
+    float* f = reinterpret_cast<float*>(nodes + i);
+    int32_t* q = reinterpret_cast<int32_t*>(nodes + i);
+    __hip_atomic_store(f + 0, b.mnx, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __hip_atomic_store(f + 1, b.mny, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __hip_atomic_store(f + 2, b.mnz, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __hip_atomic_store(f + 3, 0.0f, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __hip_atomic_store(f + 4, b.mxx, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __hip_atomic_store(f + 5, b.mxy, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __hip_atomic_store(f + 6, b.mxz, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __hip_atomic_store(f + 7, 0.0f, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __hip_atomic_store(q + 8, l, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __hip_atomic_store(q + 9, r, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    q[10] = 0;
+    q[11] = 0;
+}
+
+__device__ __forceinline__ Box load_box_coherent(const DNode* nodes, uint32_t i) {
+    const float* f = reinterpret_cast<const float*>(nodes + i);
+    Box b;
+    b.mnx = __hip_atomic_load(f + 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    b.mny = __hip_atomic_load(f + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    b.mnz = __hip_atomic_load(f + 2, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    b.mxx = __hip_atomic_load(f + 4, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    b.mxy = __hip_atomic_load(f + 5, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    b.mxz = __hip_atomic_load(f + 6, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    return b;
+}
+
+// Agent-coherent loads and stores of whole records (64-bit atomic accesses):
+// data shared between workgroups of one launch that may sit on different XCDs,
+// without whole-L2 writebacks or invalidates.
+__device__ __forceinline__ DNode load_node_coherent(const DNode* nodes, uint32_t i) {
+    const uint64_t* q = reinterpret_cast<const uint64_t*>(nodes + i);
+    uint64_t w[6];
+#pragma unroll
+    for (int k = 0; k < 6; ++k) w[k] = __hip_atomic_load(q + k, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    DNode d;
+    d.mn = make_float4(__uint_as_float((uint32_t)w[0]), __uint_as_float((uint32_t)(w[0] >> 32)),
+                       __uint_as_float((uint32_t)w[1]), __uint_as_float((uint32_t)(w[1] >> 32)));
+    d.mx = make_float4(__uint_as_float((uint32_t)w[2]), __uint_as_float((uint32_t)(w[2] >> 32)),
+                       __uint_as_float((uint32_t)w[3]), __uint_as_float((uint32_t)(w[3] >> 32)));
+    d.lr = make_int4((int)(uint32_t)w[4], (int)(uint32_t)(w[4] >> 32), (int)(uint32_t)w[5], (int)(uint32_t)(w[5] >> 32));
+    return d;
+}
+__device__ __forceinline__ int2 load_int2_coherent(const int2* p) {
+    const uint64_t w = __hip_atomic_load(reinterpret_cast<const uint64_t*>(p), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    return make_int2((int)(uint32_t)w, (int)(uint32_t)(w >> 32));
+}
+__device__ __forceinline__ void store_int2_coherent(int2* p, int2 v) {
+    const uint64_t w = (uint64_t)(uint32_t)v.x | ((uint64_t)(uint32_t)v.y << 32);
+    __hip_atomic_store(reinterpret_cast<uint64_t*>(p), w, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void store_u32_coherent(uint32_t* p, uint32_t v) {
+    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void store_box_coherent(DNode* nodes, uint32_t i, const Box& b) {
+    float* f = reinterpret_cast<float*>(nodes + i);
+    __hip_atomic_store(f + 0, b.mnx, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __hip_atomic_store(f + 1, b.mny, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __hip_atomic_store(f + 2, b.mnz, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __hip_atomic_store(f + 3, 0.0f, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __hip_atomic_store(f + 4, b.mxx, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __hip_atomic_store(f + 5, b.mxy, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __hip_atomic_store(f + 6, b.mxz, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __hip_atomic_store(f + 7, 0.0f, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+
 // Order-preserving map float -> uint32 (used for the order-independent
 // scene-bounds reduction through integer atomics).
 __device__ __forceinline__ uint32_t f2ord(float f) {

@@ -75,37 +75,6 @@ __device__ __forceinline__ int key_delta(const uint64_t* __restrict__ keys, cons
     return z == 0 ? 64 : __clzll((long long)z);
 }
 
-// Node record written with agent-scope atomic stores (coherent across XCDs
-// without flushing the L2), and the matching loads.
-__device__ __forceinline__ void store_node_coherent(DNode* nodes, uint32_t i, const Box& b, int32_t l, int32_t r) {
-    float* f = reinterpret_cast<float*>(nodes + i);
-    int32_t* q = reinterpret_cast<int32_t*>(nodes + i);
-    __hip_atomic_store(f + 0, b.mnx, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    __hip_atomic_store(f + 1, b.mny, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    __hip_atomic_store(f + 2, b.mnz, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    __hip_atomic_store(f + 3, 0.0f, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    __hip_atomic_store(f + 4, b.mxx, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    __hip_atomic_store(f + 5, b.mxy, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    __hip_atomic_store(f + 6, b.mxz, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    __hip_atomic_store(f + 7, 0.0f, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    __hip_atomic_store(q + 8, l, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    __hip_atomic_store(q + 9, r, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    q[10] = 0;
-    q[11] = 0;
-}
-
-__device__ __forceinline__ Box load_box_coherent(const DNode* nodes, uint32_t i) {
-    const float* f = reinterpret_cast<const float*>(nodes + i);
-    Box b;
-    b.mnx = __hip_atomic_load(f + 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    b.mny = __hip_atomic_load(f + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    b.mnz = __hip_atomic_load(f + 2, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    b.mxx = __hip_atomic_load(f + 4, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    b.mxy = __hip_atomic_load(f + 5, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    b.mxz = __hip_atomic_load(f + 6, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    return b;
-}
-
 __device__ __forceinline__ SegDesc seg_desc(const SegBlocks& sb, uint32_t seg) {
     return sb.segs ? sb.segs[seg] : sb.single;
 }

@@ -505,7 +505,7 @@ __global__ __launch_bounds__(kTailBlock) void k_ploc_tail(PlocArgs a, int k) {
         s_id[i] = ids[i];
     }
     __syncthreads();
-    while (m > 1) {
+    while (m > (uint32_t)kWave) {
         // Lanes per cluster: as m shrinks, each window is split over up to 64
         // lanes (strided candidates, then a shuffle min over the lane group;
         // the packed key's minimum is unique, so the split is exact).
@@ -605,6 +605,93 @@ __global__ __launch_bounds__(kTailBlock) void k_ploc_tail(PlocArgs a, int k) {
         if (merges == 0) {
             flags |= kPlocNoProgress;
             break;
+        }
+    }
+    // At most 64 clusters left: wave 0 finishes alone, wave-synchronously (no
+    // workgroup barriers).  Same order of merges, ids and survivors as above.
+    if (threadIdx.x < kWave && !(flags & kPlocNoProgress)) {
+        const uint32_t lane = threadIdx.x;
+        while (m > 1) {
+            uint32_t L = 1;
+            while (L < (uint32_t)kWave && m * (2 * L) <= (uint32_t)kWave) L <<= 1;
+            const uint32_t sub = lane & (L - 1);
+            const uint32_t ci = lane / L;
+            uint64_t best = ~0ull;
+            if (ci < m) {
+                Box bi;
+                bi.mnx = s_b[0][ci]; bi.mny = s_b[1][ci]; bi.mnz = s_b[2][ci];
+                bi.mxx = s_b[3][ci]; bi.mxy = s_b[4][ci]; bi.mxz = s_b[5][ci];
+                const int lo = ((int)ci - r > 0) ? (int)ci - r : 0;
+                const int hi = ((int)ci + r < (int)m - 1) ? (int)ci + r : (int)m - 1;
+                for (int j = lo + (int)sub; j <= hi; j += (int)L) {
+                    if (j == (int)ci) continue;
+                    Box bj;
+                    bj.mnx = s_b[0][j]; bj.mny = s_b[1][j]; bj.mnz = s_b[2][j];
+                    bj.mxx = s_b[3][j]; bj.mxy = s_b[4][j]; bj.mxz = s_b[5][j];
+                    const uint64_t key = nn_key(area(merge(bi, bj)), (uint32_t)j);
+                    best = key < best ? key : best;
+                }
+            }
+            for (uint32_t o = L >> 1; o >= 1; o >>= 1) {
+                const uint64_t other = __shfl_xor(best, (int)o, kWave);
+                best = other < best ? other : best;
+            }
+            if (ci < m && sub == 0) s_nn[ci] = (best == ~0ull) ? -1 : (int32_t)(uint32_t)best;
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+            // One lane per cluster: mutual pairs merge into the lower slot.
+            const uint32_t i = lane;
+            bool valid = false, merging = false;
+            Box bb;
+            uint32_t id_i = 0, id_j = 0;
+            if (i < m) {
+                const int32_t j = s_nn[i];
+                const bool mutual = (j >= 0) && s_nn[j] == (int32_t)i;
+                merging = mutual && (int32_t)i < j;
+                valid = !(mutual && j < (int32_t)i);
+                bb.mnx = s_b[0][i]; bb.mny = s_b[1][i]; bb.mnz = s_b[2][i];
+                bb.mxx = s_b[3][i]; bb.mxy = s_b[4][i]; bb.mxz = s_b[5][i];
+                id_i = s_id[i];
+                if (merging) {
+                    Box bj;
+                    bj.mnx = s_b[0][j]; bj.mny = s_b[1][j]; bj.mnz = s_b[2][j];
+                    bj.mxx = s_b[3][j]; bj.mxy = s_b[4][j]; bj.mxz = s_b[5][j];
+                    id_j = s_id[j];
+                    bb = merge(bb, bj);
+                }
+            }
+            const uint64_t vmask = __ballot(valid), mmask = __ballot(merging);
+            const uint64_t below = lanemask_lt();
+            const uint32_t pos = (uint32_t)__popcll(vmask & below);
+            uint32_t out_id = id_i;
+            if (merging) {
+                const uint32_t nid = node_base + (uint32_t)__popcll(mmask & below);
+                if (nid <= root)
+                    store_node(d.nodes, remap_id(nid, root), bb, (int32_t)remap_id(id_i, root),
+                               (int32_t)remap_id(id_j, root));
+                out_id = nid;
+            }
+            // Every lane has read its inputs before any slot is overwritten.
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+            if (valid) {
+                s_b[0][pos] = bb.mnx; s_b[1][pos] = bb.mny; s_b[2][pos] = bb.mnz;
+                s_b[3][pos] = bb.mxx; s_b[4][pos] = bb.mxy; s_b[5][pos] = bb.mxz;
+                s_id[pos] = out_id;
+            }
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+            const uint32_t merges = (uint32_t)__popcll(mmask);
+            m = (uint32_t)__popcll(vmask);
+            node_base += merges;
+            ++iters;
+            if (merges == 0) {
+                flags |= kPlocNoProgress;
+                break;
+            }
         }
     }
     if (threadIdx.x == 0) a.out[seg] = make_uint4(m, node_base, iters, flags);

@@ -17,19 +17,22 @@
 // Every float operation matches the oracle's serial transcription in order, so
 // the resulting trees are bit-identical to it.
 //
-// MI355X mapping: threads climb from the leaves as in the reference; at each
-// climbing step, every lane that became a second arrival forms its treelet
-// at once in its own registers (seven dependent rounds of node loads, issued
-// by all such lanes together), then the wave runs the dynamic programmes of
-// up to eight treelets side by side on equal lane groups (64 lanes for a
-// lone treelet, 8 each for eight), every subset size spread over the group
-// with partitions split across lanes and reduced in enumeration order, and
-// each owner lane rebuilds its treelet from the group's partition table.  A workgroup owns 2048 sorted
-// leaves; a node whose leaf span lies inside the chunk hands off through a
-// workgroup-scope counter (no cross-XCD coherence traffic), only
-// chunk-straddling nodes use agent scope.  Leaf spans are kept per internal
-// node (min/max sorted leaf position) and updated when a treelet reassigns
-// its internal nodes.
+// MI355X mapping: threads climb from the leaves as in the reference, each
+// advancing its eight chains together; at each climbing step every chain
+// that became a second arrival is ready, and the wave forms up to eight
+// ready treelets at once into LDS slots (seven dependent rounds of node
+// loads, issued by all owners together), runs their dynamic programmes side
+// by side on equal lane groups (64 lanes for a lone treelet, 8 each for
+// eight), every subset size spread over the group with partitions split
+// across lanes and reduced in enumeration order, and each owner lane rebuilds
+// its treelet from the group's partition table.  A workgroup owns 2048
+// sorted leaves; a node whose leaf span lies inside the chunk hands off
+// through a workgroup-scope counter, only chunk-straddling nodes use agent
+// scope.  Records are written with agent-coherent stores and read coherently
+// only for straddling treelets, and every arrival waits for its own stores
+// first, so no whole-L2 writeback or invalidate is issued.  Leaf spans are
+// kept per internal node (min/max sorted leaf position) and updated when a
+// treelet reassigns its internal nodes.
 #include "dev_common.hpp"
 #include "engine.hpp"
 
@@ -95,15 +98,23 @@ __device__ __forceinline__ Box node_box_of(const DNode& d) {
 }
 
 __device__ __forceinline__ void nodes_refit(const TreeletArgs& a, int32_t root, const Box& b) {
-    a.nodes[root].mn = make_float4(b.mnx, b.mny, b.mnz, 0.0f);
-    a.nodes[root].mx = make_float4(b.mxx, b.mxy, b.mxz, 0.0f);
+    store_box_coherent(a.nodes, (uint32_t)root, b);
 }
 
 
+template <bool COH>
 __device__ __forceinline__ int2 leaf_span(const TreeletArgs& a, int32_t node) {
     const uint32_t first_leaf = a.n - 1;
     if ((uint32_t)node >= first_leaf) return make_int2((int)(node - first_leaf), (int)(node - first_leaf));
-    return a.ranges[node];
+    return COH ? load_int2_coherent(a.ranges + node) : a.ranges[node];
+}
+
+// A node record: plain loads inside the workgroup's own chunk, agent-coherent
+// loads for treelets whose records other workgroups (possibly on other XCDs)
+// wrote during this launch.
+template <bool COH>
+__device__ __forceinline__ DNode node_rec(const DNode* nodes, int32_t i) {
+    return COH ? load_node_coherent(nodes, (uint32_t)i) : nodes[i];
 }
 
 // Union of the subset's leaf boxes in ascending index order from the empty
@@ -123,10 +134,11 @@ __device__ __forceinline__ Box subset_box(BoxAt at, int s) {
 // until 7 leaves.  Each act leaf's children come with its node record, so
 // an expansion costs one round of loads.  Returns the refitted root box
 // (the reference's union of the root's children) and the leaf count.
+template <bool COH>
 __device__ int form(const TreeletArgs& a, int32_t root, Slot& sl, Box& rootbox) {
     const DNode* __restrict__ nodes = a.nodes;
-    const int4 rlr = nodes[root].lr;
-    const int2 rsp = a.ranges[root];
+    const int4 rlr = node_rec<COH>(nodes, root).lr;
+    const int2 rsp = COH ? load_int2_coherent(a.ranges + root) : a.ranges[root];
     sl.act[0] = root;
     sl.lc[0] = rlr.x;
     sl.rc[0] = rlr.y;
@@ -134,8 +146,8 @@ __device__ int form(const TreeletArgs& a, int32_t root, Slot& sl, Box& rootbox) 
     sl.hi[0] = rsp.y;
     uint32_t leafmask = 0;
     int n = 1, nexp = 0;
-    DNode cl = nodes[rlr.x];
-    DNode cr = nodes[rlr.y];
+    DNode cl = node_rec<COH>(nodes, rlr.x);
+    DNode cr = node_rec<COH>(nodes, rlr.y);
     rootbox = merge(node_box_of(cl), node_box_of(cr));
     lds_put(sl.lb[0], rootbox);
     int best = 0;
@@ -153,13 +165,13 @@ __device__ int form(const TreeletArgs& a, int32_t root, Slot& sl, Box& rootbox) 
                 }
             }
             if (best < 0) break;
-            cl = nodes[sl.lc[best]];
-            cr = nodes[sl.rc[best]];
+            cl = node_rec<COH>(nodes, sl.lc[best]);
+            cr = node_rec<COH>(nodes, sl.rc[best]);
         } else if (!(area(rootbox) > -1.0f)) {
             break;  // the reference's argmax finds nothing (non-finite area)
         }
         const int32_t l = sl.lc[best], r = sl.rc[best];
-        const int2 sp_l = leaf_span(a, l), sp_r = leaf_span(a, r);
+        const int2 sp_l = leaf_span<COH>(a, l), sp_r = leaf_span<COH>(a, r);
         sl.exn[nexp] = sl.act[best];
         sl.exl[nexp] = l;
         sl.exr[nexp] = r;
@@ -350,10 +362,12 @@ __device__ void rebuild(const TreeletArgs& a, Slot& sl, int32_t root) {
                 lo = sl.lo[i] < lo ? sl.lo[i] : lo;
                 hi = sl.hi[i] > hi ? sl.hi[i] : hi;
             }
-        store_node(nodes, (uint32_t)cur, b, ln, rn);
-        a.parents[ln] = (uint32_t)cur;
-        a.parents[rn] = (uint32_t)cur;
-        a.ranges[cur] = make_int2(lo, hi);
+        // Coherent stores: an ancestor's treelet may be formed by another
+        // workgroup on another XCD later in this launch.
+        store_node_coherent(nodes, (uint32_t)cur, b, ln, rn);
+        store_u32_coherent(a.parents + ln, (uint32_t)cur);
+        store_u32_coherent(a.parents + rn, (uint32_t)cur);
+        store_int2_coherent(a.ranges + cur, make_int2(lo, hi));
     }
 }
 
@@ -365,59 +379,84 @@ __global__ __launch_bounds__(kTBlock) void k_treelet(TreeletArgs a) {
     const uint32_t ca = blockIdx.x * kTChunk;
     const uint32_t cb = (ca + kTChunk < n) ? ca + kTChunk : n;
     const uint32_t tag = 2 * a.epoch + 1;
-    for (int item = 0; item < kTItems; ++item) {
-        const uint32_t i = ca + item * kTBlock + threadIdx.x;
-        bool active = (i < cb) && (n > 1);
-        uint32_t cur = n - 1 + i;
-        while (__ballot(active) != 0) {
-            bool pending = false;
+    // Every thread advances its kTItems chains together, so a wave always has
+    // every ready treelet of all its chains to fill its kGroups slots (one
+    // chain after another would leave most rounds with one or two treelets).
+    uint32_t cur[kTItems];
+    uint32_t act = 0;
+#pragma unroll
+    for (int it = 0; it < kTItems; ++it) {
+        const uint32_t i = ca + it * kTBlock + threadIdx.x;
+        cur[it] = n - 1 + i;
+        if (i < cb && n > 1) act |= 1u << it;
+    }
+    while (__ballot(act != 0) != 0) {
+        // This lane's record stores (coherent) are complete before its
+        // arrivals are counted; a second arrival then reads them with plain
+        // loads inside the chunk (same CU) or coherent loads across chunks.
+        // No acquire/release fence, so no whole-L2 writeback or invalidate.
+        __builtin_amdgcn_s_waitcnt(0);
+        uint32_t par[kTItems];
+        uint32_t pend = 0, glob = 0;
+#pragma unroll
+        for (int it = 0; it < kTItems; ++it) {
+            par[it] = 0;
+            if (!((act >> it) & 1u)) continue;
+            const uint32_t pp = a.parents[cur[it]];
+            par[it] = pp;
+            const int2 sp = a.ranges[pp];
+            const bool local = sp.x >= (int)ca && sp.y < (int)cb;
+            const uint32_t old =
+                local ? __hip_atomic_fetch_max(a.counters + pp, tag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP)
+                      : __hip_atomic_fetch_max(a.counters + pp, tag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            if (old == tag) {
+                pend |= 1u << it;
+                if (!local) glob |= 1u << it;
+            } else {
+                act &= ~(1u << it);
+            }
+        }
+        // Ready treelets, up to kGroups per round (a lane offers its lowest
+        // pending chain): formed by their owners into LDS slots, optimised
+        // by lane groups, rebuilt by owners.
+        while (true) {
+            const bool todo = pend != 0;
+            const uint64_t tmask = __ballot(todo);
+            if (tmask == 0) break;
+            const int it0 = todo ? __ffs((int)pend) - 1 : 0;
+            uint32_t root = 0;
             bool global = false;
-            uint32_t par = 0;
-            if (active) {
-                par = a.parents[cur];
-                const int2 sp = a.ranges[par];
-                const bool local = sp.x >= (int)ca && sp.y < (int)cb;
-                uint32_t old;
-                if (local) {
-                    old = __hip_atomic_fetch_max(a.counters + par, tag, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_WORKGROUP);
-                } else {
-                    old = __hip_atomic_fetch_max(a.counters + par, tag, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
-                    global = true;
+#pragma unroll
+            for (int it = 0; it < kTItems; ++it)
+                if (it == it0) {
+                    root = par[it];
+                    global = (glob >> it) & 1u;
                 }
-                if (old == tag) pending = true;
-                else active = false;
+            const int rank = todo ? __popcll(tmask & lanemask_lt()) : kGroups;
+            const bool mine = rank < kGroups;
+            Slot& ms = w.slot[mine ? rank : 0];
+            Box rootbox;
+            int tn = 0;
+            if (mine) tn = global ? form<true>(a, (int32_t)root, ms, rootbox) : form<false>(a, (int32_t)root, ms, rootbox);
+            wave_sync();
+            // T treelets this round, each on a group of G lanes.
+            const int T = __popcll(tmask) < kGroups ? __popcll(tmask) : kGroups;
+            const int G = T == 1 ? 64 : (T == 2 ? 32 : (T <= 4 ? 16 : 8));
+            const int gi = lane / G, sub = lane & (G - 1);
+            if (gi < T && w.slot[gi].n >= 2) dp_group(w.slot[gi], sub, G);
+            wave_sync();
+            if (mine) {
+                if (tn >= 2) rebuild(a, ms, (int32_t)root);
+                else nodes_refit(a, (int32_t)root, rootbox);  // nothing to optimise
+                pend &= ~(1u << it0);
+#pragma unroll
+                for (int it = 0; it < kTItems; ++it)
+                    if (it == it0) {
+                        cur[it] = root;
+                        if (root == 0) act &= ~(1u << it);
+                    }
             }
-            if (__ballot(global && pending) != 0) __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
-            // Ready treelets, up to eight per round: formed by their owners
-            // into LDS slots, optimised by 8-lane groups, rebuilt by owners.
-            bool todo = pending;
-            while (true) {
-                const uint64_t tmask = __ballot(todo);
-                if (tmask == 0) break;
-                const int rank = todo ? __popcll(tmask & lanemask_lt()) : kGroups;
-                const bool mine = rank < kGroups;
-                Slot& ms = w.slot[mine ? rank : 0];
-                Box rootbox;
-                int tn = 0;
-                if (mine) tn = form(a, (int32_t)par, ms, rootbox);
-                wave_sync();
-                // T treelets this round, each on a group of G lanes.
-                const int T = __popcll(tmask) < kGroups ? __popcll(tmask) : kGroups;
-                const int G = T == 1 ? 64 : (T == 2 ? 32 : (T <= 4 ? 16 : 8));
-                const int gi = lane / G, sub = lane & (G - 1);
-                if (gi < T && w.slot[gi].n >= 2) dp_group(w.slot[gi], sub, G);
-                wave_sync();
-                if (mine) {
-                    if (tn >= 2) rebuild(a, ms, (int32_t)par);
-                    else nodes_refit(a, (int32_t)par, rootbox);  // nothing to optimise
-                    todo = false;
-                }
-                wave_sync();
-            }
-            if (pending) {
-                cur = par;
-                if (par == 0) active = false;
-            }
+            wave_sync();
         }
     }
 }

@@ -18,15 +18,16 @@
 // the resulting trees are bit-identical to it.
 //
 // MI355X mapping: threads climb from the leaves as in the reference, each
-// advancing its eight chains together; at each climbing step every chain
+// advancing its two to eight chains together; at each climbing step every chain
 // that became a second arrival is ready, and the wave forms up to eight
 // ready treelets at once into LDS slots (seven dependent rounds of node
 // loads, issued by all owners together), runs their dynamic programmes side
 // by side on equal lane groups (64 lanes for a lone treelet, 8 each for
 // eight), every subset size spread over the group with partitions split
 // across lanes and reduced in enumeration order, and each owner lane rebuilds
-// its treelet from the group's partition table.  A workgroup owns 2048
-// sorted leaves; a node whose leaf span lies inside the chunk hands off
+// its treelet from the group's partition table.  A workgroup owns 512-2048
+// sorted leaves (fewer for small trees, so that at least 512 workgroups run);
+// a node whose leaf span lies inside the chunk hands off
 // through a workgroup-scope counter, only chunk-straddling nodes use agent
 // scope.  Records are written with agent-coherent stores and read coherently
 // only for straddling treelets, and every arrival waits for its own stores
@@ -39,8 +40,7 @@
 namespace bvhk {
 
 constexpr int kTBlock = 256;
-constexpr int kTItems = 8;
-constexpr int kTChunk = kTBlock * kTItems;
+constexpr int kTItemsMax = 8;  // leaf chains per thread (2, 4 or 8: a chunk of kTBlock x items leaves)
 constexpr int kTWaves = kTBlock / kWave;
 constexpr int kTMax = 7;
 constexpr int kGroups = 8;               // treelets per wave per round (groups of >= 8 lanes)
@@ -371,7 +371,10 @@ __device__ void rebuild(const TreeletArgs& a, Slot& sl, int32_t root) {
     }
 }
 
+template <int ITEMS>
 __global__ __launch_bounds__(kTBlock) void k_treelet(TreeletArgs a) {
+    constexpr int kTItems = ITEMS;
+    constexpr int kTChunk = kTBlock * ITEMS;
     __shared__ WaveLds s_w[kTWaves];
     const int lane = threadIdx.x % kWave;
     WaveLds& w = s_w[threadIdx.x / kWave];
@@ -462,8 +465,14 @@ __global__ __launch_bounds__(kTBlock) void k_treelet(TreeletArgs a) {
 }
 
 void launch_treelet(const TreeletArgs& a, hipStream_t s) {
-    uint32_t grid = (a.n + kTChunk - 1) / kTChunk;
-    hipLaunchKernelGGL(k_treelet, dim3(grid), dim3(kTBlock), 0, s, a);
+    // Large trees: eight chains per thread (longer chunks, fewer straddling
+    // nodes); small ones: fewer, so that at least 512 workgroups share the CUs.
+    int items = kTItemsMax;
+    while (items > 2 && (a.n + kTBlock * items - 1) / (uint32_t)(kTBlock * items) < 512) items >>= 1;
+    const uint32_t grid = (a.n + kTBlock * items - 1) / (uint32_t)(kTBlock * items);
+    if (items == 8) hipLaunchKernelGGL(k_treelet<8>, dim3(grid), dim3(kTBlock), 0, s, a);
+    else if (items == 4) hipLaunchKernelGGL(k_treelet<4>, dim3(grid), dim3(kTBlock), 0, s, a);
+    else hipLaunchKernelGGL(k_treelet<2>, dim3(grid), dim3(kTBlock), 0, s, a);
 }
 
 }  // namespace bvhk

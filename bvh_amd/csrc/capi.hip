@@ -71,6 +71,8 @@ struct bvh_context {
     SegTables tabs;  // eager segmented builds
     DevBuf trace_ovf, trace_list, trace_io;  // traversal queue and host-call staging
     DevBuf ray_keys[2];                      // ray coherence keys (sorted before tracing)
+    uint32_t* host_flags = nullptr;          // pinned: each build's per-mesh flag words land here
+    size_t host_flags_words = 0;
     // last build's sorted keys (for bvh_debug_sorted_keys)
     uint64_t* last_keys = nullptr;
     uint32_t* last_vals = nullptr;
@@ -382,8 +384,17 @@ HierArgs hier_args(Workspace& w, const Job& j, const uint64_t* keys, const uint3
 
 // Non-finite input and internal-error words of every segment.
 int check_input_flags(bvh_ctx_t ctx, Workspace& w, uint32_t k) {
-    std::vector<uint32_t> sc((size_t)k * 8);
-    HIP_TRY(ctx, hipMemcpyAsync(sc.data(), w.scene.p, sc.size() * 4, hipMemcpyDeviceToHost, ctx->stream));
+    // Pinned destination: a pageable one would add a staged copy to every build.
+    const size_t words = (size_t)k * 8;
+    if (ctx->host_flags_words < words) {
+        if (ctx->host_flags) HIP_TRY(ctx, hipHostFree(ctx->host_flags));
+        ctx->host_flags = nullptr;
+        ctx->host_flags_words = 0;
+        HIP_TRY(ctx, hipHostMalloc((void**)&ctx->host_flags, words * 4, hipHostMallocDefault));
+        ctx->host_flags_words = words;
+    }
+    const uint32_t* sc = ctx->host_flags;
+    HIP_TRY(ctx, hipMemcpyAsync(ctx->host_flags, w.scene.p, words * 4, hipMemcpyDeviceToHost, ctx->stream));
     HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
     for (uint32_t s = 0; s < k; ++s) {
         if (sc[8 * s + 6]) {
@@ -973,6 +984,7 @@ int bvh_ctx_destroy(bvh_ctx_t ctx) {
     free_buf(ctx->ray_keys[0]);
     free_buf(ctx->ray_keys[1]);
     free_buf(ctx->trace_io);
+    if (ctx->host_flags) (void)hipHostFree(ctx->host_flags);
     for (auto& e : ctx->ev)
         if (e) (void)hipEventDestroy(e);
     (void)hipStreamDestroy(ctx->stream);

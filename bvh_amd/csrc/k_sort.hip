@@ -12,10 +12,9 @@
 //     (no separate reduce or scan launch);
 //   * each onesweep tile (512 threads x 16 keys) takes a tile id from an
 //     atomic counter (ids follow start order, so every look-back target is
-//     already resident), counts its digits and publishes them at once (so
-//     later tiles' look-backs pass it while it ranks), ranks keys with wave64
-//     ballot matching into per-wave LDS histograms (stable: items are
-//     consumed in input order), resolves its global digit offsets by
+//     already resident), ranks keys with wave64 ballot matching into per-wave
+//     LDS histograms (stable: items are consumed in input order), publishes
+//     its per-digit counts, resolves its global digit offsets by
 //     decoupled look-back over agent-scope status words (flag | count in one
 //     32-bit word, so no payload fence is needed), reorders the tile in LDS
 //     and writes each digit run contiguously.
@@ -26,9 +25,6 @@
 namespace bvhk {
 
 constexpr int kSortBlock = kSortThreads;
-// Count and publish a tile's digits before ranking it (LDS atomics per key)
-// rather than after (from the ranking's per-wave histograms).
-constexpr bool kSortEarlyCounts = false;
 constexpr uint32_t kFlagA = 1u << 30;
 constexpr uint32_t kFlagP = 2u << 30;
 constexpr uint32_t kValMask = (1u << 30) - 1;
@@ -130,7 +126,6 @@ __global__ __launch_bounds__(kSortBlock) void k_onesweep(const uint64_t* __restr
     // Bins per thread; with more threads than bins the first BINS threads own one each.
     constexpr int BPT = (BINS + kSortBlock - 1) / kSortBlock;
     __shared__ uint32_t s_whist[WAVES * BINS];
-    __shared__ uint32_t s_thist[BINS];
     __shared__ uint32_t s_local[BINS];
     __shared__ uint32_t s_global[BINS];
     __shared__ uint64_t s_keys[TILE];
@@ -144,7 +139,6 @@ __global__ __launch_bounds__(kSortBlock) void k_onesweep(const uint64_t* __restr
     const bool owner = tid * BPT < BINS;
     if (tid == 0) s_tile = atomicAdd(tile_ctr, 1u);
     for (int i = tid; i < WAVES * BINS; i += kSortBlock) s_whist[i] = 0;
-    for (int i = tid; i < BINS; i += kSortBlock) s_thist[i] = 0;
     __syncthreads();
     const uint32_t tile = s_tile;
     if (tile >= gridDim.x) return;  // tile counter not cleared: publish nothing, touch nothing
@@ -161,41 +155,20 @@ __global__ __launch_bounds__(kSortBlock) void k_onesweep(const uint64_t* __restr
         key[it] = valid ? kin[idx] : ~0ull;
         if (VALS) val[it] = valid ? vin[idx] : 0u;
     }
-    // The tile's digit counts first, published before the ranking so that
-    // later tiles' look-backs can pass this tile while it ranks.
-    if (kSortEarlyCounts) {
-#pragma unroll
-        for (int it = 0; it < kSortItems; ++it) {
-            uint32_t idx = seg + it * kWave + lane;
-            if (idx < n) atomicAdd(&s_thist[(uint32_t)(key[it] >> shift) & MASK], 1u);
-        }
-    }
     // Global start of this thread's bins: exclusive scan of the pass's digit
-    // totals.
+    // totals (their loads overlap the key loads above).
     uint32_t tot[BPT];
     uint32_t cnt[BPT];
     uint32_t tot_sum = 0, tsum = 0;
 #pragma unroll
-    for (int j = 0; j < BPT; ++j) cnt[j] = 0;
-    __syncthreads();
-#pragma unroll
     for (int j = 0; j < BPT; ++j) {
         const int b = tid * BPT + j;
+        cnt[j] = 0;
         tot[j] = owner ? hist_tot[b] : 0u;
         tot_sum += tot[j];
-        if (kSortEarlyCounts) {
-            cnt[j] = owner ? s_thist[b] : 0u;
-            tsum += cnt[j];
-            if (owner)
-                __hip_atomic_store(status + (size_t)tile * BINS + b, (tile == 0 ? kFlagP : kFlagA) | cnt[j],
-                                   __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        }
     }
     uint32_t all_keys;
     uint32_t gstart = block_excl_sum<kSortBlock>(tot_sum, s_w, &all_keys);
-    uint32_t total;
-    uint32_t lrun = 0;
-    if (kSortEarlyCounts) lrun = block_excl_sum<kSortBlock>(tsum, s_w, &total);
 
     // Stable ranking inside each wave (items consumed in input order) with
     // wave64 ballot matching into per-wave LDS histograms.
@@ -223,8 +196,8 @@ __global__ __launch_bounds__(kSortBlock) void k_onesweep(const uint64_t* __restr
         rk[it] = r;
     }
     __syncthreads();
-    // Per bin: exclusive prefix over the waves (in place); without early
-    // counts the tile's counts are published here.
+    // Per bin: exclusive prefix over the waves (in place), and the tile's
+    // counts published for later tiles' look-backs.
 #pragma unroll
     for (int j = 0; j < BPT; ++j) {
         const int b = tid * BPT + j;
@@ -236,14 +209,13 @@ __global__ __launch_bounds__(kSortBlock) void k_onesweep(const uint64_t* __restr
             s_whist[w * BINS + b] = run;
             run += c;
         }
-        if (!kSortEarlyCounts) {
-            cnt[j] = run;
-            tsum += run;
-            __hip_atomic_store(status + (size_t)tile * BINS + b, (tile == 0 ? kFlagP : kFlagA) | run,
-                               __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        }
+        cnt[j] = run;
+        tsum += run;
+        __hip_atomic_store(status + (size_t)tile * BINS + b, (tile == 0 ? kFlagP : kFlagA) | run, __ATOMIC_RELAXED,
+                           __HIP_MEMORY_SCOPE_AGENT);
     }
-    if (!kSortEarlyCounts) lrun = block_excl_sum<kSortBlock>(tsum, s_w, &total);
+    uint32_t total;
+    uint32_t lrun = block_excl_sum<kSortBlock>(tsum, s_w, &total);
     // Decoupled look-back: the thread's BPT bins walk back together, so their
     // status loads overlap.
     uint32_t g[BPT];

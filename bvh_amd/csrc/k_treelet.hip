@@ -44,6 +44,9 @@ constexpr int kTItemsMax = 8;  // leaf chains per thread (2, 4 or 8: a chunk of 
 constexpr int kTWaves = kTBlock / kWave;
 constexpr int kTMax = 7;
 constexpr int kGroups = 8;               // treelets per wave per round (groups of >= 8 lanes)
+// dp_group initialises a treelet's single-leaf costs one lane per leaf, so a
+// group needs at least kTMax lanes.
+static_assert(kWave / kGroups >= kTMax, "a treelet group needs a lane per leaf");
 constexpr float kCostInf = 1e30f;
 
 // One treelet in LDS: formed by its owner lane, optimised by one 8-lane

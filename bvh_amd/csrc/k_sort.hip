@@ -12,12 +12,13 @@
 //     (no separate reduce or scan launch);
 //   * each onesweep tile (512 threads x 16 keys) takes a tile id from an
 //     atomic counter (ids follow start order, so every look-back target is
-//     already resident), ranks keys with wave64 ballot matching into per-wave
-//     LDS histograms (stable: items are consumed in input order), publishes
-//     its per-digit counts, resolves its global digit offsets by
-//     decoupled look-back over agent-scope status words (flag | count in one
-//     32-bit word, so no payload fence is needed), reorders the tile in LDS
-//     and writes each digit run contiguously.
+//     already resident), ranks keys into per-wave LDS histograms (stable:
+//     items are consumed in input order; lanes sharing a digit meet in a
+//     64-bit LDS match mask, or by wave64 ballots in the top pass), publishes
+//     its per-digit counts, resolves its global digit offsets by decoupled
+//     look-back over agent-scope status words (flag | count in one 32-bit
+//     word, so no payload fence is needed; eight predecessors per step),
+//     reorders the tile in LDS and writes each digit run contiguously.
 // Key traffic per pass: 8 B read + 8 B write per key (+4 + 4 with payload).
 #include "dev_common.hpp"
 #include "engine.hpp"
@@ -28,6 +29,8 @@ constexpr int kSortBlock = kSortThreads;
 constexpr uint32_t kFlagA = 1u << 30;
 constexpr uint32_t kFlagP = 2u << 30;
 constexpr uint32_t kValMask = (1u << 30) - 1;
+// Predecessor status words read per look-back step.
+constexpr int kLookWindow = 8;
 
 __global__ void k_fill_zero(uint4* p, size_t n4) {
     size_t stride = (size_t)gridDim.x * blockDim.x;
@@ -114,7 +117,7 @@ __device__ __forceinline__ uint32_t block_excl_sum(uint32_t v, uint32_t* s_w, ui
     return before + incl - v;
 }
 
-template <int BITS, bool VALS>
+template <int BITS, bool VALS, bool LDS_MATCH>
 __global__ __launch_bounds__(kSortBlock) void k_onesweep(const uint64_t* __restrict__ kin, uint64_t* __restrict__ kout,
                                                         const uint32_t* __restrict__ vin, uint32_t* __restrict__ vout,
                                                         uint32_t n, int shift, uint32_t MASK,
@@ -133,12 +136,20 @@ __global__ __launch_bounds__(kSortBlock) void k_onesweep(const uint64_t* __restr
     __shared__ uint32_t s_tile;
     __shared__ uint32_t s_w[WAVES];
 
+    // Per-wave digit match masks for the ranking, in the key staging array
+    // (unused until the reorder).
+    static_assert(WAVES * BINS <= TILE, "match masks must fit the key staging array");
+    uint64_t* const s_match = s_keys;
+
     const int tid = threadIdx.x;
     const int wave = tid / kWave;
     const int lane = tid % kWave;
     const bool owner = tid * BPT < BINS;
     if (tid == 0) s_tile = atomicAdd(tile_ctr, 1u);
-    for (int i = tid; i < WAVES * BINS; i += kSortBlock) s_whist[i] = 0;
+    for (int i = tid; i < WAVES * BINS; i += kSortBlock) {
+        s_whist[i] = 0;
+        s_match[i] = 0;
+    }
     __syncthreads();
     const uint32_t tile = s_tile;
     if (tile >= gridDim.x) return;  // tile counter not cleared: publish nothing, touch nothing
@@ -170,28 +181,57 @@ __global__ __launch_bounds__(kSortBlock) void k_onesweep(const uint64_t* __restr
     uint32_t all_keys;
     uint32_t gstart = block_excl_sum<kSortBlock>(tot_sum, s_w, &all_keys);
 
-    // Stable ranking inside each wave (items consumed in input order) with
-    // wave64 ballot matching into per-wave LDS histograms.
+    // Stable ranking inside each wave (items consumed in input order) into
+    // per-wave LDS histograms; the lanes holding one digit find each other
+    // in one of two ways:
+    //  * LDS_MATCH: through the digit's 64-bit LDS match mask: every lane ORs
+    //    in its bit, then reads the mask back.  A wave's LDS instructions
+    //    complete in order, so the read sees all of this item's ORs and none
+    //    of the next item's; the digit's lowest lane clears the mask after
+    //    the read.  One atomic and one read per item, but lanes sharing a
+    //    digit serialise on its address, so this suits well-mixed digits;
+    //  * otherwise one ballot per digit bit present in MASK, at a cost that
+    //    does not depend on the digit distribution (the top pass, whose
+    //    digits are few and skewed).
     uint32_t* wh = s_whist + wave * BINS;
+    uint64_t* wm = s_match + wave * BINS;
     const uint64_t lt_mask = lanemask_lt();
+    const uint64_t my_bit = 1ull << lane;
 #pragma unroll
     for (int it = 0; it < kSortItems; ++it) {
         uint32_t idx = seg + it * kWave + lane;
         bool valid = idx < n;
         uint32_t d = (uint32_t)(key[it] >> shift) & MASK;
-        uint64_t peers = __ballot(valid);
-#pragma unroll
-        for (int b = 0; b < BITS; ++b) {
-            bool bit = (d >> b) & 1u;
-            uint64_t bb = __ballot(bit);
-            peers &= bit ? bb : ~bb;
-        }
         uint32_t r = 0;
-        if (valid) {
-            uint64_t lt = peers & lt_mask;
-            uint32_t old = wh[d];
-            r = old + (uint32_t)__popcll(lt);
-            if (lt == 0) wh[d] = old + (uint32_t)__popcll(peers);
+        if (LDS_MATCH) {
+            if (valid) __hip_atomic_fetch_or(wm + d, my_bit, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+            __builtin_amdgcn_wave_barrier();
+            if (valid) {
+                const uint64_t peers = __hip_atomic_load(wm + d, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+                const uint64_t lt = peers & lt_mask;
+                const uint32_t old = wh[d];
+                r = old + (uint32_t)__popcll(lt);
+                if (lt == 0) {
+                    wh[d] = old + (uint32_t)__popcll(peers);
+                    __hip_atomic_store(wm + d, 0ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+                }
+            }
+            __builtin_amdgcn_wave_barrier();
+        } else {
+            uint64_t peers = __ballot(valid);
+#pragma unroll
+            for (int b = 0; b < BITS; ++b) {
+                if (!((MASK >> b) & 1u)) continue;
+                bool bit = (d >> b) & 1u;
+                uint64_t bb = __ballot(bit);
+                peers &= bit ? bb : ~bb;
+            }
+            if (valid) {
+                uint64_t lt = peers & lt_mask;
+                uint32_t old = wh[d];
+                r = old + (uint32_t)__popcll(lt);
+                if (lt == 0) wh[d] = old + (uint32_t)__popcll(peers);
+            }
         }
         rk[it] = r;
     }
@@ -216,8 +256,12 @@ __global__ __launch_bounds__(kSortBlock) void k_onesweep(const uint64_t* __restr
     }
     uint32_t total;
     uint32_t lrun = block_excl_sum<kSortBlock>(tsum, s_w, &total);
-    // Decoupled look-back: the thread's BPT bins walk back together, so their
-    // status loads overlap.
+    // Decoupled look-back: each step reads the status words of the next
+    // kLookWindow predecessors of all the thread's BPT bins at once (their
+    // loads overlap), then consumes them nearest first: aggregates are added
+    // and the walk goes on, an inclusive prefix ends it, an unpublished word
+    // stalls it there.  (Tile 0 always publishes an inclusive prefix, so no
+    // window reaches below it.)
     uint32_t g[BPT];
     int tj[BPT];
     uint32_t pending = 0;
@@ -228,24 +272,37 @@ __global__ __launch_bounds__(kSortBlock) void k_onesweep(const uint64_t* __restr
         if (tile > 0 && owner) pending |= 1u << j;
     }
     while (pending) {
-        uint32_t w[BPT];
+        uint32_t w[BPT][kLookWindow];
 #pragma unroll
         for (int j = 0; j < BPT; ++j)
-            w[j] = ((pending >> j) & 1u) ? __hip_atomic_load(status + (size_t)tj[j] * BINS + tid * BPT + j,
-                                                              __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)
-                                          : 0u;
+#pragma unroll
+            for (int q = 0; q < kLookWindow; ++q)
+                w[j][q] = (((pending >> j) & 1u) && tj[j] - q >= 0)
+                              ? __hip_atomic_load(status + (size_t)(tj[j] - q) * BINS + tid * BPT + j,
+                                                  __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)
+                              : 0u;
         bool stalled = false;
 #pragma unroll
         for (int j = 0; j < BPT; ++j) {
             if (!((pending >> j) & 1u)) continue;
-            const uint32_t flag = w[j] & ~kValMask;
-            if (flag == 0) {
-                stalled = true;
-                continue;
+            bool go = true;
+#pragma unroll
+            for (int q = 0; q < kLookWindow; ++q) {
+                if (!go) continue;
+                const uint32_t flag = w[j][q] & ~kValMask;
+                if (flag == 0) {
+                    stalled = true;
+                    go = false;
+                    continue;
+                }
+                g[j] += w[j][q] & kValMask;
+                if (flag == kFlagP) {
+                    pending &= ~(1u << j);
+                    go = false;
+                } else {
+                    --tj[j];
+                }
             }
-            g[j] += w[j] & kValMask;
-            if (flag == kFlagP) pending &= ~(1u << j);
-            else --tj[j];
         }
         if (stalled) __builtin_amdgcn_s_sleep(1);
     }
@@ -287,16 +344,16 @@ __global__ __launch_bounds__(kSortBlock) void k_onesweep(const uint64_t* __restr
     }
 }
 
-template <int BITS>
+template <int BITS, bool LDS_MATCH>
 static void onesweep_pass(bool vals, const uint64_t* kin, uint64_t* kout, const uint32_t* vin, uint32_t* vout,
                           uint32_t n, int shift, uint32_t mask, uint32_t tiles, const uint32_t* hist_tot, uint32_t* status,
                           uint32_t* tile_ctr, hipStream_t s) {
     if (vals)
-        hipLaunchKernelGGL((k_onesweep<BITS, true>), dim3(tiles), dim3(kSortBlock), 0, s, kin, kout, vin, vout, n,
-                           shift, mask, hist_tot, status, tile_ctr);
+        hipLaunchKernelGGL((k_onesweep<BITS, true, LDS_MATCH>), dim3(tiles), dim3(kSortBlock), 0, s, kin, kout, vin,
+                           vout, n, shift, mask, hist_tot, status, tile_ctr);
     else
-        hipLaunchKernelGGL((k_onesweep<BITS, false>), dim3(tiles), dim3(kSortBlock), 0, s, kin, kout, vin, vout, n,
-                           shift, mask, hist_tot, status, tile_ctr);
+        hipLaunchKernelGGL((k_onesweep<BITS, false, LDS_MATCH>), dim3(tiles), dim3(kSortBlock), 0, s, kin, kout, vin,
+                           vout, n, shift, mask, hist_tot, status, tile_ctr);
 }
 
 void launch_onesweep_pass(const SortPlan& plan, int p, uint32_t n, const uint64_t* kin, uint64_t* kout,
@@ -306,10 +363,19 @@ void launch_onesweep_pass(const SortPlan& plan, int p, uint32_t n, const uint64_
     const bool vals = vin != nullptr;
     uint32_t* st = lookback + (size_t)p * plan.tiles * bins;
     uint32_t* ctr = lookback + (size_t)plan.passes * plan.tiles * bins + p;
-    if (plan.bits == 8)
-        onesweep_pass<8>(vals, kin, kout, vin, vout, n, plan.shift[p], plan.mask[p], plan.tiles, hist_tot + p * bins, st, ctr, s);
-    else
-        onesweep_pass<10>(vals, kin, kout, vin, vout, n, plan.shift[p], plan.mask[p], plan.tiles, hist_tot + p * bins, st, ctr, s);
+    // LDS matching for the lower digits (well mixed after the earlier passes);
+    // ballots for the top digit, where a few hot values are typical.
+    const bool lds_match = p + 1 < plan.passes;
+    const int sh = plan.shift[p];
+    const uint32_t mk = plan.mask[p];
+    const uint32_t* ht = hist_tot + p * bins;
+    if (plan.bits == 8) {
+        if (lds_match) onesweep_pass<8, true>(vals, kin, kout, vin, vout, n, sh, mk, plan.tiles, ht, st, ctr, s);
+        else onesweep_pass<8, false>(vals, kin, kout, vin, vout, n, sh, mk, plan.tiles, ht, st, ctr, s);
+    } else {
+        if (lds_match) onesweep_pass<10, true>(vals, kin, kout, vin, vout, n, sh, mk, plan.tiles, ht, st, ctr, s);
+        else onesweep_pass<10, false>(vals, kin, kout, vin, vout, n, sh, mk, plan.tiles, ht, st, ctr, s);
+    }
 }
 
 void launch_onesweep(const SortPlan& plan, uint32_t n, uint64_t* keys_a, uint64_t* keys_b, uint32_t* vals_a,

@@ -294,7 +294,7 @@ Job single_job(const bvh_mesh_rec* mesh, bvh_tree_rec* tree, bool m60) {
 }
 
 SortPlan plan_for(const Job& j, int radix_bits) {
-    return make_sort_plan(j.total, j.begin_bit, j.end_bit, pick_bits(radix_bits), (int)j.fsb.nblocks);
+    return make_sort_plan(j.total, j.begin_bit, j.end_bit, pick_bits(radix_bits), (int)j.fsb.nblocks, j.m60);
 }
 
 size_t zero_bytes_of(const SortPlan& plan) { return (sort_zero_bytes(plan) + 15) & ~(size_t)15; }
@@ -346,8 +346,9 @@ int enqueue_front(bvh_ctx_t ctx, Workspace& w, const Job& j, const SortPlan& pla
     uint32_t* vin = j.m60 ? P<uint32_t>(w.vals_a) : nullptr;
     uint32_t* vout = j.m60 ? P<uint32_t>(w.vals_b) : nullptr;
     for (int p = 0; p < plan.passes; ++p) {
-        launch_onesweep_pass(plan, p, j.total, kin, kout, vin, vout, P<uint32_t>(w.hist_tot),
-                             P<uint32_t>(w.lookback), s);
+        if (!launch_onesweep_pass(plan, p, j.total, kin, kout, vin, vout, P<uint32_t>(w.hist_tot),
+                                  P<uint32_t>(w.lookback), s))
+            return set_err(ctx, BVH_E_ARG, "sort plan made for keys only");
         TRY(check_launch(ctx, "k_onesweep"));
         TRY(mark(clk, pass_names[p]));
         std::swap(kin, kout);
@@ -1319,8 +1320,9 @@ int bvh_trace(bvh_ctx_t ctx, bvh_tree_t tree, bvh_mesh_t mesh, const float* d_ra
         launch_fill_zero(P<uint32_t>(w.lookback), zero_bytes_of(plan) / 4, ctx->stream);
         uint64_t* kout = nullptr;
         uint32_t* vout = nullptr;
-        launch_onesweep(plan, nrays, P<uint64_t>(ctx->ray_keys[0]), P<uint64_t>(ctx->ray_keys[1]), nullptr, nullptr,
-                        P<uint32_t>(w.hist_tot), P<uint32_t>(w.lookback), &kout, &vout, ctx->stream);
+        if (!launch_onesweep(plan, nrays, P<uint64_t>(ctx->ray_keys[0]), P<uint64_t>(ctx->ray_keys[1]), nullptr,
+                             nullptr, P<uint32_t>(w.hist_tot), P<uint32_t>(w.lookback), &kout, &vout, ctx->stream))
+            return set_err(ctx, BVH_E_ARG, "sort plan made for keys only");
         TRY(check_launch(ctx, "ray ordering"));
         a.order = kout;
         a.order_mask = (1ull << idx_bits) - 1;
@@ -1392,7 +1394,7 @@ int bvh_sort_pairs_u64(bvh_ctx_t ctx, uint64_t* d_keys, uint32_t* d_vals, uint32
     if (begin_bit < 0 || end_bit > 64 || begin_bit >= end_bit) return set_err(ctx, BVH_E_ARG, "bad bit range");
     if (radix_bits != 0 && radix_bits != 8 && radix_bits != 10)
         return set_err(ctx, BVH_E_UNSUPPORTED, "radix_bits must be 0, 8 or 10");
-    SortPlan plan = make_sort_plan(n, begin_bit, end_bit, pick_bits(radix_bits));
+    SortPlan plan = make_sort_plan(n, begin_bit, end_bit, pick_bits(radix_bits), 0, d_vals != nullptr);
     if (plan.passes > 8) return set_err(ctx, BVH_E_ARG, "too many passes");
     const size_t bins = 1u << plan.bits;
     const size_t zero_bytes = zero_bytes_of(plan);
@@ -1407,8 +1409,9 @@ int bvh_sort_pairs_u64(bvh_ctx_t ctx, uint64_t* d_keys, uint32_t* d_vals, uint32
     launch_fill_zero(P<uint32_t>(w.lookback), zero_bytes / 4, ctx->stream);
     uint64_t* kout = nullptr;
     uint32_t* vout = nullptr;
-    launch_onesweep(plan, n, d_keys, P<uint64_t>(w.keys_b), d_vals, d_vals ? P<uint32_t>(w.vals_b) : nullptr,
-                    P<uint32_t>(w.hist_tot), P<uint32_t>(w.lookback), &kout, &vout, ctx->stream);
+    if (!launch_onesweep(plan, n, d_keys, P<uint64_t>(w.keys_b), d_vals, d_vals ? P<uint32_t>(w.vals_b) : nullptr,
+                         P<uint32_t>(w.hist_tot), P<uint32_t>(w.lookback), &kout, &vout, ctx->stream))
+        return set_err(ctx, BVH_E_ARG, "sort plan made for keys only");
     TRY(check_launch(ctx, "k_onesweep"));
     HIP_TRY(ctx, hipEventRecord(ctx->ev[1], ctx->stream));
     if (kout != d_keys) {

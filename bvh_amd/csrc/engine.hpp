@@ -55,9 +55,16 @@ void launch_init_build(uint32_t* seg_scene, uint32_t k, uint32_t* epoch, hipStre
 // Per-triangle boxes (2 x float4 per triangle) and each segment's scene box.
 void launch_tri_bounds(const SegBlocks& sb, float4* tri_boxes, uint32_t* seg_scene, hipStream_t s);
 
-// Onesweep tile: threads x keys per thread.
+// Onesweep tile: threads x keys per thread.  Key-only passes with 8-bit
+// digits over at least kSortWideMinKeys keys use the wide tile (twice the
+// keys, so half the tiles: half the tile tickets, which serialise on one
+// counter, and a shorter look-back chain; 146 KB of LDS).  Below that the
+// narrow tiles keep more CUs busy (measured crossover near 300 narrow tiles
+// on 256 CUs).
 constexpr int kSortThreads = 512;
+constexpr int kSortThreadsWide = 1024;
 constexpr int kSortItems = 16;
+constexpr uint32_t kSortWideMinKeys = 2500000;
 
 // Sort plan: digit width, pass count and shifts for a key bit range.
 struct SortPlan {
@@ -67,10 +74,12 @@ struct SortPlan {
     uint32_t mask[8];  // digit mask of each pass (the last pass may be narrower)
     uint32_t tiles;    // onesweep tiles
     uint32_t tile;     // keys per tile
+    int block;         // threads per tile (kSortThreads or kSortThreadsWide)
     int hist_blocks;   // blocks of the histogram producer
 };
-// hist_blocks <= 0 picks a default producer size.
-SortPlan make_sort_plan(uint32_t n, int begin_bit, int end_bit, int bits, int hist_blocks = 0);
+// hist_blocks <= 0 picks a default producer size; vals: the sort carries a
+// 32-bit payload (narrow tiles).
+SortPlan make_sort_plan(uint32_t n, int begin_bit, int end_bit, int bits, int hist_blocks = 0, bool vals = false);
 // Bytes of the zero-initialised lookback area a plan needs (status words + tile counters).
 size_t sort_zero_bytes(const SortPlan& p);
 
@@ -93,11 +102,13 @@ void launch_sort_histogram(const uint64_t* keys, uint32_t n, const SortPlan& pla
 void launch_fill_zero(uint32_t* p, size_t words, hipStream_t s);
 void launch_stream_copy(const void* src, void* dst, size_t bytes, hipStream_t s);
 // All onesweep passes.  Result ends in (*keys_out, *vals_out) which is one of
-// the two buffers passed in.
-void launch_onesweep_pass(const SortPlan& plan, int pass, uint32_t n, const uint64_t* kin, uint64_t* kout,
+// the two buffers passed in.  Both return false, launching nothing, when a
+// payload is passed with a plan made for keys only (its tiles are too large
+// for a payload's LDS staging).
+bool launch_onesweep_pass(const SortPlan& plan, int pass, uint32_t n, const uint64_t* kin, uint64_t* kout,
                           const uint32_t* vin, uint32_t* vout, const uint32_t* hist_tot, uint32_t* lookback,
                           hipStream_t s);
-void launch_onesweep(const SortPlan& plan, uint32_t n, uint64_t* keys_a, uint64_t* keys_b, uint32_t* vals_a,
+bool launch_onesweep(const SortPlan& plan, uint32_t n, uint64_t* keys_a, uint64_t* keys_b, uint32_t* vals_a,
                      uint32_t* vals_b, const uint32_t* hist_tot, uint32_t* lookback, uint64_t** keys_out,
                      uint32_t** vals_out, hipStream_t s);
 

@@ -25,7 +25,6 @@
 
 namespace bvhk {
 
-constexpr int kSortBlock = kSortThreads;
 constexpr uint32_t kFlagA = 1u << 30;
 constexpr uint32_t kFlagP = 2u << 30;
 constexpr uint32_t kValMask = (1u << 30) - 1;
@@ -117,17 +116,17 @@ __device__ __forceinline__ uint32_t block_excl_sum(uint32_t v, uint32_t* s_w, ui
     return before + incl - v;
 }
 
-template <int BITS, bool VALS, bool LDS_MATCH>
-__global__ __launch_bounds__(kSortBlock) void k_onesweep(const uint64_t* __restrict__ kin, uint64_t* __restrict__ kout,
+template <int BLOCK, int BITS, bool VALS, bool LDS_MATCH>
+__global__ __launch_bounds__(BLOCK) void k_onesweep(const uint64_t* __restrict__ kin, uint64_t* __restrict__ kout,
                                                         const uint32_t* __restrict__ vin, uint32_t* __restrict__ vout,
                                                         uint32_t n, int shift, uint32_t MASK,
                                                         const uint32_t* __restrict__ hist_tot, uint32_t* status,
                                                         uint32_t* tile_ctr) {
     constexpr int BINS = 1 << BITS;
-    constexpr int WAVES = kSortBlock / kWave;
-    constexpr int TILE = kSortBlock * kSortItems;
+    constexpr int WAVES = BLOCK / kWave;
+    constexpr int TILE = BLOCK * kSortItems;
     // Bins per thread; with more threads than bins the first BINS threads own one each.
-    constexpr int BPT = (BINS + kSortBlock - 1) / kSortBlock;
+    constexpr int BPT = (BINS + BLOCK - 1) / BLOCK;
     __shared__ uint32_t s_whist[WAVES * BINS];
     __shared__ uint32_t s_local[BINS];
     __shared__ uint32_t s_global[BINS];
@@ -146,7 +145,7 @@ __global__ __launch_bounds__(kSortBlock) void k_onesweep(const uint64_t* __restr
     const int lane = tid % kWave;
     const bool owner = tid * BPT < BINS;
     if (tid == 0) s_tile = atomicAdd(tile_ctr, 1u);
-    for (int i = tid; i < WAVES * BINS; i += kSortBlock) {
+    for (int i = tid; i < WAVES * BINS; i += BLOCK) {
         s_whist[i] = 0;
         s_match[i] = 0;
     }
@@ -179,7 +178,7 @@ __global__ __launch_bounds__(kSortBlock) void k_onesweep(const uint64_t* __restr
         tot_sum += tot[j];
     }
     uint32_t all_keys;
-    uint32_t gstart = block_excl_sum<kSortBlock>(tot_sum, s_w, &all_keys);
+    uint32_t gstart = block_excl_sum<BLOCK>(tot_sum, s_w, &all_keys);
 
     // Stable ranking inside each wave (items consumed in input order) into
     // per-wave LDS histograms; the lanes holding one digit find each other
@@ -255,7 +254,7 @@ __global__ __launch_bounds__(kSortBlock) void k_onesweep(const uint64_t* __restr
                            __HIP_MEMORY_SCOPE_AGENT);
     }
     uint32_t total;
-    uint32_t lrun = block_excl_sum<kSortBlock>(tsum, s_w, &total);
+    uint32_t lrun = block_excl_sum<BLOCK>(tsum, s_w, &total);
     // Decoupled look-back: each step reads the status words of the next
     // kLookWindow predecessors of all the thread's BPT bins at once (their
     // loads overlap), then consumes them nearest first: aggregates are added
@@ -333,7 +332,7 @@ __global__ __launch_bounds__(kSortBlock) void k_onesweep(const uint64_t* __restr
     __syncthreads();
 
     const uint32_t tile_n = (n - base < (uint32_t)TILE) ? (n - base) : (uint32_t)TILE;
-    for (uint32_t j = tid; j < tile_n; j += kSortBlock) {
+    for (uint32_t j = tid; j < tile_n; j += BLOCK) {
         uint64_t k = s_keys[j];
         uint32_t d = (uint32_t)(k >> shift) & MASK;
         uint32_t o = s_global[d] + j;
@@ -345,22 +344,26 @@ __global__ __launch_bounds__(kSortBlock) void k_onesweep(const uint64_t* __restr
 }
 
 template <int BITS, bool LDS_MATCH>
-static void onesweep_pass(bool vals, const uint64_t* kin, uint64_t* kout, const uint32_t* vin, uint32_t* vout,
-                          uint32_t n, int shift, uint32_t mask, uint32_t tiles, const uint32_t* hist_tot, uint32_t* status,
-                          uint32_t* tile_ctr, hipStream_t s) {
+static void onesweep_pass(int block, bool vals, const uint64_t* kin, uint64_t* kout, const uint32_t* vin,
+                          uint32_t* vout, uint32_t n, int shift, uint32_t mask, uint32_t tiles, const uint32_t* hist_tot,
+                          uint32_t* status, uint32_t* tile_ctr, hipStream_t s) {
     if (vals)
-        hipLaunchKernelGGL((k_onesweep<BITS, true, LDS_MATCH>), dim3(tiles), dim3(kSortBlock), 0, s, kin, kout, vin,
-                           vout, n, shift, mask, hist_tot, status, tile_ctr);
+        hipLaunchKernelGGL((k_onesweep<kSortThreads, BITS, true, LDS_MATCH>), dim3(tiles), dim3(kSortThreads), 0, s,
+                           kin, kout, vin, vout, n, shift, mask, hist_tot, status, tile_ctr);
+    else if (BITS == 8 && block == kSortThreadsWide)
+        hipLaunchKernelGGL((k_onesweep<kSortThreadsWide, 8, false, LDS_MATCH>), dim3(tiles), dim3(kSortThreadsWide), 0,
+                           s, kin, kout, vin, vout, n, shift, mask, hist_tot, status, tile_ctr);
     else
-        hipLaunchKernelGGL((k_onesweep<BITS, false, LDS_MATCH>), dim3(tiles), dim3(kSortBlock), 0, s, kin, kout, vin,
-                           vout, n, shift, mask, hist_tot, status, tile_ctr);
+        hipLaunchKernelGGL((k_onesweep<kSortThreads, BITS, false, LDS_MATCH>), dim3(tiles), dim3(kSortThreads), 0, s,
+                           kin, kout, vin, vout, n, shift, mask, hist_tot, status, tile_ctr);
 }
 
-void launch_onesweep_pass(const SortPlan& plan, int p, uint32_t n, const uint64_t* kin, uint64_t* kout,
+bool launch_onesweep_pass(const SortPlan& plan, int p, uint32_t n, const uint64_t* kin, uint64_t* kout,
                           const uint32_t* vin, uint32_t* vout, const uint32_t* hist_tot, uint32_t* lookback,
                           hipStream_t s) {
     const uint32_t bins = 1u << plan.bits;
     const bool vals = vin != nullptr;
+    if (vals && plan.block != kSortThreads) return false;
     uint32_t* st = lookback + (size_t)p * plan.tiles * bins;
     uint32_t* ctr = lookback + (size_t)plan.passes * plan.tiles * bins + p;
     // LDS matching for the lower digits (well mixed after the earlier passes);
@@ -370,15 +373,16 @@ void launch_onesweep_pass(const SortPlan& plan, int p, uint32_t n, const uint64_
     const uint32_t mk = plan.mask[p];
     const uint32_t* ht = hist_tot + p * bins;
     if (plan.bits == 8) {
-        if (lds_match) onesweep_pass<8, true>(vals, kin, kout, vin, vout, n, sh, mk, plan.tiles, ht, st, ctr, s);
-        else onesweep_pass<8, false>(vals, kin, kout, vin, vout, n, sh, mk, plan.tiles, ht, st, ctr, s);
+        if (lds_match) onesweep_pass<8, true>(plan.block, vals, kin, kout, vin, vout, n, sh, mk, plan.tiles, ht, st, ctr, s);
+        else onesweep_pass<8, false>(plan.block, vals, kin, kout, vin, vout, n, sh, mk, plan.tiles, ht, st, ctr, s);
     } else {
-        if (lds_match) onesweep_pass<10, true>(vals, kin, kout, vin, vout, n, sh, mk, plan.tiles, ht, st, ctr, s);
-        else onesweep_pass<10, false>(vals, kin, kout, vin, vout, n, sh, mk, plan.tiles, ht, st, ctr, s);
+        if (lds_match) onesweep_pass<10, true>(plan.block, vals, kin, kout, vin, vout, n, sh, mk, plan.tiles, ht, st, ctr, s);
+        else onesweep_pass<10, false>(plan.block, vals, kin, kout, vin, vout, n, sh, mk, plan.tiles, ht, st, ctr, s);
     }
+    return true;
 }
 
-void launch_onesweep(const SortPlan& plan, uint32_t n, uint64_t* keys_a, uint64_t* keys_b, uint32_t* vals_a,
+bool launch_onesweep(const SortPlan& plan, uint32_t n, uint64_t* keys_a, uint64_t* keys_b, uint32_t* vals_a,
                      uint32_t* vals_b, const uint32_t* hist_tot, uint32_t* lookback, uint64_t** keys_out,
                      uint32_t** vals_out, hipStream_t s) {
     uint64_t* kin = keys_a;
@@ -386,12 +390,13 @@ void launch_onesweep(const SortPlan& plan, uint32_t n, uint64_t* keys_a, uint64_
     uint32_t* vin = vals_a;
     uint32_t* vout = vals_b;
     for (int p = 0; p < plan.passes; ++p) {
-        launch_onesweep_pass(plan, p, n, kin, kout, vin, vout, hist_tot, lookback, s);
+        if (!launch_onesweep_pass(plan, p, n, kin, kout, vin, vout, hist_tot, lookback, s)) return false;
         uint64_t* tk = kin; kin = kout; kout = tk;
         uint32_t* tv = vin; vin = vout; vout = tv;
     }
     *keys_out = kin;
     *vals_out = vin;
+    return true;
 }
 
 }  // namespace bvhk

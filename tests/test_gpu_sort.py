@@ -35,3 +35,14 @@ def test_sort_few_distinct_digits(ctx):
     k, _, _ = ctx.sort_pairs(keys, None, 32, 62, 10)
     order = np.argsort(keys >> np.uint64(32), kind="stable")
     np.testing.assert_array_equal(k, keys[order])
+
+
+@pytest.mark.parametrize("n", [1, 8191, 8192, 8193, 1_000_003, 2_499_999, 2_500_000, 2_506_753, 3_000_001])
+@pytest.mark.parametrize("bits", [8, 10])
+def test_sort_keys_only_tile_edges(ctx, n, bits):
+    """Key-only sorts at tile edges: 8192-key tiles, and from 2.5M keys with
+    8-bit digits the wide 16384-key tiles (2,506,753 = 153 wide tiles + 1)."""
+    rng = np.random.default_rng(n * 7 + bits)
+    keys = rng.integers(0, 1 << 62, n, dtype=np.uint64)
+    k, _, _ = ctx.sort_pairs(keys, None, 0, 62, bits)
+    np.testing.assert_array_equal(k, np.sort(keys, kind="stable"))

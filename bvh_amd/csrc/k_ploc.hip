@@ -31,7 +31,9 @@
 //     consecutive clusters and scans the union of their windows once, so each
 //     box read from LDS (stride 3 across lanes: conflict-free) feeds three
 //     candidates; candidates pack (distance bits << 32 | j) into one u64
-//     minimum each;
+//     minimum each; up to r = 128 each pair is evaluated once (the distance
+//     is symmetric) and the later cluster's half arrives by 64-bit LDS
+//     atomic minimum;
 //   * k_ploc_merge: persistent blocks take tiles of 1024 clusters in order
 //     and run mutual-pair detection, new nodes, and stream compaction of the
 //     survivors behind one decoupled look-back scan whose 64-bit words pack
@@ -51,6 +53,7 @@ constexpr int kNnBlock = 256;
 constexpr int kNnPer = 3;                     // consecutive clusters per lane (odd: conflict-free LDS)
 constexpr int kNnTile = kNnBlock * kNnPer;
 constexpr uint32_t kNnFill = 1u << 18;        // lanes that keep every CU busy
+constexpr int kNnSymMaxR = 128;               // symmetric search up to this radius (its halo pairs grow as r^2)
 constexpr int kMergeBlock = 256;
 constexpr uint32_t kMergeBigMin = 1u << 18;   // from here on, 4 clusters per thread per merge tile
 constexpr int kTailBlock = 1024;
@@ -158,6 +161,99 @@ __device__ __forceinline__ bool ploc_global_step(const PlocState& st) {
     return st.maxm > (uint32_t)kPlocSmall && st.flags == 0;
 }
 
+// Symmetric neighbour search of one staged tile (three clusters per lane).
+// dist(i, j) = dist(j, i) bit for bit: the two unions differ at most in the
+// sign of a zero coordinate, which changes no extent except a zero one's
+// sign, and the area of such a box is never -0 (that needs all three
+// products -0, i.e. an odd number of -0 extents in each of three pairs).  So
+// every pair i < j in a window is evaluated once, by the lane owning i: the
+// candidate (dist, j) goes into i's running minimum, and (dist, i) into j's
+// LDS minimum with a 64-bit LDS atomic.  Pairs whose i lies in the r clusters
+// before the tile (and j in it) are evaluated by the tile as well; pairs whose
+// j lies past the tile belong to the next tile's backward half.  The minimum
+// of (distance bits, index) over both halves is the reference's first minimum
+// of its ascending strict-< scan.
+template <bool SEG>
+__device__ __forceinline__ void nn_tile_symmetric(const PlocArgs& a, const float* s_nn, int span, int64_t t0,
+                                                  uint32_t m, int r, uint32_t par) {
+    uint64_t* s_back = reinterpret_cast<uint64_t*>(const_cast<float*>(s_nn) + 6 * span);
+    const uint4* __restrict__ segs = a.seg[par];
+    const int64_t tend = t0 + kNnTile;  // first cluster past the tile
+    auto box_at = [&](int64_t j) {
+        const int q = (int)(j - t0) + r;
+        Box b;
+        b.mnx = s_nn[0 * span + q]; b.mny = s_nn[1 * span + q]; b.mnz = s_nn[2 * span + q];
+        b.mxx = s_nn[3 * span + q]; b.mxy = s_nn[4 * span + q]; b.mxz = s_nn[5 * span + q];
+        return b;
+    };
+    auto back_min = [&](int64_t j, uint64_t key) {
+        __hip_atomic_fetch_min(reinterpret_cast<unsigned long long*>(s_back + (j - t0)), (unsigned long long)key,
+                               __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+    };
+    const int64_t i0 = t0 + (int64_t)kNnPer * threadIdx.x;
+    uint64_t best[kNnPer];
+#pragma unroll
+    for (int c = 0; c < kNnPer; ++c) best[c] = ~0ull;
+    if (i0 < (int64_t)m) {
+        // Forward windows (i, hi[c]] clipped to the mesh; hi is non-decreasing
+        // in c, so one pass over (i0, max hi] serves all three.
+        Box bi[kNnPer];
+        int64_t hi[kNnPer];
+#pragma unroll
+        for (int c = 0; c < kNnPer; ++c) {
+            const int64_t i = i0 + c;
+            if (i < (int64_t)m) {
+                const uint4 sg = segs[SEG ? a.sid[par][i] : 0u];
+                const int64_t shi = (int64_t)sg.x + sg.y - 1;
+                hi[c] = (i + r < shi) ? i + r : shi;
+                bi[c] = box_at(i);
+            } else {
+                hi[c] = i;  // empty forward window
+                bi[c] = bi[0];
+            }
+        }
+        int64_t jhi = hi[0];
+#pragma unroll
+        for (int c = 1; c < kNnPer; ++c) jhi = hi[c] > jhi ? hi[c] : jhi;
+        for (int64_t j = i0 + 1; j <= jhi; ++j) {
+            const Box bj = box_at(j);
+            uint64_t back = ~0ull;
+#pragma unroll
+            for (int c = 0; c < kNnPer; ++c) {
+                if (j > i0 + c && j <= hi[c]) {
+                    const float d = area(merge(bi[c], bj));
+                    const uint64_t kf = nn_key(d, (uint32_t)j);
+                    best[c] = kf < best[c] ? kf : best[c];
+                    const uint64_t kb = nn_key(d, (uint32_t)(i0 + c));
+                    back = kb < back ? kb : back;
+                }
+            }
+            if (back != ~0ull && j < tend) back_min(j, back);
+        }
+    }
+    // Pairs (i, j) with i in the r clusters before the tile and j in it:
+    // i = t0 - u, j = t0 + v for 1 <= u <= r, 0 <= v <= r - u.
+    for (int q = threadIdx.x; q < r * r; q += kNnBlock) {
+        const int64_t u = q / r + 1, v = q % r;
+        const int64_t i = t0 - u, j = t0 + v;
+        if (u + v > r || i < 0 || j >= (int64_t)m || j >= tend) continue;
+        const uint4 sg = segs[SEG ? a.sid[par][i] : 0u];
+        if (j > (int64_t)sg.x + sg.y - 1) continue;  // j is in a later mesh
+        const uint64_t kb = nn_key(area(merge(box_at(i), box_at(j))), (uint32_t)i);
+        if (kb != ~0ull) back_min(j, kb);
+    }
+    __syncthreads();
+#pragma unroll
+    for (int c = 0; c < kNnPer; ++c) {
+        const int64_t i = i0 + c;
+        if (i < (int64_t)m) {
+            const uint64_t b = s_back[i - t0];
+            const uint64_t v = b < best[c] ? b : best[c];
+            a.nn[i] = (v == ~0ull) ? -1 : (int32_t)(uint32_t)v;
+        }
+    }
+}
+
 template <bool SEG>
 __global__ __launch_bounds__(kNnBlock) void k_ploc_nn(PlocArgs a, int k) {
     extern __shared__ float s_nn[];  // 6 SoA arrays of (kNnTile + 2r) floats
@@ -189,6 +285,10 @@ __global__ __launch_bounds__(kNnBlock) void k_ploc_nn(PlocArgs a, int k) {
     for (uint32_t tile = blockIdx.x; tile < tiles; tile += gridDim.x) {
         __syncthreads();
         const int64_t t0 = (int64_t)tile * tc;
+        if (G == 0 && r <= kNnSymMaxR) {
+            uint64_t* s_back = reinterpret_cast<uint64_t*>(s_nn + 6 * span);
+            for (int q = threadIdx.x; q < kNnTile; q += kNnBlock) s_back[q] = ~0ull;
+        }
         for (int q = threadIdx.x; q < span; q += kNnBlock) {
             int64_t p = t0 - r + q;
             if (p >= 0 && p < (int64_t)m) {
@@ -232,6 +332,10 @@ __global__ __launch_bounds__(kNnBlock) void k_ploc_nn(PlocArgs a, int k) {
                 best = o < best ? o : best;
             }
             if (g == 0 && i < (int64_t)m) a.nn[i] = (best == ~0ull) ? -1 : (int32_t)(uint32_t)best;
+            continue;
+        }
+        if (r <= kNnSymMaxR) {
+            nn_tile_symmetric<SEG>(a, s_nn, span, t0, m, r, par);
             continue;
         }
         // This lane's clusters i0, i0+1, i0+2 and their windows, clipped to
@@ -706,7 +810,9 @@ void launch_ploc_nn(const PlocArgs& a, int k, uint32_t m_bound, uint32_t max_gri
     uint32_t g = (m_bound + kNnBlock / 8 - 1) / (kNnBlock / 8);  // enough for the finest split
     if (g > max_grid) g = max_grid;
     g = g < 1 ? 1 : (g > kNnGridMax ? kNnGridMax : g);
-    size_t lds = (size_t)6 * (kNnTile + 2 * a.radius) * sizeof(float);
+    // Six SoA float arrays of the staged span, then the tile's backward
+    // minima (symmetric search; 6 * span floats keep them 8-byte aligned).
+    size_t lds = (size_t)6 * (kNnTile + 2 * a.radius) * sizeof(float) + (size_t)kNnTile * sizeof(uint64_t);
     if (a.sid[0]) hipLaunchKernelGGL(k_ploc_nn<true>, dim3(g), dim3(kNnBlock), lds, s, a, k);
     else hipLaunchKernelGGL(k_ploc_nn<false>, dim3(g), dim3(kNnBlock), lds, s, a, k);
 }

@@ -53,6 +53,15 @@ def _assert_identical(results, ref):
             assert nodes.tobytes() == rn.tobytes(), f"run {r} mesh {s}: {O.nodes_equal(nodes, rn)}"
 
 
+@pytest.mark.parametrize("radius", [16, 100])
+def test_batch_ploc_large_segments(ctx, radius):
+    """Four 300K-triangle meshes: the batch's first iterations use the
+    large-m (segmented) neighbour search, each mesh alone the small-m one."""
+    meshes = [B.Mesh.batch_mesh(31, i, 300_000) for i in range(4)]
+    results, ref, info = _run_batch(ctx, meshes, B.ALGO_PLOC, radius=radius, runs=1)
+    _assert_identical(results, ref)
+
+
 @pytest.mark.parametrize("algo", [B.ALGO_LBVH, B.ALGO_PLOC])
 def test_batch_uneven_meshes(ctx, algo):
     meshes = _meshes(11, 24, 1000, 30000)

@@ -41,6 +41,13 @@ def test_ploc_blobs_1m_r25(ctx):
     _check_ploc(ctx, B.Mesh.blobs(7, 1_000_000), 25)
 
 
+@pytest.mark.parametrize("radius", [100, 200])
+def test_ploc_blobs_1m_wide_radius(ctx, radius):
+    """Large-m neighbour search (three clusters per lane): symmetric pair
+    evaluation up to r = 128, the one-sided scan above it."""
+    _check_ploc(ctx, B.Mesh.blobs(11, 1_000_000), radius)
+
+
 def test_ploc_identical_triangles(ctx):
     n = 2600  # one merge per iteration: exercises ~550 global iterations and the LDS tail
     soa = np.tile(np.array([[0.1], [0.2], [0.3], [0.4], [0.2], [0.3], [0.1], [0.5], [0.3]], np.float32), (1, n))

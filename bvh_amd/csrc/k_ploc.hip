@@ -37,7 +37,9 @@
 //   * k_ploc_merge: persistent blocks take tiles of 1024 clusters in order
 //     and run mutual-pair detection, new nodes, and stream compaction of the
 //     survivors behind one decoupled look-back scan whose 64-bit words pack
-//     (segment-start flag, merges within the mesh, survivors overall);
+//     (segment-start flag, merges within the mesh, survivors overall); the
+//     tile's global traffic is one lane per cluster, the scan runs on its
+//     flags in LDS;
 //   * loop control lives on the device: iteration k reads state[k] and
 //     writes state[k+1] (cluster count, largest mesh, buffer parity; a ring of
 //     kPlocMaxIter slots) and per-mesh records (first cluster, count, next
@@ -423,11 +425,27 @@ __device__ __forceinline__ uint64_t block_excl_scan_seg(uint64_t v, uint64_t* s_
     return scomb(before, prev);
 }
 
-// Merge + compaction over tiles of ITEMS clusters per thread (the tile size
-// is a function of m, shared with the look-back reset in k_ploc_nn).
+// Per-tile LDS of the merge: flags, partners, mesh ids, output slots and
+// new node ids of the tile's clusters, plus the scan's scratch.
+template <int ITEMS>
+struct MergeLds {
+    int32_t nj[kMergeBlock * ITEMS];
+    uint32_t sid[kMergeBlock * ITEMS];
+    uint32_t pos[kMergeBlock * ITEMS];
+    uint32_t nid[kMergeBlock * ITEMS];
+    uint8_t fl[kMergeBlock * ITEMS];
+    uint64_t w[kMergeBlock / kWave];
+    uint64_t prefix;
+    uint32_t tile;
+};
+
+// Merge + compaction over tiles of kMergeBlock * ITEMS clusters (the tile
+// size is a function of m, shared with the look-back reset in k_ploc_nn).
+// Global reads and writes go one lane per cluster (coalesced); the scan,
+// which needs each thread's clusters consecutive, works on the tile's flags
+// in LDS and hands output slots back through LDS.
 template <bool SEG, int ITEMS>
-__device__ void merge_tiles(const PlocArgs& a, const PlocState& st, int kn, uint32_t* s_tile, uint64_t* s_w,
-                            uint64_t* s_prefix) {
+__device__ void merge_tiles(const PlocArgs& a, const PlocState& st, int kn, MergeLds<ITEMS>& sm) {
     constexpr int kMergeItems = ITEMS;
     constexpr uint32_t kMergeTile = kMergeBlock * ITEMS;
     const uint32_t m = st.m;
@@ -440,46 +458,58 @@ __device__ void merge_tiles(const PlocArgs& a, const PlocState& st, int kn, uint
     const uint32_t* __restrict__ iin = a.id[par];
     float4* __restrict__ bout = a.box[par ^ 1u];
     uint32_t* __restrict__ iout = a.id[par ^ 1u];
+    const int tid = threadIdx.x;
     while (true) {
         __syncthreads();
-        if (threadIdx.x == 0) *s_tile = atomicAdd(a.tile_ctr, 1u);
+        if (tid == 0) sm.tile = atomicAdd(a.tile_ctr, 1u);
         __syncthreads();
-        const uint32_t tile = *s_tile;
+        const uint32_t tile = sm.tile;
         if (tile >= tiles) break;
-        const uint32_t base = tile * kMergeTile + threadIdx.x * kMergeItems;
+        const uint32_t tbase = tile * kMergeTile;
 
-        int32_t nj[kMergeItems];
-        uint32_t sid[kMergeItems];
-        uint32_t fl[kMergeItems];  // bit0 survivor, bit1 merging, bit2 first of its mesh, bit3 last
-        uint64_t agg = 0;
+        // Phase A, one lane per cluster (coalesced): mutual pairs and flags.
 #pragma unroll
-        for (int it = 0; it < kMergeItems; ++it) {
-            const uint32_t i = base + it;
-            fl[it] = 0;
-            nj[it] = -1;
-            sid[it] = 0;
+        for (int u = 0; u < kMergeItems; ++u) {
+            const uint32_t q = u * kMergeBlock + tid;
+            const uint32_t i = tbase + q;
+            uint32_t f = 0;
             if (i < m) {
                 const uint32_t s = SEG ? a.sid[par][i] : 0u;
                 const uint4 sg = segs[s];
-                sid[it] = s;
                 const int32_t j = nn[i];
-                nj[it] = j;
                 const bool mutual = (j >= 0) && nn[j] == (int32_t)i;
                 const bool merging = mutual && (int32_t)i < j;
                 const bool valid = !(mutual && j < (int32_t)i);
                 const bool first = (i == sg.x);
                 const bool last = (i == sg.x + sg.y - 1);
-                fl[it] = (valid ? 1u : 0u) | (merging ? 2u : 0u) | (first ? 4u : 0u) | (last ? 8u : 0u);
-                const uint64_t e = (first ? kSegF : 0ull) | ((uint64_t)(merging ? 1u : 0u) << 31) | (valid ? 1ull : 0ull);
+                f = (valid ? 1u : 0u) | (merging ? 2u : 0u) | (first ? 4u : 0u) | (last ? 8u : 0u);
+                sm.nj[q] = j;
+                if (SEG) sm.sid[q] = s;
+            }
+            sm.fl[q] = (uint8_t)f;
+        }
+        __syncthreads();
+
+        // Phase B, ITEMS consecutive clusters per thread: segmented scan of
+        // (mesh start, merges, survivors) across the tile and the look-back.
+        const uint32_t base = tid * kMergeItems;
+        uint32_t fl[kMergeItems];
+        uint64_t agg = 0;
+#pragma unroll
+        for (int it = 0; it < kMergeItems; ++it) {
+            fl[it] = sm.fl[base + it];
+            if (tbase + base + it < m) {
+                const uint32_t f = fl[it];
+                const uint64_t e = ((f & 4u) ? kSegF : 0ull) | ((uint64_t)((f >> 1) & 1u) << 31) | (uint64_t)(f & 1u);
                 agg = scomb(agg, e);
             }
         }
         uint64_t tagg;
-        const uint64_t excl = block_excl_scan_seg<kMergeBlock>(agg, s_w, &tagg);
+        const uint64_t excl = block_excl_scan_seg<kMergeBlock>(agg, sm.w, &tagg);
         // Decoupled look-back by wave 0, 64 predecessors' words per step; the
         // window is combined in tile order (farther first) under scomb.
-        if (threadIdx.x < kWave) {
-            const int lane = threadIdx.x;
+        if (tid < kWave) {
+            const int lane = tid;
             uint64_t* sw = a.status + tile;
             if (lane == 0)
                 __hip_atomic_store(sw, (tile == 0 ? kStP : kStA) | tagg, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -510,48 +540,36 @@ __device__ void merge_tiles(const PlocArgs& a, const PlocState& st, int kn, uint
             }
             if (lane == 0) {
                 if (tile > 0) __hip_atomic_store(sw, kStP | scomb(pre, tagg), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                *s_prefix = pre;
+                sm.prefix = pre;
                 if (tile == tiles - 1) a.state[kn].m = (uint32_t)(scomb(pre, tagg) & kVMask);
             }
         }
         __syncthreads();
-        const uint64_t run0 = scomb(*s_prefix, excl);
+        // Each cluster's output slot (survivors) and new node id (merging
+        // ones) into LDS; mesh starts and ends update the mesh records.
+        const uint64_t run0 = scomb(sm.prefix, excl);
         uint32_t pos = (uint32_t)(run0 & kVMask);
         uint32_t mrank = (uint32_t)((run0 & kMMask) >> 31);
 #pragma unroll
         for (int it = 0; it < kMergeItems; ++it) {
-            const uint32_t i = base + it;
+            const uint32_t i = tbase + base + it;
             if (i >= m) continue;
             const uint32_t f = fl[it];
-            const uint32_t s = sid[it];
+            const uint32_t s = SEG ? sm.sid[base + it] : 0u;
             if (f & 4u) {
                 mrank = 0;
                 segs_out[s].x = pos;
             }
-            const uint4 sg = segs[s];
             if (f & 1u) {
-                Box b = load_box(bin + 2 * (size_t)i);
-                uint32_t cid = iin[i];
+                sm.pos[base + it] = pos;
                 if (f & 2u) {
-                    const SegDesc& d = seg_desc(a.sb, s);
-                    const uint32_t root = 2 * d.n - 2;
-                    const uint32_t j = (uint32_t)nj[it];
-                    const Box bj = load_box(bin + 2 * (size_t)j);
-                    const uint32_t idj = iin[j];
-                    b = merge(b, bj);
-                    const uint32_t nid = sg.z + mrank;
+                    sm.nid[base + it] = segs[s].z + mrank;
                     ++mrank;
-                    if (nid <= root)
-                        store_node(d.nodes, remap_id(nid, root), b, (int32_t)remap_id(cid, root),
-                                   (int32_t)remap_id(idj, root));
-                    cid = nid;
                 }
-                store_box(bout + 2 * (size_t)pos, b);
-                iout[pos] = cid;
-                if (SEG) a.sid[par ^ 1u][pos] = s;
                 ++pos;
             }
             if (f & 8u) {
+                const uint4 sg = segs[s];
                 const uint32_t nm = sg.y - mrank;
                 segs_out[s].y = nm;
                 segs_out[s].z = sg.z + mrank;
@@ -560,22 +578,52 @@ __device__ void merge_tiles(const PlocArgs& a, const PlocState& st, int kn, uint
                 if (sg.y >= 2 && mrank == 0) atomicOr(&a.state[kn].flags, kPlocNoProgress);
             }
         }
+        __syncthreads();
+
+        // Phase C, one lane per cluster again (coalesced): new nodes and the
+        // compacted survivors.
+#pragma unroll
+        for (int u = 0; u < kMergeItems; ++u) {
+            const uint32_t q = u * kMergeBlock + tid;
+            const uint32_t i = tbase + q;
+            if (i >= m) continue;
+            const uint32_t f = sm.fl[q];
+            if (!(f & 1u)) continue;
+            const uint32_t s = SEG ? sm.sid[q] : 0u;
+            const uint32_t p = sm.pos[q];
+            Box b = load_box(bin + 2 * (size_t)i);
+            uint32_t cid = iin[i];
+            if (f & 2u) {
+                const SegDesc& d = seg_desc(a.sb, s);
+                const uint32_t root = 2 * d.n - 2;
+                const uint32_t j = (uint32_t)sm.nj[q];
+                const Box bj = load_box(bin + 2 * (size_t)j);
+                const uint32_t idj = iin[j];
+                b = merge(b, bj);
+                const uint32_t nid = sm.nid[q];
+                if (nid <= root)
+                    store_node(d.nodes, remap_id(nid, root), b, (int32_t)remap_id(cid, root),
+                               (int32_t)remap_id(idj, root));
+                cid = nid;
+            }
+            store_box(bout + 2 * (size_t)p, b);
+            iout[p] = cid;
+            if (SEG) a.sid[par ^ 1u][p] = s;
+        }
     }
 }
 
 template <bool SEG>
 __global__ __launch_bounds__(kMergeBlock) void k_ploc_merge(PlocArgs a, int k) {
-    __shared__ uint32_t s_tile;
-    __shared__ uint64_t s_w[kMergeBlock / kWave];
-    __shared__ uint64_t s_prefix;
+    __shared__ MergeLds<4> s_m;  // also holds the one-cluster-per-thread tile
     const PlocState st = a.state[k];
     const int kn = (k + 1) % kPlocMaxIter;
     if (!ploc_global_step(st)) {
         if (blockIdx.x == 0 && threadIdx.x == 0) a.state[kn] = st;
         return;
     }
-    if (merge_tile(st.m) == 4u * kMergeBlock) merge_tiles<SEG, 4>(a, st, kn, &s_tile, s_w, &s_prefix);
-    else merge_tiles<SEG, 1>(a, st, kn, &s_tile, s_w, &s_prefix);
+    if (merge_tile(st.m) == 4u * kMergeBlock) merge_tiles<SEG, 4>(a, st, kn, s_m);
+    else merge_tiles<SEG, 1>(a, st, kn, *reinterpret_cast<MergeLds<1>*>(&s_m));
 }
 
 // Finishes every remaining iteration of one mesh (block = mesh) in LDS.

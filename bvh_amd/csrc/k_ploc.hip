@@ -82,9 +82,10 @@ __host__ __device__ __forceinline__ uint32_t merge_tile(uint32_t m) {
 
 // Lanes per cluster in the neighbour search: 0 = three clusters per lane
 // (large m), else G lanes split each window (small m: more lanes in flight).
+// Measured: three per lane (with symmetric pairs) beats one per lane from
+// kNnFill clusters on.
 __device__ __forceinline__ int nn_lanes(uint32_t m) {
-    if (m >= 3u * kNnFill) return 0;
-    if (m >= kNnFill) return 1;
+    if (m >= kNnFill) return 0;
     if (m >= kNnFill / 2) return 2;
     if (m >= kNnFill / 4) return 4;
     return 8;

@@ -254,7 +254,34 @@ __global__ __launch_bounds__(BLOCK) void k_onesweep(const uint64_t* __restrict__
                            __HIP_MEMORY_SCOPE_AGENT);
     }
     uint32_t total;
-    uint32_t lrun = block_excl_sum<BLOCK>(tsum, s_w, &total);
+    const uint32_t lrun0 = block_excl_sum<BLOCK>(tsum, s_w, &total);
+    // The tile's local bin starts: all the reorder into LDS needs.
+    {
+        uint32_t lr = lrun0;
+#pragma unroll
+        for (int j = 0; j < BPT; ++j) {
+            if (!owner) continue;
+            s_local[tid * BPT + j] = lr;
+            lr += cnt[j];
+        }
+    }
+    __syncthreads();
+    // Reorder of the tile into LDS by digit.  Waves that own no bin (8-bit
+    // digits: all but the first four) do theirs while the owners walk back.
+    auto reorder = [&]() {
+#pragma unroll
+        for (int it = 0; it < kSortItems; ++it) {
+            uint32_t idx = seg + it * kWave + lane;
+            if (idx < n) {
+                uint32_t d = (uint32_t)(key[it] >> shift) & MASK;
+                uint32_t pos = s_local[d] + wh[d] + rk[it];
+                s_keys[pos] = key[it];
+                if (VALS) s_vals[pos] = val[it];
+            }
+        }
+    };
+    const bool owner_wave = wave * kWave * BPT < BINS;
+    if (!owner_wave) reorder();
     // Decoupled look-back: each step reads the status words of the next
     // kLookWindow predecessors of all the thread's BPT bins at once (their
     // loads overlap), then consumes them nearest first: aggregates are added
@@ -305,6 +332,7 @@ __global__ __launch_bounds__(BLOCK) void k_onesweep(const uint64_t* __restrict__
         }
         if (stalled) __builtin_amdgcn_s_sleep(1);
     }
+    uint32_t lrun = lrun0;
 #pragma unroll
     for (int j = 0; j < BPT; ++j) {
         const int b = tid * BPT + j;
@@ -312,23 +340,11 @@ __global__ __launch_bounds__(BLOCK) void k_onesweep(const uint64_t* __restrict__
         if (tile > 0)
             __hip_atomic_store(status + (size_t)tile * BINS + b, kFlagP | (g[j] + cnt[j]), __ATOMIC_RELAXED,
                                __HIP_MEMORY_SCOPE_AGENT);
-        s_local[b] = lrun;
         s_global[b] = gstart + g[j] - lrun;
         lrun += cnt[j];
         gstart += tot[j];
     }
-    __syncthreads();
-
-#pragma unroll
-    for (int it = 0; it < kSortItems; ++it) {
-        uint32_t idx = seg + it * kWave + lane;
-        if (idx < n) {
-            uint32_t d = (uint32_t)(key[it] >> shift) & MASK;
-            uint32_t pos = s_local[d] + wh[d] + rk[it];
-            s_keys[pos] = key[it];
-            if (VALS) s_vals[pos] = val[it];
-        }
-    }
+    if (owner_wave) reorder();
     __syncthreads();
 
     const uint32_t tile_n = (n - base < (uint32_t)TILE) ? (n - base) : (uint32_t)TILE;

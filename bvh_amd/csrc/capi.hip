@@ -71,8 +71,7 @@ struct bvh_context {
     SegTables tabs;  // eager segmented builds
     DevBuf trace_ovf, trace_list, trace_io;  // traversal queue and host-call staging
     DevBuf ray_keys[2];                      // ray coherence keys (sorted before tracing)
-    uint32_t* host_flags = nullptr;          // pinned: each build's per-mesh flag words land here
-    size_t host_flags_words = 0;
+    uint32_t* host_flags = nullptr;          // pinned: a single build's 8 flag words land here
     // last build's sorted keys (for bvh_debug_sorted_keys)
     uint64_t* last_keys = nullptr;
     uint32_t* last_vals = nullptr;
@@ -385,17 +384,21 @@ HierArgs hier_args(Workspace& w, const Job& j, const uint64_t* keys, const uint3
 
 // Non-finite input and internal-error words of every segment.
 int check_input_flags(bvh_ctx_t ctx, Workspace& w, uint32_t k) {
-    // Pinned destination: a pageable one would add a staged copy to every build.
+    // Single builds read their flag words into a pinned buffer (a pageable
+    // destination adds a staged copy to every build); multi-mesh builds,
+    // where that copy is small beside the build, use a pageable one.
     const size_t words = (size_t)k * 8;
-    if (ctx->host_flags_words < words) {
-        if (ctx->host_flags) HIP_TRY(ctx, hipHostFree(ctx->host_flags));
-        ctx->host_flags = nullptr;
-        ctx->host_flags_words = 0;
-        HIP_TRY(ctx, hipHostMalloc((void**)&ctx->host_flags, words * 4, hipHostMallocDefault));
-        ctx->host_flags_words = words;
+    std::vector<uint32_t> pageable;
+    uint32_t* dst = nullptr;
+    if (k == 1) {
+        if (!ctx->host_flags) HIP_TRY(ctx, hipHostMalloc((void**)&ctx->host_flags, words * 4, hipHostMallocDefault));
+        dst = ctx->host_flags;
+    } else {
+        pageable.resize(words);
+        dst = pageable.data();
     }
-    const uint32_t* sc = ctx->host_flags;
-    HIP_TRY(ctx, hipMemcpyAsync(ctx->host_flags, w.scene.p, words * 4, hipMemcpyDeviceToHost, ctx->stream));
+    const uint32_t* sc = dst;
+    HIP_TRY(ctx, hipMemcpyAsync(dst, w.scene.p, words * 4, hipMemcpyDeviceToHost, ctx->stream));
     HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
     for (uint32_t s = 0; s < k; ++s) {
         if (sc[8 * s + 6]) {
@@ -769,8 +772,10 @@ int batch_enqueue(bvh_batch_t b, const SortPlan& plan) {
 int batch_capture(bvh_batch_t b) {
     bvh_ctx_t ctx = b->ctx;
     batch_destroy_graph(b);
+    // Captured graphs keep the 8192-key sort tile (the wide tile has not run
+    // inside a captured graph).
     SortPlan plan = make_sort_plan(b->job.total, b->job.begin_bit, b->job.end_bit, b->radix_bits,
-                                   (int)b->job.fsb.nblocks);
+                                   (int)b->job.fsb.nblocks, false, false);
     if (plan.passes > 8) return set_err(ctx, BVH_E_UNSUPPORTED, "batch keys need more than 8 sort passes");
     TRY(reserve_front(ctx, b->ws, b->job, plan, b->algo == BVH_ALGO_LBVH));
     if (b->algo == BVH_ALGO_PLOC) TRY(reserve_ploc(ctx, b->ws, b->job));

@@ -78,8 +78,9 @@ struct SortPlan {
     int hist_blocks;   // blocks of the histogram producer
 };
 // hist_blocks <= 0 picks a default producer size; vals: the sort carries a
-// 32-bit payload (narrow tiles).
-SortPlan make_sort_plan(uint32_t n, int begin_bit, int end_bit, int bits, int hist_blocks = 0, bool vals = false);
+// 32-bit payload (narrow tiles); wide_ok = false keeps the narrow tile.
+SortPlan make_sort_plan(uint32_t n, int begin_bit, int end_bit, int bits, int hist_blocks = 0, bool vals = false,
+                        bool wide_ok = true);
 // Bytes of the zero-initialised lookback area a plan needs (status words + tile counters).
 size_t sort_zero_bytes(const SortPlan& p);
 

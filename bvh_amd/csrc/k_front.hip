@@ -144,7 +144,7 @@ void launch_tri_bounds(const SegBlocks& sb, float4* tri_boxes, uint32_t* seg_sce
     hipLaunchKernelGGL(k_tri_bounds, dim3(sb.nblocks), dim3(kFrontBlock), 0, s, sb, tri_boxes, seg_scene);
 }
 
-SortPlan make_sort_plan(uint32_t n, int begin_bit, int end_bit, int bits, int hist_blocks, bool vals) {
+SortPlan make_sort_plan(uint32_t n, int begin_bit, int end_bit, int bits, int hist_blocks, bool vals, bool wide_ok) {
     SortPlan p;
     p.bits = bits;
     int span = end_bit - begin_bit;
@@ -155,7 +155,7 @@ SortPlan make_sort_plan(uint32_t n, int begin_bit, int end_bit, int bits, int hi
         w = w < 0 ? 0 : (w > bits ? bits : w);
         p.mask[i] = (1u << w) - 1u;
     }
-    p.block = (bits == 8 && !vals && n >= kSortWideMinKeys) ? kSortThreadsWide : kSortThreads;
+    p.block = (wide_ok && bits == 8 && !vals && n >= kSortWideMinKeys) ? kSortThreadsWide : kSortThreads;
     p.tile = (uint32_t)p.block * kSortItems;
     p.tiles = (n + p.tile - 1) / p.tile;
     if (p.tiles < 1) p.tiles = 1;

@@ -167,6 +167,25 @@ def test_build_batched_eager(ctx, algo):
 
 
 @pytest.mark.parametrize("algo", [B.ALGO_LBVH, B.ALGO_PLOC])
+def test_build_batched_eager_wide_sort(ctx, algo):
+    # Three 1M-triangle meshes eagerly: 3M segmented keys take the wide
+    # (16384-key) sort tile; each tree equals its standalone build.
+    meshes = [B.Mesh.batch_mesh(41, i, 1_000_000) for i in range(3)]
+    dms = [ctx.upload(m) for m in meshes]
+    trees = [ctx.tree() for _ in meshes]
+    try:
+        ctx.build_batched(dms, trees, algo=algo, radius=16)
+        got = [tr.download() for tr in trees]
+        ref = [_standalone(ctx, dm, algo, 16) for dm in dms]
+    finally:
+        for tr in trees:
+            tr.free()
+        for d in dms:
+            d.free()
+    _assert_identical([got], ref)
+
+
+@pytest.mark.parametrize("algo", [B.ALGO_LBVH, B.ALGO_PLOC])
 def test_batch_many_tiny_meshes(ctx, algo):
     # 3000 meshes of 1-40 triangles: 12 mesh-id bits in the keys, thousands of
     # one-chunk segments, every mesh finished in the per-mesh LDS tail (PLOC).

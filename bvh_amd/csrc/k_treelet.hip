@@ -65,10 +65,6 @@ struct Slot {
     int32_t exr[kTMax - 1];
     int32_t n;
     int32_t nexp;
-    int32_t avail[8];        // reconstruction scratch
-    int32_t stack[8];
-    int32_t qmask[8];
-    int32_t qnode[8];
 };
 
 struct WaveLds {
@@ -305,62 +301,123 @@ __device__ void dp_group(Slot& sl, int sub, int G) {
     }
 }
 
+// Small per-lane tables of the reconstruction as 8-wide vector values: they
+// live in VGPRs and lane-dependent lookups become selects (a plain local
+// array would be turned into an indexed load and land in scratch or LDS).
+typedef int32_t RegTab __attribute__((ext_vector_type(8)));
+
+// The entry whose bit is set in the one-hot mask.
+__device__ __forceinline__ int32_t reg_pick(RegTab v, uint32_t onehot) {
+    int32_t r = v[0];
+#pragma unroll
+    for (int k = 1; k < 8; ++k) r = ((onehot >> k) & 1u) ? v[k] : r;
+    return r;
+}
+__device__ __forceinline__ RegTab reg_put(RegTab v, uint32_t onehot, int32_t x) {
+#pragma unroll
+    for (int k = 0; k < 8; ++k) v[k] = ((onehot >> k) & 1u) ? x : v[k];
+    return v;
+}
+
 // Reconstruction (reference lbvh_plus_builder.cu:162-245): the treelet's
 // internal node ids in DFS pop order (right child popped first), assigned
-// breadth first from the partition table; boxes are subset unions.
+// breadth first from the partition table; boxes are subset unions.  The DFS
+// order is computed in closed form from the expansion records (every node is
+// expanded after its parent): an internal node's rank is its parent's + 1 for
+// a right child and + 1 + (internal nodes under the right sibling) for a left
+// child.  Ranks, the BFS queue and the leaves stay in registers; per BFS node
+// only the partition entry and the leaf boxes come from LDS.
 __device__ void rebuild(const TreeletArgs& a, Slot& sl, int32_t root) {
+    constexpr int kI = kTMax - 1;  // internal nodes of a full treelet
     DNode* __restrict__ nodes = a.nodes;
     const int n = sl.n;
-    const int nexp = sl.nexp;
+    const int nexp = sl.nexp;  // n - 1
     const int full = (1 << n) - 1;
-    int navail = 0, top = 0;
-    sl.stack[top++] = root;
-    while (top > 0) {
-        const int32_t curr = sl.stack[--top];
-        bool is_leaf = false;
-        for (int i = 0; i < n; ++i) is_leaf |= (sl.act[i] == curr);
-        if (is_leaf) continue;
-        sl.avail[navail++] = curr;
-        int32_t cl = -1, cr = -1;
-        for (int e = 0; e < nexp; ++e)
-            if (sl.exn[e] == curr) {
-                cl = sl.exl[e];
-                cr = sl.exr[e];
-            }
-        sl.stack[top++] = cl;
-        sl.stack[top++] = cr;
+    RegTab act = 0, en = INT32_MIN, el = 0, er = 0;
+#pragma unroll
+    for (int i = 0; i < kTMax; ++i) act[i] = sl.act[i];
+#pragma unroll
+    for (int e = 0; e < kI; ++e) {
+        en[e] = (e < nexp) ? sl.exn[e] : INT32_MIN;
+        el[e] = sl.exl[e];
+        er[e] = sl.exr[e];
     }
-    int qh = 0, qt = 0, used = 1;
-    sl.qmask[qt] = full;
-    sl.qnode[qt] = sl.avail[0];
-    ++qt;
-    while (qh < qt) {
-        const int mask = sl.qmask[qh];
-        const int32_t cur = sl.qnode[qh];
-        ++qh;
+    // Internal nodes under each expanded node (children come later).
+    RegTab sz = 0;
+#pragma unroll
+    for (int e = kI - 1; e >= 0; --e) {
+        int32_t c = 0;
+        if (e < nexp) {
+            c = 1;
+#pragma unroll
+            for (int f = e + 1; f < kI; ++f)
+                if (en[f] == el[e] || en[f] == er[e]) c += sz[f];
+        }
+        sz[e] = c;
+    }
+    // Right-first preorder ranks (the root, expanded first, has rank 0).
+    RegTab rk = 0;
+#pragma unroll
+    for (int e = 0; e < kI; ++e) {
+        if (e >= nexp) continue;
+        int32_t szr = 0;
+#pragma unroll
+        for (int f = e + 1; f < kI; ++f)
+            if (en[f] == er[e]) szr = sz[f];
+#pragma unroll
+        for (int f = e + 1; f < kI; ++f) {
+            if (en[f] == er[e]) rk[f] = rk[e] + 1;
+            if (en[f] == el[e]) rk[f] = rk[e] + 1 + szr;
+        }
+    }
+    RegTab avail = root;
+#pragma unroll
+    for (int r = 0; r < kI; ++r) {
+        avail[r] = root;
+#pragma unroll
+        for (int e = 0; e < kI; ++e)
+            if (e < nexp && rk[e] == r) avail[r] = en[e];
+    }
+    // The BFS takes internal ids in rank order: avail as a FIFO (shifted on
+    // every take), the queue appended at the one-hot tail `qt`.
+    RegTab qm = 0, qn = 0;
+    qm[0] = full;
+    qn[0] = avail[0];
+    auto take = [&]() {
+#pragma unroll
+        for (int k = 0; k < kI - 1; ++k) avail[k] = avail[k + 1];
+        return avail[0];
+    };
+    uint32_t qt = 2u;  // one-hot: next free queue slot (slot 0 holds the root)
+#pragma unroll
+    for (int qh = 0; qh < kI; ++qh) {
+        if (!((qt >> qh) > 1u)) continue;  // slot qh not filled
+        const int mask = qm[qh];
+        const int32_t cur = qn[qh];
         const int lm = sl.part[mask];
         const int rm = mask ^ lm;
         int32_t ln, rn;
         if (__popc(lm) == 1) {
-            ln = sl.act[__ffs(lm) - 1];
+            ln = reg_pick(act, (uint32_t)lm);
         } else {
-            ln = sl.avail[used++];
-            sl.qmask[qt] = lm;
-            sl.qnode[qt] = ln;
-            ++qt;
+            ln = take();
+            qm = reg_put(qm, qt, lm);
+            qn = reg_put(qn, qt, ln);
+            qt <<= 1;
         }
         if (__popc(rm) == 1) {
-            rn = sl.act[__ffs(rm) - 1];
+            rn = reg_pick(act, (uint32_t)rm);
         } else {
-            rn = sl.avail[used++];
-            sl.qmask[qt] = rm;
-            sl.qnode[qt] = rn;
-            ++qt;
+            rn = take();
+            qm = reg_put(qm, qt, rm);
+            qn = reg_put(qn, qt, rn);
+            qt <<= 1;
         }
         Box b = empty_box();
         int lo = 0x7fffffff, hi = -1;
-        for (int i = 0; i < n; ++i)
-            if ((mask >> i) & 1) {
+#pragma unroll
+        for (int i = 0; i < kTMax; ++i)
+            if (i < n && ((mask >> i) & 1)) {
                 b = merge(b, lds_box(sl.lb[i]));
                 lo = sl.lo[i] < lo ? sl.lo[i] : lo;
                 hi = sl.hi[i] > hi ? sl.hi[i] : hi;

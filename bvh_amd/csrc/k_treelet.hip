@@ -53,7 +53,7 @@ constexpr float kCostInf = 1e30f;
 // group, rebuilt by the owner.
 struct Slot {
     float lb[kTMax][6];      // treelet leaf boxes, in treelet order
-    float cost[128];
+    float cost1[127];        // cost of leaf subset s (1..127) at cost1[s - 1]
     uint8_t part[128];
     int32_t act[kTMax];      // treelet leaves (node ids)
     int32_t lc[kTMax];       // their children (when internal)
@@ -63,13 +63,17 @@ struct Slot {
     int32_t exn[kTMax - 1];  // expanded internal nodes and their children, in order
     int32_t exl[kTMax - 1];
     int32_t exr[kTMax - 1];
-    int32_t n;
-    int32_t nexp;
+    int32_t n;  // leaves; the treelet has n - 1 expanded (internal) nodes
 };
 
 struct WaveLds {
     Slot slot[kGroups];
 };
+// An odd slot stride in words puts equal offsets of the eight slots in eight
+// different LDS banks (the groups walk identical subset sequences), and four
+// waves' slots fit 32 KB (five workgroups per CU).
+static_assert(sizeof(Slot) % 8 == 4, "slot stride must be an odd number of words");
+static_assert(sizeof(WaveLds) * kTWaves + 127 <= 32768, "treelet LDS (slots + subset table) must stay within 32 KB");
 
 __device__ __forceinline__ void wave_sync() {
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
@@ -118,13 +122,13 @@ __device__ __forceinline__ DNode node_rec(const DNode* nodes, int32_t i) {
 
 // Union of the subset's leaf boxes in ascending index order from the empty
 // box (the oracle's order: min/max by selects are order-sensitive only for
-// signed zeros, and this keeps even those identical).
-template <class BoxAt>
-__device__ __forceinline__ Box subset_box(BoxAt at, int s) {
-    Box b = empty_box();
-#pragma unroll
-    for (int i = 0; i < kTMax; ++i)
-        if ((s >> i) & 1) b = merge(b, at(i));
+// signed zeros, and this keeps even those identical).  Merging a finite box
+// into the empty box returns it bit for bit, so the fold starts at the first
+// leaf and costs one merge per further leaf (leaf boxes read from LDS; the
+// subsets of one DP step all have the same size, so the loop is uniform).
+__device__ __forceinline__ Box subset_box(const Slot& sl, int s) {
+    Box b = lds_box(sl.lb[__ffs(s) - 1]);
+    for (int t = s & (s - 1); t != 0; t &= t - 1) b = merge(b, lds_box(sl.lb[__ffs(t) - 1]));
     return b;
 }
 
@@ -192,45 +196,39 @@ __device__ int form(const TreeletArgs& a, int32_t root, Slot& sl, Box& rootbox) 
         ++n;
     }
     sl.n = n;
-    sl.nexp = nexp;
     return n;
 }
 
 // Cost and best partition of subset s (reference lbvh_plus_builder.cu:131-160):
 // partitions in the reference's enumeration order, strict '<'.
-__device__ __forceinline__ void dp_subset(Slot& sl, const Box (&lb)[kTMax], int s) {
-    const Box b = subset_box([&](int i) { return lb[i]; }, s);
+__device__ __forceinline__ void dp_subset(Slot& sl, int s) {
+    const Box b = subset_box(sl, s);
     const float areaS = area(b);
     float minCost = kCostInf;
     int bestP = -1;
     const int delta = (s - 1) & s;
     int p = (-delta) & s;
     do {
-        const float c = sl.cost[p] + sl.cost[s ^ p];
+        const float c = sl.cost1[p - 1] + sl.cost1[(s ^ p) - 1];
         if (c < minCost) {
             minCost = c;
             bestP = p;
         }
         p = (p - delta) & s;
     } while (p != 0);
-    sl.cost[s] = minCost + 1.0f * areaS;
+    sl.cost1[s - 1] = minCost + 1.0f * areaS;
     sl.part[s] = (uint8_t)bestP;
 }
 
 // Subsets of 7 leaves grouped by size, increasing within a size (so the
-// subsets of n < 7 leaves are a prefix of each group), and binomials.
+// subsets of n < 7 leaves are a prefix of each group); copied into LDS by
+// every workgroup.
 __constant__ uint8_t c_sub[127] = {
     1, 2, 4, 8, 16, 32, 64, 3, 5, 6, 9, 10, 12, 17, 18, 20, 24, 33, 34, 36, 40, 48, 65, 66, 68, 72, 80, 96,
     7, 11, 13, 14, 19, 21, 22, 25, 26, 28, 35, 37, 38, 41, 42, 44, 49, 50, 52, 56, 67, 69, 70, 73, 74, 76, 81,
     82, 84, 88, 97, 98, 100, 104, 112, 15, 23, 27, 29, 30, 39, 43, 45, 46, 51, 53, 54, 57, 58, 60, 71, 75, 77,
     78, 83, 85, 86, 89, 90, 92, 99, 101, 102, 105, 106, 108, 113, 114, 116, 120, 31, 47, 55, 59, 61, 62, 79,
     87, 91, 93, 94, 103, 107, 109, 110, 115, 117, 118, 121, 122, 124, 63, 95, 111, 119, 123, 125, 126, 127};
-__constant__ uint8_t c_subk[9] = {0, 0, 7, 28, 63, 98, 119, 126, 127};
-__constant__ uint8_t c_binom[8][8] = {{1, 0, 0, 0, 0, 0, 0, 0},    {1, 1, 0, 0, 0, 0, 0, 0},
-                                      {1, 2, 1, 0, 0, 0, 0, 0},    {1, 3, 3, 1, 0, 0, 0, 0},
-                                      {1, 4, 6, 4, 1, 0, 0, 0},    {1, 5, 10, 10, 5, 1, 0, 0},
-                                      {1, 6, 15, 20, 15, 6, 1, 0}, {1, 7, 21, 35, 35, 21, 7, 1}};
-
 // The t-th (0-based) nonzero submask of `mask` in increasing order: the
 // reference's partition enumeration p -> (p - delta) & s visits exactly these.
 __device__ __forceinline__ int submask(int t, int mask) {
@@ -248,19 +246,31 @@ __device__ __forceinline__ int submask(int t, int mask) {
 // split its partitions by enumeration position and reduce by (cost,
 // position) -- the reference's first-minimum rule; with L = 1 a lane takes
 // whole subsets.
-__device__ void dp_group(Slot& sl, int sub, int G) {
-    const int n = sl.n;
-    Box lb[kTMax];
+// Binomial C(n, k) for n <= 7 and the first index of the k-leaf subsets in
+// c_sub, in ALU: per-lane reads of constant tables are vector loads whose
+// vmcnt wait would also drain the previous round's outstanding node stores.
+__device__ __forceinline__ int binom7(int n, int k) {
+    int c = 1;
 #pragma unroll
-    for (int i = 0; i < kTMax; ++i) lb[i] = lds_box(sl.lb[i]);
-    if (sub < n) sl.cost[1 << sub] = 0.0f;
+    for (int i = 1; i <= kTMax; ++i)
+        if (i <= k) c = c * (n - k + i) / i;  // exact at every step
+    return c;
+}
+__device__ __forceinline__ int subk7(int k) {  // sum of C(7, j) for 1 <= j < k
+    return k <= 1 ? 0 : k == 2 ? 7 : k == 3 ? 28 : k == 4 ? 63 : k == 5 ? 98 : k == 6 ? 119 : k == 7 ? 126 : 127;
+}
+
+__device__ void dp_group(Slot& sl, int sub, int G, const uint8_t* s_sub) {
+    const int n = sl.n;
+    if (sub < n) sl.cost1[(1 << sub) - 1] = 0.0f;
     wave_sync();
     for (int k = 2; k <= n; ++k) {
-        const int cnt = c_binom[n][k];
+        const int cnt = binom7(n, k);
+        const int subk = subk7(k);
         int L = 1;
         while (L * 2 * cnt <= G) L *= 2;
         if (L == 1) {
-            for (int q = sub; q < cnt; q += G) dp_subset(sl, lb, c_sub[c_subk[k] + q]);
+            for (int q = sub; q < cnt; q += G) dp_subset(sl, s_sub[subk + q]);
         } else {
             const int q = sub / L, j = sub & (L - 1);
             float best = kCostInf;
@@ -268,13 +278,13 @@ __device__ void dp_group(Slot& sl, int sub, int G) {
             int s = 0;
             float areaS = 0.0f;
             if (q < cnt) {
-                s = c_sub[c_subk[k] + q];
-                areaS = area(subset_box([&](int i) { return lb[i]; }, s));
+                s = s_sub[subk + q];
+                areaS = area(subset_box(sl, s));
                 const int delta = (s - 1) & s;
                 const int m = (1 << __popc(delta)) - 1;
                 for (int t = j; t < m; t += L) {
                     const int p = submask(t, delta);
-                    const float c = sl.cost[p] + sl.cost[s ^ p];
+                    const float c = sl.cost1[p - 1] + sl.cost1[(s ^ p) - 1];
                     if (c < best) {
                         best = c;
                         bestP = p;
@@ -293,7 +303,7 @@ __device__ void dp_group(Slot& sl, int sub, int G) {
                 }
             }
             if (q < cnt && j == 0) {
-                sl.cost[s] = best + 1.0f * areaS;
+                sl.cost1[s - 1] = best + 1.0f * areaS;
                 sl.part[s] = (uint8_t)bestP;
             }
         }
@@ -331,7 +341,7 @@ __device__ void rebuild(const TreeletArgs& a, Slot& sl, int32_t root) {
     constexpr int kI = kTMax - 1;  // internal nodes of a full treelet
     DNode* __restrict__ nodes = a.nodes;
     const int n = sl.n;
-    const int nexp = sl.nexp;  // n - 1
+    const int nexp = n - 1;
     const int full = (1 << n) - 1;
     RegTab act = 0, en = INT32_MIN, el = 0, er = 0;
 #pragma unroll
@@ -436,6 +446,9 @@ __global__ __launch_bounds__(kTBlock) void k_treelet(TreeletArgs a) {
     constexpr int kTItems = ITEMS;
     constexpr int kTChunk = kTBlock * ITEMS;
     __shared__ WaveLds s_w[kTWaves];
+    __shared__ uint8_t s_sub[127];
+    if (threadIdx.x < 127) s_sub[threadIdx.x] = c_sub[threadIdx.x];
+    __syncthreads();
     const int lane = threadIdx.x % kWave;
     WaveLds& w = s_w[threadIdx.x / kWave];
     const uint32_t n = a.n;
@@ -506,7 +519,7 @@ __global__ __launch_bounds__(kTBlock) void k_treelet(TreeletArgs a) {
             const int T = __popcll(tmask) < kGroups ? __popcll(tmask) : kGroups;
             const int G = T == 1 ? 64 : (T == 2 ? 32 : (T <= 4 ? 16 : 8));
             const int gi = lane / G, sub = lane & (G - 1);
-            if (gi < T && w.slot[gi].n >= 2) dp_group(w.slot[gi], sub, G);
+            if (gi < T && w.slot[gi].n >= 2) dp_group(w.slot[gi], sub, G, s_sub);
             wave_sync();
             if (mine) {
                 if (tn >= 2) rebuild(a, ms, (int32_t)root);

@@ -89,21 +89,27 @@ __device__ __forceinline__ Box node_box(const DNode* nodes, uint32_t i) {
 
 // Node record written with agent-scope atomic stores (coherent across XCDs
 // without flushing the L2), and the matching loads.
+// 64-bit halves of a record: agent-scope relaxed atomic stores of 8 bytes
+// each (global_store_dwordx2 sc1), half the instructions of 32-bit ones.
+__device__ __forceinline__ uint64_t pack2f(float a, float b) {
+    return (uint64_t)__float_as_uint(a) | ((uint64_t)__float_as_uint(b) << 32);
+}
+__device__ __forceinline__ void store_u64_coherent(uint64_t* p, uint64_t v) {
+    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// PAD = false leaves the record's last 8 (padding) bytes alone: for records
+// that already exist with zero padding (treelet restructuring reassigns
+// existing internal nodes).
+template <bool PAD = true>
 __device__ __forceinline__ void store_node_coherent(DNode* nodes, uint32_t i, const Box& b, int32_t l, int32_t r) {
-    float* f = reinterpret_cast<float*>(nodes + i);
-    int32_t* q = reinterpret_cast<int32_t*>(nodes + i);
-    __hip_atomic_store(f + 0, b.mnx, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    __hip_atomic_store(f + 1, b.mny, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    __hip_atomic_store(f + 2, b.mnz, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    __hip_atomic_store(f + 3, 0.0f, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    __hip_atomic_store(f + 4, b.mxx, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    __hip_atomic_store(f + 5, b.mxy, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    __hip_atomic_store(f + 6, b.mxz, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    __hip_atomic_store(f + 7, 0.0f, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    __hip_atomic_store(q + 8, l, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    __hip_atomic_store(q + 9, r, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    q[10] = 0;
-    q[11] = 0;
+    uint64_t* q = reinterpret_cast<uint64_t*>(nodes + i);
+    store_u64_coherent(q + 0, pack2f(b.mnx, b.mny));
+    store_u64_coherent(q + 1, pack2f(b.mnz, 0.0f));
+    store_u64_coherent(q + 2, pack2f(b.mxx, b.mxy));
+    store_u64_coherent(q + 3, pack2f(b.mxz, 0.0f));
+    store_u64_coherent(q + 4, (uint64_t)(uint32_t)l | ((uint64_t)(uint32_t)r << 32));
+    if (PAD) q[5] = 0;
 }
 
 __device__ __forceinline__ Box load_box_coherent(const DNode* nodes, uint32_t i) {
@@ -146,15 +152,11 @@ __device__ __forceinline__ void store_u32_coherent(uint32_t* p, uint32_t v) {
     __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 __device__ __forceinline__ void store_box_coherent(DNode* nodes, uint32_t i, const Box& b) {
-    float* f = reinterpret_cast<float*>(nodes + i);
-    __hip_atomic_store(f + 0, b.mnx, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    __hip_atomic_store(f + 1, b.mny, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    __hip_atomic_store(f + 2, b.mnz, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    __hip_atomic_store(f + 3, 0.0f, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    __hip_atomic_store(f + 4, b.mxx, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    __hip_atomic_store(f + 5, b.mxy, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    __hip_atomic_store(f + 6, b.mxz, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    __hip_atomic_store(f + 7, 0.0f, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    uint64_t* q = reinterpret_cast<uint64_t*>(nodes + i);
+    store_u64_coherent(q + 0, pack2f(b.mnx, b.mny));
+    store_u64_coherent(q + 1, pack2f(b.mnz, 0.0f));
+    store_u64_coherent(q + 2, pack2f(b.mxx, b.mxy));
+    store_u64_coherent(q + 3, pack2f(b.mxz, 0.0f));
 }
 
 

@@ -434,7 +434,7 @@ __device__ void rebuild(const TreeletArgs& a, Slot& sl, int32_t root) {
             }
         // Coherent stores: an ancestor's treelet may be formed by another
         // workgroup on another XCD later in this launch.
-        store_node_coherent(nodes, (uint32_t)cur, b, ln, rn);
+        store_node_coherent<false>(nodes, (uint32_t)cur, b, ln, rn);  // an existing internal node
         store_u32_coherent(a.parents + ln, (uint32_t)cur);
         store_u32_coherent(a.parents + rn, (uint32_t)cur);
         store_int2_coherent(a.ranges + cur, make_int2(lo, hi));

@@ -56,8 +56,6 @@ struct Slot {
     float cost1[127];        // cost of leaf subset s (1..127) at cost1[s - 1]
     uint8_t part[128];
     int32_t act[kTMax];      // treelet leaves (node ids)
-    int32_t lc[kTMax];       // their children (when internal)
-    int32_t rc[kTMax];
     int32_t lo[kTMax];       // their sorted-leaf spans
     int32_t hi[kTMax];
     int32_t exn[kTMax - 1];  // expanded internal nodes and their children, in order
@@ -132,66 +130,97 @@ __device__ __forceinline__ Box subset_box(const Slot& sl, int s) {
     return b;
 }
 
+// Small per-lane tables (formation, reconstruction) as 8-wide vector values: they
+// live in VGPRs and lane-dependent lookups become selects (a plain local
+// array would be turned into an indexed load and land in scratch or LDS).
+typedef int32_t RegTab __attribute__((ext_vector_type(8)));
+
+// The entry whose bit is set in the one-hot mask.
+__device__ __forceinline__ int32_t reg_pick(RegTab v, uint32_t onehot) {
+    int32_t r = v[0];
+#pragma unroll
+    for (int k = 1; k < 8; ++k) r = ((onehot >> k) & 1u) ? v[k] : r;
+    return r;
+}
+__device__ __forceinline__ RegTab reg_put(RegTab v, uint32_t onehot, int32_t x) {
+#pragma unroll
+    for (int k = 0; k < 8; ++k) v[k] = ((onehot >> k) & 1u) ? x : v[k];
+    return v;
+}
+
 // Greedy formation (reference lbvh_plus_builder.cu:79-99): expand the
 // treelet leaf of largest area that is not a BVH leaf (first maximum wins)
 // until 7 leaves.  Each act leaf's children come with its node record, so
-// an expansion costs one round of loads.  Returns the refitted root box
-// (the reference's union of the root's children) and the leaf count.
+// an expansion costs one round of loads.  The leaves' areas and child ids
+// are kept in registers as well (the argmax and the next expansion's loads
+// need no LDS round trip); LDS gets what the DP and the rebuild read.
+// Returns the refitted root box (the reference's union of the root's
+// children) and the leaf count.
 template <bool COH>
 __device__ int form(const TreeletArgs& a, int32_t root, Slot& sl, Box& rootbox) {
+    typedef float AreaTab __attribute__((ext_vector_type(8)));
     const DNode* __restrict__ nodes = a.nodes;
     const int4 rlr = node_rec<COH>(nodes, root).lr;
     const int2 rsp = COH ? load_int2_coherent(a.ranges + root) : a.ranges[root];
     sl.act[0] = root;
-    sl.lc[0] = rlr.x;
-    sl.rc[0] = rlr.y;
     sl.lo[0] = rsp.x;
     sl.hi[0] = rsp.y;
+    RegTab lcv = 0, rcv = 0;
+    AreaTab ar = -1.0f;
+    lcv[0] = rlr.x;
+    rcv[0] = rlr.y;
     uint32_t leafmask = 0;
     int n = 1, nexp = 0;
     DNode cl = node_rec<COH>(nodes, rlr.x);
     DNode cr = node_rec<COH>(nodes, rlr.y);
     rootbox = merge(node_box_of(cl), node_box_of(cr));
     lds_put(sl.lb[0], rootbox);
+    ar[0] = area(rootbox);
     int best = 0;
     for (int step = 0; step < kTMax - 1; ++step) {
         if (step > 0) {
             best = -1;
             float maxA = -1.0f;
-            for (int i = 0; i < n; ++i) {
-                if (!((leafmask >> i) & 1u)) {
-                    const float ar = area(lds_box(sl.lb[i]));
-                    if (ar > maxA) {
-                        maxA = ar;
-                        best = i;
-                    }
+#pragma unroll
+            for (int i = 0; i < kTMax; ++i) {
+                if (i < n && !((leafmask >> i) & 1u) && ar[i] > maxA) {
+                    maxA = ar[i];
+                    best = i;
                 }
             }
             if (best < 0) break;
-            cl = node_rec<COH>(nodes, sl.lc[best]);
-            cr = node_rec<COH>(nodes, sl.rc[best]);
-        } else if (!(area(rootbox) > -1.0f)) {
+            cl = node_rec<COH>(nodes, reg_pick(lcv, 1u << best));
+            cr = node_rec<COH>(nodes, reg_pick(rcv, 1u << best));
+        } else if (!(ar[0] > -1.0f)) {
             break;  // the reference's argmax finds nothing (non-finite area)
         }
-        const int32_t l = sl.lc[best], r = sl.rc[best];
+        const uint32_t bbit = 1u << best, nbit = 1u << n;
+        const int32_t l = reg_pick(lcv, bbit), r = reg_pick(rcv, bbit);
         const int2 sp_l = leaf_span<COH>(a, l), sp_r = leaf_span<COH>(a, r);
         sl.exn[nexp] = sl.act[best];
         sl.exl[nexp] = l;
         sl.exr[nexp] = r;
         ++nexp;
+        const Box bl = node_box_of(cl), br = node_box_of(cr);
         sl.act[best] = l;
-        lds_put(sl.lb[best], node_box_of(cl));
-        sl.lc[best] = cl.lr.x;
-        sl.rc[best] = cl.lr.y;
+        lds_put(sl.lb[best], bl);
         sl.lo[best] = sp_l.x;
         sl.hi[best] = sp_l.y;
-        leafmask = (leafmask & ~(1u << best)) | ((cl.lr.x < 0 ? 1u : 0u) << best);
+        lcv = reg_put(lcv, bbit, cl.lr.x);
+        rcv = reg_put(rcv, bbit, cl.lr.y);
+        const float al = area(bl), arr = area(br);
+#pragma unroll
+        for (int k = 0; k < kTMax; ++k) {
+            ar[k] = ((bbit >> k) & 1u) ? al : ar[k];
+            ar[k] = ((nbit >> k) & 1u) ? arr : ar[k];
+        }
+        leafmask = (leafmask & ~bbit) | ((cl.lr.x < 0 ? 1u : 0u) << best);
         sl.act[n] = r;
-        lds_put(sl.lb[n], node_box_of(cr));
-        sl.lc[n] = cr.lr.x;
-        sl.rc[n] = cr.lr.y;
+        lds_put(sl.lb[n], br);
         sl.lo[n] = sp_r.x;
         sl.hi[n] = sp_r.y;
+        lcv = reg_put(lcv, nbit, cr.lr.x);
+        rcv = reg_put(rcv, nbit, cr.lr.y);
         leafmask |= (cr.lr.x < 0 ? 1u : 0u) << n;
         ++n;
     }
@@ -309,24 +338,6 @@ __device__ void dp_group(Slot& sl, int sub, int G, const uint8_t* s_sub) {
         }
         wave_sync();
     }
-}
-
-// Small per-lane tables of the reconstruction as 8-wide vector values: they
-// live in VGPRs and lane-dependent lookups become selects (a plain local
-// array would be turned into an indexed load and land in scratch or LDS).
-typedef int32_t RegTab __attribute__((ext_vector_type(8)));
-
-// The entry whose bit is set in the one-hot mask.
-__device__ __forceinline__ int32_t reg_pick(RegTab v, uint32_t onehot) {
-    int32_t r = v[0];
-#pragma unroll
-    for (int k = 1; k < 8; ++k) r = ((onehot >> k) & 1u) ? v[k] : r;
-    return r;
-}
-__device__ __forceinline__ RegTab reg_put(RegTab v, uint32_t onehot, int32_t x) {
-#pragma unroll
-    for (int k = 0; k < 8; ++k) v[k] = ((onehot >> k) & 1u) ? x : v[k];
-    return v;
 }
 
 // Reconstruction (reference lbvh_plus_builder.cu:162-245): the treelet's

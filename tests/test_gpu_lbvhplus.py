@@ -42,6 +42,22 @@ def test_lbvh_plus_rounds(ctx, rounds):
     _check(ctx, B.Mesh.blobs(21, 200_000), rounds)
 
 
+def _signed_zero_mesh(seed, n):
+    rng = np.random.default_rng(seed)
+    soa = rng.random((9, n), dtype=np.float32)
+    soa[2::3] = 0.0                      # flat in z
+    soa[2, : n // 2] = -0.0              # mixed signed zeros
+    soa[0, ::7] = -0.0
+    soa[3, ::7] = 0.0
+    return B.Mesh(soa)
+
+
+def test_lbvh_plus_signed_zero(ctx):
+    """Mixed signed zeros: treelet subset unions are select folds in leaf
+    order, so the zero signs of the optimised boxes match the oracle's bits."""
+    _check(ctx, _signed_zero_mesh(5, 20000), 2)
+
+
 def test_lbvh_plus_blobs_1m_three_rounds(ctx):
     _check(ctx, B.Mesh.blobs(7, 1_000_000), 3)
 

@@ -48,6 +48,20 @@ def test_ploc_blobs_1m_wide_radius(ctx, radius):
     _check_ploc(ctx, B.Mesh.blobs(11, 1_000_000), radius)
 
 
+@pytest.mark.parametrize("n,radius", [(20000, 25), (300_000, 25), (300_000, 100)])
+def test_ploc_signed_zero(ctx, n, radius):
+    """Mixed signed zeros and flat boxes: the neighbour search evaluates each
+    pair once from the lower cluster's side (300K clusters reach that path),
+    which is exact only if SA(a u b) and SA(b u a) have the same bits."""
+    rng = np.random.default_rng(n + radius)
+    soa = rng.random((9, n), dtype=np.float32)
+    soa[2::3] = 0.0
+    soa[2, : n // 2] = -0.0
+    soa[0, ::7] = -0.0
+    soa[3, ::7] = 0.0
+    _check_ploc(ctx, B.Mesh(soa), radius)
+
+
 def test_ploc_identical_triangles(ctx):
     n = 2600  # one merge per iteration: exercises ~550 global iterations and the LDS tail
     soa = np.tile(np.array([[0.1], [0.2], [0.3], [0.4], [0.2], [0.3], [0.1], [0.5], [0.3]], np.float32), (1, n))

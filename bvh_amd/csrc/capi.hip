@@ -535,9 +535,14 @@ int ploc_drive(bvh_ctx_t ctx, const PlocArgs& a, int* slot, uint32_t m_bound, ui
     std::memset(&hs, 0, sizeof(hs));
     hs.m = m_bound;
     hs.maxm = maxm;
+    // The first batch assumes the fastest measured decay; later ones the
+    // decay the previous batch showed (fewer state reads: each is a host
+    // round trip, while surplus iterations exit at once).
+    double ratio = 0.80;
     while (hs.maxm > (uint32_t)kPlocSmall && hs.flags == 0) {
-        int batch = ploc_iter_estimate(hs.maxm, 0.80);
+        int batch = ploc_iter_estimate(hs.maxm, ratio);
         batch = batch < 2 ? 2 : (batch > 48 ? 48 : batch);
+        const uint32_t before = hs.maxm;
         for (int b = 0; b < batch; ++b, k = (k + 1) % kPlocMaxIter) {
             launch_ploc_nn(a, k, hs.m, nn_grid, ctx->stream);
             launch_ploc_merge(a, k, hs.m, merge_grid, ctx->stream);
@@ -545,6 +550,10 @@ int ploc_drive(bvh_ctx_t ctx, const PlocArgs& a, int* slot, uint32_t m_bound, ui
         TRY(check_launch(ctx, "k_ploc_nn/merge"));
         HIP_TRY(ctx, hipMemcpyAsync(&hs, a.state + k, sizeof(hs), hipMemcpyDeviceToHost, ctx->stream));
         HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+        if (hs.maxm > (uint32_t)kPlocSmall && hs.maxm < before) {
+            const double r = std::pow((double)hs.maxm / (double)before, 1.0 / batch);
+            ratio = r < 0.5 ? 0.5 : (r > 0.97 ? 0.97 : r);
+        }
     }
     *slot = k;
     if (last) *last = hs;

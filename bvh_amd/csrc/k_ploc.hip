@@ -632,7 +632,11 @@ __global__ __launch_bounds__(kTailBlock) void k_ploc_tail(PlocArgs a, int k) {
     constexpr int PER = (kPlocSmall + kTailBlock - 1) / kTailBlock;
     __shared__ float s_b[6][kPlocSmall];
     __shared__ uint32_t s_id[kPlocSmall];
-    __shared__ int32_t s_nn[kPlocSmall];
+    // Packed neighbour minimum per cluster; the neighbour ids (s_nn) reuse its
+    // storage once the minima are read (keeps two workgroups per CU).
+    __shared__ uint64_t s_best[kPlocSmall];
+    int32_t* const s_nn = reinterpret_cast<int32_t*>(s_best);
+    static_assert(kPlocSmall == 2 * kTailBlock, "each thread converts at most two minima");
     __shared__ uint64_t s_w[kTailBlock / kWave];
     const uint32_t seg = blockIdx.x;
     const PlocState st = a.state[k];
@@ -661,30 +665,49 @@ __global__ __launch_bounds__(kTailBlock) void k_ploc_tail(PlocArgs a, int k) {
     while (m > (uint32_t)kWave) {
         // Lanes per cluster: as m shrinks, each window is split over up to 64
         // lanes (strided candidates, then a shuffle min over the lane group;
-        // the packed key's minimum is unique, so the split is exact).
+        // the packed key's minimum is unique, so the split is exact).  Each
+        // pair i < j of a window is evaluated once, by i's lanes (the distance
+        // is symmetric bit for bit, as in k_ploc_nn): (d, j) goes into i's
+        // minimum, (d, i) into j's through a 64-bit LDS atomic minimum.
         uint32_t L = 1;
         while (L < (uint32_t)kWave && m * (2 * L) <= (uint32_t)kTailBlock) L <<= 1;
         const uint32_t sub = threadIdx.x & (L - 1);
+        for (uint32_t i = threadIdx.x; i < m; i += kTailBlock) s_best[i] = ~0ull;
+        __syncthreads();
         for (uint32_t i = threadIdx.x / L; i < m; i += kTailBlock / L) {
             Box bi;
             bi.mnx = s_b[0][i]; bi.mny = s_b[1][i]; bi.mnz = s_b[2][i];
             bi.mxx = s_b[3][i]; bi.mxy = s_b[4][i]; bi.mxz = s_b[5][i];
-            const int lo = ((int)i - r > 0) ? (int)i - r : 0;
             const int hi = ((int)i + r < (int)m - 1) ? (int)i + r : (int)m - 1;
             uint64_t best = ~0ull;
-            for (int j = lo + (int)sub; j <= hi; j += (int)L) {
-                if (j == (int)i) continue;
+            for (int j = (int)i + 1 + (int)sub; j <= hi; j += (int)L) {
                 Box bj;
                 bj.mnx = s_b[0][j]; bj.mny = s_b[1][j]; bj.mnz = s_b[2][j];
                 bj.mxx = s_b[3][j]; bj.mxy = s_b[4][j]; bj.mxz = s_b[5][j];
-                uint64_t key = nn_key(area(merge(bi, bj)), (uint32_t)j);
-                best = key < best ? key : best;
+                const float dist = area(merge(bi, bj));
+                const uint64_t kf = nn_key(dist, (uint32_t)j);
+                best = kf < best ? kf : best;
+                const uint64_t kb = nn_key(dist, i);
+                if (kb != ~0ull)
+                    __hip_atomic_fetch_min(reinterpret_cast<unsigned long long*>(s_best + j), (unsigned long long)kb,
+                                           __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
             }
             for (uint32_t o = L >> 1; o >= 1; o >>= 1) {
                 const uint64_t other = __shfl_xor(best, (int)o, kWave);
                 best = other < best ? other : best;
             }
-            if (sub == 0) s_nn[i] = (best == ~0ull) ? -1 : (int32_t)(uint32_t)best;
+            if (sub == 0 && best != ~0ull)
+                __hip_atomic_fetch_min(reinterpret_cast<unsigned long long*>(s_best + i), (unsigned long long)best,
+                                       __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+        }
+        __syncthreads();
+        {
+            const uint32_t i0 = threadIdx.x, i1 = threadIdx.x + kTailBlock;
+            const uint64_t v0 = i0 < m ? s_best[i0] : ~0ull;
+            const uint64_t v1 = i1 < m ? s_best[i1] : ~0ull;
+            __syncthreads();  // every minimum read before s_nn overwrites them
+            if (i0 < m) s_nn[i0] = (v0 == ~0ull) ? -1 : (int32_t)(uint32_t)v0;
+            if (i1 < m) s_nn[i1] = (v1 == ~0ull) ? -1 : (int32_t)(uint32_t)v1;
         }
         __syncthreads();
         // Per thread PER consecutive slots; scan of packed (merges, survivors).

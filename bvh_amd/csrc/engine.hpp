@@ -142,6 +142,7 @@ struct HierArgs {
     uint32_t* ctl;          // 2 x hier_max_levels() words cleared by launch_init_build:
                             // per-level chunk tickets, then per-level unit counts
     int levels;             // chunked levels (hier_levels); a climbing pass finishes the rest
+    int fold1;              // set by launch_hierarchy: level 1 maps the leaf level's slots itself
 };
 int hier_chunk();
 int hier_unit_bytes();

@@ -99,6 +99,7 @@ struct HierSmem {
     uint32_t ul[kChunk], ur[kChunk];    // unit leaf ranges
     uint32_t us[kChunk];                // unit mesh
     uint32_t start[kChunk / 32];        // bit u: a segment starts at unit u
+    uint32_t src[kChunk];               // level 1 (first chunked level): slot of unit u in level 0's output
     uint32_t wsum[kHierThreads / kWave];
     uint32_t ticket, out_base;
 };
@@ -126,11 +127,14 @@ __device__ Box range_box(const HierSmem& s, int x, int y) {
     return merge(lo_acc, hi_acc);
 }
 
-// One chunk of one level; the block's ticket picks the chunk.  LEAF: level 1
-// (units are the sorted leaves of one mesh chunk).  Otherwise units come from
-// p.units_in.
+// One chunk of one level; the block's ticket picks the chunk.  LEAF: the
+// leaf level (units are the sorted leaves of one mesh chunk).  Otherwise
+// `total_in` units come from p.units_in, compacted, except for a folded level
+// 1 (p.fold1), which reads the leaf level's per-chunk slots (p.counts[c] exits
+// at c * kChunk) through a slot map it builds itself (no compaction pass).
 template <bool LEAF, bool K60>
-__device__ void hier_chunk_body(const HierArgs& p, HierSmem& s, uint32_t t, uint32_t nchunks, int level) {
+__device__ void hier_chunk_body(const HierArgs& p, HierSmem& s, uint32_t t, uint32_t nchunks, int level,
+                                uint32_t total_in = 0) {
     const int tid = threadIdx.x;
     const int u = tid;
     const uint32_t epoch = *p.epoch;
@@ -180,13 +184,37 @@ __device__ void hier_chunk_body(const HierArgs& p, HierSmem& s, uint32_t t, uint
         s.box[0][kChunk + u] = b.mnx; s.box[1][kChunk + u] = b.mny; s.box[2][kChunk + u] = b.mnz;
         s.box[3][kChunk + u] = b.mxx; s.box[4][kChunk + u] = b.mxy; s.box[5][kChunk + u] = b.mxz;
     } else {
-        const uint32_t cnt = p.ctl[kHierMaxLevels + level - 1];
+        const uint32_t cnt = total_in;
         const uint32_t base = t * kChunk;
         m = (cnt - base < (uint32_t)kChunk) ? cnt - base : (uint32_t)kChunk;
+        if (level == 1 && p.fold1) {
+            // Slots of units [base, base + m): each thread owns a run of
+            // leaf-level chunks; an exclusive scan of their exit counts gives
+            // the run's first unit index.
+            const uint32_t nch = p.sb.nblocks;
+            const uint32_t cp = (nch + kHierThreads - 1) / kHierThreads;
+            const uint32_t c0 = (uint32_t)tid * cp < nch ? (uint32_t)tid * cp : nch;
+            const uint32_t c1 = c0 + cp < nch ? c0 + cp : nch;
+            uint32_t run = 0;
+            for (uint32_t c = c0; c < c1; ++c) run += p.counts[c];
+            const uint32_t incl = wave_incl_sum(run);
+            if (lane_id() == kWave - 1) s.wsum[tid / kWave] = incl;
+            __syncthreads();
+            uint32_t off = incl - run;
+            for (int w = 0; w < tid / kWave; ++w) off += s.wsum[w];
+            for (uint32_t c = c0; c < c1 && off < base + m; ++c) {
+                const uint32_t cc = p.counts[c];
+                const uint32_t lo = off > base ? off : base;
+                const uint32_t hi = (off + cc < base + m) ? off + cc : base + m;
+                for (uint32_t g = lo; g < hi; ++g) s.src[g - base] = c * kChunk + (g - off);
+                off += cc;
+            }
+            __syncthreads();
+        }
         float4 w0 = make_float4(kBig, kBig, kBig, -kBig), w1 = make_float4(-kBig, -kBig, 0.0f, 0.0f);
         int dr = -1;
         if ((uint32_t)u < m) {
-            const HUnit* src = p.units_in + base + u;
+            const HUnit* src = p.units_in + ((level == 1 && p.fold1) ? s.src[u] : base + u);
             const uint4 h = *reinterpret_cast<const uint4*>(src);
             w0 = reinterpret_cast<const float4*>(src)[1];
             w1 = reinterpret_cast<const float4*>(src)[2];
@@ -319,8 +347,9 @@ __device__ void hier_chunk_body(const HierArgs& p, HierSmem& s, uint32_t t, uint
     const uint32_t rank = before + (uint32_t)__popcll(bal & lanemask_lt());
 
     if (LEAF) {
-        // Level 1: exits go to the chunk's own slots; k_hier_gather compacts
-        // them, so no level-1 block waits for another.
+        // Leaf level: exits go to the chunk's own slots (level 1 maps them
+        // itself; with no chunked level above, k_hier_gather compacts them),
+        // so no leaf-level block waits for another.
         if (tid == 0) {
             p.counts[t] = total;
             s.out_base = t * kChunk;
@@ -385,6 +414,7 @@ __device__ void hier_chunk_body(const HierArgs& p, HierSmem& s, uint32_t t, uint
 // one ordered sequence.  Block b owns chunks [b*kGather, (b+1)*kGather): it
 // sums the counts of all earlier chunks, scans its own, and copies.
 constexpr int kGather = 64;
+constexpr uint32_t kFoldMaxChunks = 8192;  // leaf chunks up to which level 1 maps the slots itself
 __global__ __launch_bounds__(kHierThreads) void k_hier_gather(HierArgs p) {
     __shared__ uint32_t s_off[kGather + 1];
     __shared__ uint32_t s_w[kHierThreads / kWave];
@@ -432,12 +462,27 @@ __global__ __launch_bounds__(kHierThreads) void k_hier_gather(HierArgs p) {
 template <bool LEAF, bool K60>
 __global__ __launch_bounds__(kHierThreads) void k_hier_level(HierArgs p, int level) {
     __shared__ HierSmem s;
-    const uint32_t nchunks =
-        LEAF ? p.sb.nblocks : (p.ctl[kHierMaxLevels + level - 1] + kChunk - 1) / (uint32_t)kChunk;
     if (LEAF) {  // one block per chunk, no ordering needed
-        hier_chunk_body<LEAF, K60>(p, s, blockIdx.x, nchunks, level);
+        hier_chunk_body<LEAF, K60>(p, s, blockIdx.x, p.sb.nblocks, level);
         return;
     }
+    // Units entering this level: a folded level 1 sums the leaf level's exit
+    // counts.
+    uint32_t total_in;
+    if (level == 1 && p.fold1) {
+        uint32_t acc = 0;
+        for (uint32_t c = threadIdx.x; c < p.sb.nblocks; c += kHierThreads) acc += p.counts[c];
+#pragma unroll
+        for (int o = 32; o >= 1; o >>= 1) acc += __shfl_xor(acc, o, kWave);
+        if (lane_id() == 0) s.wsum[threadIdx.x / kWave] = acc;
+        __syncthreads();
+        total_in = 0;
+        for (int w = 0; w < kHierThreads / kWave; ++w) total_in += s.wsum[w];
+        __syncthreads();
+    } else {
+        total_in = p.ctl[kHierMaxLevels + level - 1];
+    }
+    const uint32_t nchunks = (total_in + kChunk - 1) / (uint32_t)kChunk;
     // Later levels take chunks in ticket order (the look-back needs every
     // smaller ticket to be running or done).
     while (true) {
@@ -445,7 +490,7 @@ __global__ __launch_bounds__(kHierThreads) void k_hier_level(HierArgs p, int lev
         __syncthreads();
         const uint32_t t = s.ticket;
         if (t >= nchunks) break;
-        hier_chunk_body<LEAF, K60>(p, s, t, nchunks, level);
+        hier_chunk_body<LEAF, K60>(p, s, t, nchunks, level, total_in);
         __syncthreads();
     }
 }
@@ -535,23 +580,31 @@ int hier_levels(uint32_t maxn, uint32_t meshes) {
 void launch_hierarchy(const HierArgs& a0, hipStream_t s) {
     HierArgs a = a0;
     const bool k60 = a.vals != nullptr;
-    // Level 1 writes per-chunk slots of units[0]; k_hier_gather compacts them
-    // into units[1]; level lv >= 1 reads units[lv & 1] and writes the other.
+    // The leaf level writes per-chunk slots of units[0].  Folded (a chunked
+    // level above it, at most kFoldMaxChunks leaf chunks): level 1 maps the
+    // slots itself and writes units[1], level lv reads units[(lv + 1) & 1] and
+    // writes units[lv & 1].  Otherwise k_hier_gather compacts the slots into
+    // units[1] and level lv reads units[lv & 1] and writes the other.  (Every
+    // folded level-1 block scans all leaf chunks' counts, which costs more
+    // than the compaction pass beyond ~8M leaves.)
+    a.fold1 = (a.levels > 1 && a.sb.nblocks <= kFoldMaxChunks) ? 1 : 0;
     a.units_in = nullptr;
     a.units_out = a.units[0];
     if (k60) hipLaunchKernelGGL((k_hier_level<true, true>), dim3(a.sb.nblocks), dim3(kHierThreads), 0, s, a, 0);
     else hipLaunchKernelGGL((k_hier_level<true, false>), dim3(a.sb.nblocks), dim3(kHierThreads), 0, s, a, 0);
-    hipLaunchKernelGGL(k_hier_gather, dim3((a.sb.nblocks + kGather - 1) / kGather), dim3(kHierThreads), 0, s, a);
+    if (!a.fold1)
+        hipLaunchKernelGGL(k_hier_gather, dim3((a.sb.nblocks + kGather - 1) / kGather), dim3(kHierThreads), 0, s, a);
     uint32_t grid = a.sb.nblocks;
     for (int lv = 1; lv < a.levels; ++lv) {
-        a.units_in = a.units[lv & 1];
-        a.units_out = a.units[(lv + 1) & 1];
+        a.units_in = a.fold1 ? a.units[(lv + 1) & 1] : a.units[lv & 1];
+        a.units_out = a.fold1 ? a.units[lv & 1] : a.units[(lv + 1) & 1];
         grid = (grid + 31) / 32;
         if (grid < 1) grid = 1;
         if (k60) hipLaunchKernelGGL((k_hier_level<false, true>), dim3(grid), dim3(kHierThreads), 0, s, a, lv);
         else hipLaunchKernelGGL((k_hier_level<false, false>), dim3(grid), dim3(kHierThreads), 0, s, a, lv);
     }
-    a.units_in = a.units[a.levels & 1];
+    // The climb reads the last level's output.
+    a.units_in = a.fold1 ? a.units[(a.levels - 1) & 1] : a.units[a.levels & 1];
     uint32_t cgrid = (a.sb.nblocks + 63) / 64;
     cgrid = cgrid < 1 ? 1 : (cgrid > 512 ? 512 : cgrid);
     if (k60) hipLaunchKernelGGL(k_hier_climb<true>, dim3(cgrid), dim3(256), 0, s, a, a.levels - 1);

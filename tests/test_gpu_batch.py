@@ -185,6 +185,25 @@ def test_build_batched_eager_wide_sort(ctx, algo):
     _assert_identical([got], ref)
 
 
+def test_build_batched_eager_large_lbvh(ctx):
+    # Nine 1M-triangle meshes (8,793 leaf chunks): the hierarchy compacts the
+    # leaf level in a separate pass above 8,192 chunks, while each mesh alone
+    # takes the folded level 1; both must give the same trees.
+    meshes = [B.Mesh.batch_mesh(43, i, 1_000_000) for i in range(9)]
+    dms = [ctx.upload(m) for m in meshes]
+    trees = [ctx.tree() for _ in meshes]
+    try:
+        ctx.build_batched(dms, trees, algo=B.ALGO_LBVH)
+        got = [tr.download() for tr in trees]
+        ref = [_standalone(ctx, dm, B.ALGO_LBVH, 16) for dm in dms]
+    finally:
+        for tr in trees:
+            tr.free()
+        for d in dms:
+            d.free()
+    _assert_identical([got], ref)
+
+
 @pytest.mark.parametrize("algo", [B.ALGO_LBVH, B.ALGO_PLOC])
 def test_batch_many_tiny_meshes(ctx, algo):
     # 3000 meshes of 1-40 triangles: 12 mesh-id bits in the keys, thousands of

@@ -352,8 +352,9 @@ def main():
     avg_launch_ms = dom["ms"] / dom["launches"]
     achieved = dom["unit"] * mesh.n / (avg_launch_ms * 1e-3) / 1e9
     traffic = pmc_traffic(dom_name, mesh.n)
-    # k_hier is the hierarchy stage: k_hier_level<true,false> (level 1), k_hier_gather,
-    # k_hier_level<false,false> per later level and k_hier_climb, one stage per build.
+    # k_hier is the hierarchy stage: k_hier_level<true,false> (level 1),
+    # k_hier_level<false,false> per later level (k_hier_gather between them above
+    # 8M leaves) and k_hier_climb, one stage per build.
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic, "kernel": dom_name,
                 "avg_launch_ms": round(avg_launch_ms, 4), "bytes_per_launch": dom["unit"] * mesh.n,

@@ -101,7 +101,7 @@ struct bvh_tree_rec {
 
 // A build job: the segments and how kernel blocks map onto them.
 struct Job {
-    SegBlocks fsb;        // front blocks (kFrontChunk triangles each)
+    SegBlocks fsb;        // front blocks (fsb.chunk triangles each)
     SegBlocks hsb;        // hierarchy blocks (hier_chunk() leaves each)
     uint32_t k = 1;       // segments
     uint32_t total = 0;   // triangles over all segments
@@ -279,7 +279,8 @@ Job single_job(const bvh_mesh_rec* mesh, bvh_tree_rec* tree, bool m60) {
     j.fsb.segs = nullptr;
     j.fsb.blocks = nullptr;
     j.fsb.single = seg_of(mesh, tree, 0);
-    j.fsb.nblocks = cdiv(mesh->n, kFrontChunk);
+    j.fsb.chunk = mesh->n >= kFrontLargeMin ? kFrontChunkLarge : kFrontChunk;
+    j.fsb.nblocks = cdiv(mesh->n, j.fsb.chunk);
     j.hsb = j.fsb;
     j.hsb.nblocks = cdiv(mesh->n, (uint32_t)hier_chunk());
     j.k = 1;
@@ -685,6 +686,7 @@ int plan_job(bvh_ctx_t ctx, SegTables& t, const bvh_mesh_t* meshes, const bvh_tr
     j.fsb.blocks = P<uint2>(t.fblocks);
     std::memset(&j.fsb.single, 0, sizeof(j.fsb.single));
     j.fsb.nblocks = (uint32_t)fb.size();
+    j.fsb.chunk = kFrontChunk;
     j.hsb = j.fsb;
     j.hsb.blocks = P<uint2>(t.hblocks);
     j.hsb.nblocks = (uint32_t)hb.size();

@@ -34,6 +34,7 @@ struct SegBlocks {
     const uint2* blocks;
     SegDesc single;
     uint32_t nblocks;
+    uint32_t chunk;  // elements per chunk for the front kernels and PLOC init (unused by the hierarchy)
 };
 
 // Packed 64-bit sort key of LBVH/PLOC: seg << seg_shift | morton30 << idx_bits | index.
@@ -44,7 +45,14 @@ struct KeyFormat {
 };
 
 // ---- LBVH/LBVH+/PLOC front end (k_front.hip) ----------------------------
-constexpr uint32_t kFrontChunk = 4096;   // triangles per block of the front kernels
+// Triangles per block of the front kernels: kFrontChunk, or kFrontChunkLarge
+// for a single mesh of at least kFrontLargeMin triangles (fewer blocks, so
+// fewer scene and digit-total atomics; at 4M blob triangles bounds + Morton
+// take 103 us instead of 114, while below ~1M the smaller chunk keeps more CUs
+// busy).
+constexpr uint32_t kFrontChunk = 4096;
+constexpr uint32_t kFrontChunkLarge = 8192;
+constexpr uint32_t kFrontLargeMin = 2000000;
 constexpr int kFrontUnroll = 4;          // triangles per thread whose loads are in flight together
 
 // Resets the per-segment scene words (8 per segment), clears hist_words words
@@ -205,7 +213,7 @@ struct PlocState {
 };
 
 struct PlocArgs {
-    SegBlocks sb;       // meshes; blocks over kFrontChunk chunks (for the init kernel)
+    SegBlocks sb;       // meshes; blocks over sb.chunk-triangle chunks (for the init kernel)
     uint32_t nseg;      // meshes
     int radius;
     uint32_t m0, maxm0; // total and largest triangle count

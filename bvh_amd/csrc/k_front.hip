@@ -24,7 +24,7 @@
 
 namespace bvhk {
 
-constexpr int kFrontBlock = 256;
+constexpr int kFrontBlock = 512;
 
 __device__ __forceinline__ void block_segment(const SegBlocks& sb, uint32_t& seg, uint32_t& chunk, SegDesc& d) {
     if (sb.blocks) {
@@ -75,8 +75,8 @@ __global__ __launch_bounds__(kFrontBlock) void k_tri_bounds(SegBlocks sb, float4
     uint32_t omn[3] = {0xFFFFFFFFu, 0xFFFFFFFFu, 0xFFFFFFFFu};
     uint32_t omx[3] = {0u, 0u, 0u};
     uint32_t bad = 0;
-    const uint32_t lo = chunk * kFrontChunk;
-    const uint32_t hi = (lo + kFrontChunk < d.n) ? lo + kFrontChunk : d.n;
+    const uint32_t lo = chunk * sb.chunk;
+    const uint32_t hi = (lo + sb.chunk < d.n) ? lo + sb.chunk : d.n;
     float4* __restrict__ out = tri_boxes + 2 * (size_t)d.start;
     const float* __restrict__ v[9];
 #pragma unroll
@@ -203,8 +203,8 @@ __global__ __launch_bounds__(kFrontBlock) void k_morton(SegBlocks sb, const floa
     const float smxx = ord2f(sc[3]), smxy = ord2f(sc[4]), smxz = ord2f(sc[5]);
     const uint64_t segbits = (uint64_t)seg << kf.seg_shift;
     __syncthreads();
-    const uint32_t lo = chunk * kFrontChunk;
-    const uint32_t hi = (lo + kFrontChunk < d.n) ? lo + kFrontChunk : d.n;
+    const uint32_t lo = chunk * sb.chunk;
+    const uint32_t hi = (lo + sb.chunk < d.n) ? lo + sb.chunk : d.n;
     const float4* __restrict__ bx = tri_boxes + 2 * (size_t)d.start;
     for (uint32_t i0 = lo + threadIdx.x; i0 < hi; i0 += kFrontUnroll * kFrontBlock) {
         Box bq[kFrontUnroll];
@@ -279,8 +279,8 @@ __global__ __launch_bounds__(kFrontBlock) void k_morton_codes(SegBlocks sb, cons
     const uint32_t* sc = seg_scene + 8 * (size_t)seg;
     const float smnx = ord2f(sc[0]), smny = ord2f(sc[1]), smnz = ord2f(sc[2]);
     const float smxx = ord2f(sc[3]), smxy = ord2f(sc[4]), smxz = ord2f(sc[5]);
-    const uint32_t lo = chunk * kFrontChunk;
-    const uint32_t hi = (lo + kFrontChunk < d.n) ? lo + kFrontChunk : d.n;
+    const uint32_t lo = chunk * sb.chunk;
+    const uint32_t hi = (lo + sb.chunk < d.n) ? lo + sb.chunk : d.n;
     for (uint32_t i = lo + threadIdx.x; i < hi; i += kFrontBlock) {
         Box b = load_box(tri_boxes + 2 * ((size_t)d.start + i));
         float x = norm_coord(b.mnx, b.mxx, smnx, smxx);

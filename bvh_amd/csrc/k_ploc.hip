@@ -130,8 +130,8 @@ __global__ __launch_bounds__(256) void k_ploc_init(PlocArgs a, const uint64_t* _
     const uint32_t n = d.n;
     const uint32_t root = 2 * n - 2;
     const uint64_t idx_mask = (kf.idx_bits >= 32) ? 0xFFFFFFFFull : ((1ull << kf.idx_bits) - 1);
-    const uint32_t lo = chunk * kFrontChunk;
-    const uint32_t hi = (lo + kFrontChunk < n) ? lo + kFrontChunk : n;
+    const uint32_t lo = chunk * a.sb.chunk;
+    const uint32_t hi = (lo + a.sb.chunk < n) ? lo + a.sb.chunk : n;
     const float4* __restrict__ tb = tri_boxes + 2 * (size_t)d.start;
     for (uint32_t i = lo + threadIdx.x; i < hi; i += 256) {
         const size_t g = (size_t)d.start + i;

@@ -367,8 +367,34 @@ class _Builder:
     def get_last_build_time_ms(self) -> float:
         return self._times["total_ms"]
 
+    def _stage_sum(self, prefix):
+        return sum(v for k, v in self._times["stages"].items() if k.startswith(prefix))
+
     def get_timing_breakdown(self) -> str:
-        return "\n".join(f"  {k + ':':<18}{v:.3f} ms" for k, v in self._times["stages"].items())
+        """The reference's stage lines (lbvh_builder.cu:490-498 and the
+        LBVH+/PLOC counterparts); fused stages read 0 (the C++ classes give
+        the same text).  The finer per-kernel stages are in `stages()`."""
+        s = self._stage_sum
+        if self.algo == ALGO_LBVH:
+            rows = [("Bounds/Centroids", s("Bounds/Centroids")), ("Morton Codes", s("Morton Codes")),
+                    ("CUB Radix Sort", s("Sort Pass")), ("Topology Build", s("Topology+Refit")),
+                    ("Bottom-Up Refit", 0.0)]
+        elif self.algo == ALGO_LBVH_PLUS:
+            opt = s("Treelet Round")
+            rows = [("Bounds/Centroids", s("Bounds/Centroids")), ("Morton Codes", s("Morton Codes")),
+                    ("Radix Sort", s("Sort Pass")), ("Topology Build", s("Topology+Refit")),
+                    ("Refit + Optimize", opt)]
+        else:
+            rows = [("Initialization", s("Bounds/Centroids") + s("Morton Codes") + s("Sort Pass") + s("PLOC Init")),
+                    ("NN Search (Red)", s("PLOC Global") + s("PLOC LDS Tail")), ("Merging (Green)", 0.0),
+                    ("Compaction (Mag)", 0.0)]
+        lines = [f"  {name + ':':<18}{v:.3f} ms" for name, v in rows]
+        if self.algo == ALGO_LBVH_PLUS:
+            lines[-1] += f" (opt: {rows[-1][1]:.3f} ms)"
+        return "\n".join(lines)
+
+    def stages(self) -> dict:
+        return dict(self._times["stages"])
 
     def cleanup(self):
         self._nodes = np.zeros(0, dtype=NODE_DTYPE)

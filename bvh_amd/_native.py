@@ -116,6 +116,7 @@ _HIP_SIGS = {
 }
 
 _HOST_SIGS = {
+    "bvh_rng_u64": (_i32, [_u64, _u32, _vp]),
     "bvh_scene_uniform": (_i32, [_u64, _u32, C.POINTER(_vp)]),
     "bvh_scene_blobs": (_i32, [_u64, _u32, C.POINTER(_vp)]),
     "bvh_scene_strands": (_i32, [_u64, _u32, _u32, C.POINTER(_vp)]),

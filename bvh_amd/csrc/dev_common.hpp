@@ -207,7 +207,7 @@ __device__ __forceinline__ uint32_t wave_incl_sum(uint32_t v) {
     return v;
 }
 
-// 30-bit Morton code (reference lbvh_builder.cu:75-91): x in the top bit of each triple.
+// 30-bit Morton code (reference lbvh_builder.cu:11-27): x in the top bit of each triple.
 __device__ __forceinline__ uint32_t expand10(uint32_t v) {
     v = (v * 0x00010001u) & 0xFF0000FFu;
     v = (v * 0x00000101u) & 0x0F00F00Fu;
@@ -239,7 +239,7 @@ __device__ __forceinline__ uint64_t morton60(float x, float y, float z) {
     return (expand21((uint64_t)z) << 2) | (expand21((uint64_t)y) << 1) | expand21((uint64_t)x);
 }
 
-// Normalised centroid coordinate (reference lbvh_builder.cu:235-241).
+// Normalised centroid coordinate (reference lbvh_builder.cu:167-181).
 __device__ __forceinline__ float norm_coord(float mn, float mx, float smn, float smx) {
     float c = (mn + mx) * 0.5f;
     float ext = smx - smn;

@@ -3,10 +3,10 @@
 // same pass.
 //
 // Reference stages replaced:
-//   kComputeBoundsAndCentroids_nt/_P/_ploc  (lbvh_builder.cu:111-130,
+//   kComputeBoundsAndCentroids_nt/_P/_ploc  (lbvh_builder.cu:47-66,
 //       lbvh_plus_builder.cu:255-274, ploc_builder.cu:33-57)
-//   kReduceBounds_Step1/2 + D2H sync         (lbvh_builder.cu:137-229,453-466)
-//   kComputeMortonCodes_nt / _P              (lbvh_builder.cu:231-245,
+//   kReduceBounds_Step1/2 + D2H sync         (lbvh_builder.cu:73-165,385-398)
+//   kComputeMortonCodes_nt / _P              (lbvh_builder.cu:167-181,
 //       lbvh_plus_builder.cu:276-290, ploc_builder.cu:121-141)
 //
 // MI355X mapping: both passes stream HBM with coalesced loads (nine SoA fp32

@@ -1,7 +1,7 @@
 // k_hier.hip — LBVH topology, leaf initialisation and node boxes.
 //
 // Replaces kBuildInternalNodes_nt/_P + kInitLeafNodes_nt/_P + kRefitHierarchy_nt
-// (reference lbvh_builder.cu:247-338, lbvh_plus_builder.cu:292-335,341-366 with
+// (reference lbvh_builder.cu:183-274, lbvh_plus_builder.cu:292-335,341-366 with
 // doOptimize = false).
 //
 // The reference finds each internal node top-down (Karras 2012: direction,

@@ -1,7 +1,7 @@
 // k_sort.hip — stable LSD radix sort of 64-bit keys (with optional 32-bit
 // payload), one kernel per digit pass, decoupled look-back between tiles.
 //
-// Replaces cub::DeviceRadixSort::SortPairs (reference lbvh_builder.cu:485-493)
+// Replaces cub::DeviceRadixSort::SortPairs (reference lbvh_builder.cu:412-439)
 // and thrust::sort_by_key (lbvh_plus_builder.cu:465, ploc_builder.cu:459-461).
 //
 // Design for gfx950:

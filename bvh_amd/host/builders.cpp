@@ -2,6 +2,7 @@
 #include "hip_builder.hpp"
 
 #include <cstdio>
+#include <cstring>
 #include <iomanip>
 #include <sstream>
 #include <stdexcept>
@@ -87,13 +88,41 @@ void HipBuilderCore::cleanup() {
     indices_.shrink_to_fit();
 }
 
+// The reference's stage lines (lbvh_builder.cu:490-498,
+// lbvh_plus_builder.cu:535-543, ploc_builder.cu:393-400), filled from the
+// finer stages the C ABI reports.  Fused stages: the hierarchy kernel builds
+// topology and boxes in one pass (its time is the "Topology Build" line and
+// "Bottom-Up Refit" reads 0), the treelet kernel refits while it optimises,
+// and PLOC's iteration kernels search, merge and compact together (their time
+// is the "NN Search" line; "Merging" and "Compaction" read 0).
 std::string HipBuilderCore::breakdown() const {
+    auto sum = [&](const char* prefix) {
+        float t = 0.0f;
+        for (const auto& st : stages_)
+            if (st.first.compare(0, std::strlen(prefix), prefix) == 0) t += st.second;
+        return t;
+    };
     std::ostringstream oss;
     oss << std::fixed << std::setprecision(3);
-    for (size_t i = 0; i < stages_.size(); ++i) {
-        std::string label = stages_[i].first + ":";
-        oss << "  " << std::left << std::setw(24) << label << stages_[i].second << " ms";
-        if (i + 1 < stages_.size()) oss << "\n";
+    if (algo_ == BVH_ALGO_LBVH) {
+        oss << "  Bounds/Centroids: " << sum("Bounds/Centroids") << " ms\n";
+        oss << "  Morton Codes:     " << sum("Morton Codes") << " ms\n";
+        oss << "  CUB Radix Sort:   " << sum("Sort Pass") << " ms\n";
+        oss << "  Topology Build:   " << sum("Topology+Refit") << " ms\n";
+        oss << "  Bottom-Up Refit:  " << 0.0f << " ms";
+    } else if (algo_ == BVH_ALGO_LBVH_PLUS) {
+        const float opt = sum("Treelet Round");
+        oss << "  Bounds/Centroids: " << sum("Bounds/Centroids") << " ms\n";
+        oss << "  Morton Codes:     " << sum("Morton Codes") << " ms\n";
+        oss << "  Radix Sort:       " << sum("Sort Pass") << " ms\n";
+        oss << "  Topology Build:   " << sum("Topology+Refit") << " ms\n";
+        oss << "  Refit + Optimize: " << opt << " ms (opt: " << opt << " ms)";
+    } else {
+        oss << "  Initialization:   "
+            << sum("Bounds/Centroids") + sum("Morton Codes") + sum("Sort Pass") + sum("PLOC Init") << " ms\n";
+        oss << "  NN Search (Red):  " << sum("PLOC Global") + sum("PLOC LDS Tail") << " ms\n";
+        oss << "  Merging (Green):  " << 0.0f << " ms\n";
+        oss << "  Compaction (Mag): " << 0.0f << " ms";
     }
     return oss.str();
 }

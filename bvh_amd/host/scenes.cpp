@@ -81,7 +81,7 @@ inline void crossf(const float* a, const float* b, float* o) {
     o[1] = a[2] * b[0] - a[0] * b[2];
     o[2] = a[0] * b[1] - a[1] * b[0];
 }
-// d_normalize (render_engine.cu:128-133).
+// d_normalize (render_engine.cu:130-135).
 inline void normalizef(float* v) {
     float len = std::sqrt(dotf(v, v));
     if (len < 1e-12f) { v[0] = v[1] = v[2] = 0.0f; return; }
@@ -102,6 +102,13 @@ uint64_t bvh_fnv1a64(const void* data, size_t bytes, uint64_t h) {
         h *= 1099511628211ull;
     }
     return h;
+}
+
+int bvh_rng_u64(uint64_t seed, uint32_t n, uint64_t* out) {
+    if (!out) return 1;
+    Rng g(seed);
+    for (uint32_t i = 0; i < n; ++i) out[i] = g.next();
+    return 0;
 }
 
 int bvh_scene_uniform(uint64_t seed, uint32_t n, float* const* soa) {

@@ -10,20 +10,20 @@
  * What each entry point replaces in the reference (paths relative to the
  * reference repository):
  *   bvh_mesh_upload      the nine cudaMalloc + cudaMemcpy H2D uploads at the
- *                        start of every build (src/cuda/lbvh_builder.cu:398-438,
+ *                        start of every build (src/cuda/lbvh_builder.cu:334-374,
  *                        src/cuda/lbvh_plus_builder.cu:424-439,
  *                        src/cuda/ploc_builder.cu:422-435) and
  *                        src/render_engine.cu:452-479 (uploadMesh)
- *   bvh_build            LBVHBuilderCUDA::runCompute  (lbvh_builder.cu:440-527),
+ *   bvh_build            LBVHBuilderCUDA::runCompute  (lbvh_builder.cu:376-463),
  *                        LBVHPlusBuilderCUDA::runCompute (lbvh_plus_builder.cu:441-508),
  *                        PLOCBuilderCUDA::build (ploc_builder.cu:403-586)
- *   bvh_tree_download    LBVHBuilderCUDA::downloadResults (lbvh_builder.cu:529-546),
+ *   bvh_tree_download    LBVHBuilderCUDA::downloadResults (lbvh_builder.cu:465-482),
  *                        LBVHPlusBuilderCUDA::downloadResults (lbvh_plus_builder.cu:510-527),
  *                        the D2H copies and host root swap at ploc_builder.cu:554-582
  *   bvh_tree_from_nodes  the node upload of renderImage (render_engine.cu:742-746)
  *   bvh_trace            kRenderKernel/traverseBVH (render_engine.cu:289-446)
  *                        over caller-supplied rays
- *   bvh_sort_pairs_u64   cub::DeviceRadixSort::SortPairs (lbvh_builder.cu:485-493) and
+ *   bvh_sort_pairs_u64   cub::DeviceRadixSort::SortPairs (lbvh_builder.cu:412-439) and
  *                        thrust::sort_by_key (lbvh_plus_builder.cu:465, ploc_builder.cu:459-461)
  *   bvh_batch_*          no reference counterpart: the reference builds one mesh
  *                        per builder->build call (src/batch_runner.cpp:72-120 loops

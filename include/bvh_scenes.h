@@ -18,6 +18,10 @@
 extern "C" {
 #endif
 
+/* The first n raw 64-bit outputs of the generators' RNG for `seed` (xoshiro256**
+ * whose four state words are the first four SplitMix64 outputs of `seed`). */
+int bvh_rng_u64(uint64_t seed, uint32_t n, uint64_t* out);
+
 /* Uniform soup: centroid ~ U[0,1]^3, each vertex = centroid + U[-0.005,0.005]^3. */
 int bvh_scene_uniform(uint64_t seed, uint32_t n, float* const* soa);
 

@@ -14,22 +14,32 @@
 // (what std::min/std::max compute; reference include/common.h:56-62 host path).
 //
 // Function -> reference it follows -> what pins it (tests/test_oracle_*.py):
-//   tri_box            src/cuda/lbvh_builder.cu:111-130     brute force per triangle, -0/+0 fixture
-//   morton30           src/cuda/lbvh_builder.cu:75-91,231-245  expandBits known answers, bit-loop
+//   tri_box            src/cuda/lbvh_builder.cu:47-66       brute-force min/max per triangle, hand-derived
+//                                                          signed-zero box fixture (test_oracle_pins.py)
+//   scene_box          lbvh_builder.cu:73-165                brute-force min/max over all vertices
+//   norm_centroid      lbvh_builder.cu:167-181               mesh-level known cells: exact cell-centre
+//                                                          meshes over non-unit boxes, box-centroid vs
+//                                                          vertex-mean triangle, the ext > 1e-6 rule at,
+//                                                          below and above 1e-6 (test_oracle_pins.py)
+//   morton30           src/cuda/lbvh_builder.cu:11-27,167-181  expandBits known answers, bit-loop
 //                                                          interleave, cell boundaries in closed form
 //   morton60           src/cuda/lbvh_plus_builder.cu:20-35,276-290  same, z-major 20-bit interleave
-//   sort               lbvh_builder.cu:485-493 (stable)      numpy lexsort on (code, index)
-//   karras             lbvh_builder.cu:97-107,247-293         Karras 2012 Fig. 3 8-key example,
+//   sort               lbvh_builder.cu:412-439 (stable)      numpy lexsort on (code, index)
+//   karras             lbvh_builder.cu:33-43,183-229         Karras 2012 Fig. 3 8-key example,
 //                      lbvh_plus_builder.cu:41-47,292-324     top-down radix-tree recursion, invariants
-//   refit              lbvh_builder.cu:307-338                brute-force union over each subtree
+//   refit              lbvh_builder.cu:243-274                brute-force union over each subtree
 //   treelet optimise   lbvh_plus_builder.cu:71-247,341-366    brute force over all binary trees on
 //                                                          the treelet leaves (DP optimality), leaf
 //                                                          multiset and parent links preserved
 //   ploc               src/cuda/ploc_builder.cu:173-264,403-586  naive O(m^2) window transcription in
 //                                                          Python on small n, progress invariant
 //   trace              src/render_engine.cu:194-356          brute-force closest hit over all triangles
-//   sah                src/evaluator.cpp:38-88               reference evaluator.cpp compiled in
-//                                                          oracle/_ref (when the reference is present)
+//   sah                src/evaluator.cpp:38-88               closed forms: n identical boxes give
+//                                                          Ct(n-1)+Ci*n; hand-computed 3-leaf tree at
+//                                                          (3,2) and (1,1); zero-area root
+//   depth_stats        src/evaluator.cpp:90-113              complete 2^k-leaf trees and caterpillars
+// The reference itself cannot be compiled here (its headers need the CUDA
+// toolkit), so no function is checked against reference binaries.
 // Known deviations from the reference, recorded as semantics (not bugs):
 //   * PLOC numbers new nodes in ascending cluster order within an iteration
 //     (one legal serialisation of the reference's atomicAdd, ploc_builder.cu:245).
@@ -116,7 +126,7 @@ struct Soa {
     float get(int comp, int64_t i) const { return v[comp][i]; }
 };
 
-// Per-triangle box: min(v0, min(v1, v2)) per axis (lbvh_builder.cu:119-125).
+// Per-triangle box: min(v0, min(v1, v2)) per axis (lbvh_builder.cu:55-60).
 inline Box tri_box(const Soa& m, int64_t i) {
     Box b;
     for (int k = 0; k < 3; ++k) {
@@ -133,7 +143,7 @@ Box scene_box(const Soa& m, int64_t n) {
     return s;
 }
 
-// Normalised centroid coordinate (lbvh_builder.cu:235-241).
+// Normalised centroid coordinate (lbvh_builder.cu:174-176).
 inline void norm_centroid(const Box& tb, const Box& scene, float out[3]) {
     for (int k = 0; k < 3; ++k) {
         float c = (tb.mn[k] + tb.mx[k]) * 0.5f;
@@ -143,7 +153,7 @@ inline void norm_centroid(const Box& tb, const Box& scene, float out[3]) {
     }
 }
 
-// expandBits_nt (lbvh_builder.cu:75-81): 10 bits -> every third bit of 30.
+// expandBits_nt (lbvh_builder.cu:11-17): 10 bits -> every third bit of 30.
 inline uint32_t expand10(uint32_t v) {
     v = (v * 0x00010001u) & 0xFF0000FFu;
     v = (v * 0x00000101u) & 0x0F00F00Fu;
@@ -152,7 +162,7 @@ inline uint32_t expand10(uint32_t v) {
     return v;
 }
 
-// morton3D_nt (lbvh_builder.cu:83-91): x in the most significant bit of each triple.
+// morton3D_nt (lbvh_builder.cu:19-27): x in the most significant bit of each triple.
 inline uint32_t morton30(float x, float y, float z) {
     x = fmin_s(fmax_s(x * 1024.0f, 0.0f), 1023.0f);
     y = fmin_s(fmax_s(y * 1024.0f, 0.0f), 1023.0f);
@@ -185,7 +195,7 @@ inline uint64_t morton60(float x, float y, float z) {
 inline int clz32(uint32_t x) { return x == 0 ? 32 : __builtin_clz(x); }
 inline int clz64(uint64_t x) { return x == 0 ? 64 : __builtin_clzll(x); }
 
-// delta_nt (lbvh_builder.cu:97-107) over sorted (code30, index).
+// delta_nt (lbvh_builder.cu:33-43) over sorted (code30, index).
 struct Delta30 {
     const uint32_t* code;
     const uint32_t* idx;
@@ -217,7 +227,7 @@ struct LNode {
     uint32_t parent;
 };
 
-// kBuildInternalNodes_nt / _P for one internal node i (lbvh_builder.cu:247-293).
+// kBuildInternalNodes_nt / _P for one internal node i (lbvh_builder.cu:183-229).
 template <class D>
 void karras_node(const D& delta, int64_t n, int64_t i, std::vector<LNode>& nodes) {
     int64_t d = (delta(i, i + 1) - delta(i, i - 1)) >= 0 ? 1 : -1;
@@ -246,7 +256,7 @@ void karras_node(const D& delta, int64_t n, int64_t i, std::vector<LNode>& nodes
     nodes[rightIdx].parent = (uint32_t)i;
 }
 
-// kInitLeafNodes (lbvh_builder.cu:295-305).
+// kInitLeafNodes (lbvh_builder.cu:231-241).
 void init_leaves(const Soa& m, int64_t n, const uint32_t* sorted_idx, std::vector<LNode>& nodes) {
     for (int64_t i = 0; i < n; ++i) {
         LNode& L = nodes[(n - 1) + i];
@@ -366,7 +376,7 @@ void optimize_treelet(std::vector<LNode>& nodes, int rootIdx) {
 
 // kRefitHierarchy_nt / kRefitAndOptimize, executed as one legal serial schedule:
 // leaf threads in ascending order, each climbing until it is the first arrival
-// (lbvh_builder.cu:307-338, lbvh_plus_builder.cu:341-366).
+// (lbvh_builder.cu:243-274, lbvh_plus_builder.cu:341-366).
 void refit(std::vector<LNode>& nodes, int64_t n, bool optimize) {
     std::vector<int> counter(2 * n - 1, 0);
     for (int64_t i = 0; i < n; ++i) {
@@ -545,7 +555,7 @@ uint64_t orc_expand21(uint64_t v) { return expand21(v); }
 uint32_t orc_morton30_point(float x, float y, float z) { return morton30(x, y, z); }
 uint64_t orc_morton60_point(float x, float y, float z) { return morton60(x, y, z); }
 
-// Morton codes in input order (kComputeMortonCodes_nt, lbvh_builder.cu:231-245).
+// Morton codes in input order (kComputeMortonCodes_nt, lbvh_builder.cu:167-181).
 void orc_morton30(const float* const* soa, int64_t n, uint32_t* codes) {
     Soa m;
     for (int k = 0; k < 9; ++k) m.v[k] = soa[k];

@@ -11,8 +11,7 @@ import os
 import numpy as np
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
-ORACLE_SO = os.path.join(ROOT, "oracle", "liboracle.so")
-REF_EVAL_SO = os.path.join(ROOT, "oracle", "_ref", "libref_evaluator.so")
+ORACLE_SO = os.environ.get("BVH_ORACLE_SO") or os.path.join(ROOT, "oracle", "liboracle.so")
 
 NODE_DTYPE = np.dtype([
     ("bmin", "<f4", (3,)), ("pad0", "<f4"), ("bmax", "<f4", (3,)), ("pad1", "<f4"),
@@ -190,16 +189,6 @@ def depth_stats(nodes):
     lv = C.c_int64(0)
     lib().orc_depth_stats(nodes.ctypes.data, nodes.size, C.byref(md), C.byref(avg), C.byref(lv))
     return md.value, avg.value, lv.value
-
-
-def ref_evaluator():
-    """The reference's own evaluator.cpp, compiled by oracle/Makefile (None if absent)."""
-    if not os.path.exists(REF_EVAL_SO):
-        return None
-    L = C.CDLL(REF_EVAL_SO)
-    L.ref_compute_sah.restype = C.c_float
-    L.ref_compute_sah.argtypes = [_vp, C.c_long]
-    return L
 
 
 def nodes_equal(a, b):

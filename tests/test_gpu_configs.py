@@ -1,0 +1,170 @@
+"""Parity at the workloads' own sizes.
+
+* 4,000,000 blob triangles (seed 7): LBVH+ (7-leaf treelets, 3 rounds) and
+  PLOC (r = 25) bit-exact against the oracle; 1024x768 bounding-sphere camera
+  rays through the LBVH, LBVH+ and PLOC trees against the oracle traversal
+  (ids identical, t bit-identical; the stated tolerance is 1e-6 relative).
+* Random-walk strands: LBVH+ 3 rounds and PLOC r = 100 bit-exact against the
+  oracle at 1,000,000 triangles.
+* 32,000,000 strand triangles (200,000 strands x 160 steps, seed 99): PLOC
+  r = 100 and LBVH+ 3 rounds checked through size-independent properties
+  (full binary tree reached from the root, every primitive in exactly one
+  leaf, leaf boxes = triangle boxes, every internal box = the select union of
+  its children, SAH non-increasing over treelet rounds), and a 65,536-ray
+  sample of the 4,194,304 incoherent rays traced by the oracle through the
+  same trees.
+
+Oracle references: lbvh_plus_builder.cu:341-366,483-495 (rounds),
+ploc_builder.cu:489-548 (iteration loop), render_engine.cu:289-356 (traversal).
+"""
+import numpy as np
+import pytest
+
+import bvh_amd as B
+import oracle as O
+from tree_props import check_tree_properties, sah64
+
+pytestmark = pytest.mark.gpu
+
+T_REL = 1e-6   # stated tolerance for hit distances (the kernels give identical bits)
+
+
+def _build(ctx, dm, algo, **kw):
+    tree = ctx.tree()
+    ctx.build(dm, tree, algo=algo, **kw)
+    return tree
+
+
+def _compare_trace(ctx, tree, dm, nodes, mesh, rays, sample=None):
+    t, prim, _ = ctx.trace(tree, dm, rays)
+    if sample is not None:
+        rays, t, prim = rays[sample], t[sample], prim[sample]
+    ot, op, _ = O.trace(nodes, mesh.soa, rays, omp=True)
+    np.testing.assert_array_equal(prim, op)
+    hit = op >= 0
+    assert np.all(np.abs(t[hit] - ot[hit]) <= T_REL * np.abs(ot[hit]))
+    np.testing.assert_array_equal(t.view(np.uint32), ot.view(np.uint32))
+    return hit.mean()
+
+
+# ---------------------------------------------------------------- 4M blobs --
+
+@pytest.fixture(scope="module")
+def blobs4m(ctx):
+    mesh = B.Mesh.blobs(7, 4_000_000)
+    dm = ctx.upload(mesh)
+    yield mesh, dm
+    dm.free()
+
+
+def test_blobs_4m_lbvh_plus_three_rounds(ctx, blobs4m):
+    mesh, dm = blobs4m
+    tree = _build(ctx, dm, B.ALGO_LBVH_PLUS, rounds=3)
+    try:
+        nodes, idx = tree.download()
+        onodes, oidx, _ = O.build_lbvh_plus(mesh.soa, 3)
+        np.testing.assert_array_equal(idx, oidx)
+        assert O.nodes_equal(nodes, onodes) is None
+        s_gpu, s_ref = O.sah(nodes), O.sah(onodes)
+        assert abs(s_gpu - s_ref) <= 1e-5 * abs(s_ref)
+        rays = B.rays_sphere_camera(mesh.bounds(), 1024, 768)
+        assert _compare_trace(ctx, tree, dm, onodes, mesh, rays) > 0.05
+    finally:
+        tree.free()
+
+
+def test_blobs_4m_ploc_r25(ctx, blobs4m):
+    mesh, dm = blobs4m
+    tree = _build(ctx, dm, B.ALGO_PLOC, radius=25)
+    try:
+        nodes, idx = tree.download()
+        st, onodes, oidx, _ = O.build_ploc(mesh.soa, 25)
+        assert st == 0
+        np.testing.assert_array_equal(idx, oidx)
+        assert O.nodes_equal(nodes, onodes) is None
+        rays = B.rays_sphere_camera(mesh.bounds(), 1024, 768)
+        assert _compare_trace(ctx, tree, dm, onodes, mesh, rays) > 0.05
+    finally:
+        tree.free()
+
+
+def test_blobs_4m_camera_rays_lbvh(ctx, blobs4m):
+    mesh, dm = blobs4m
+    tree = _build(ctx, dm, B.ALGO_LBVH)
+    try:
+        onodes, oidx, _ = O.build_lbvh(mesh.soa)
+        nodes, idx = tree.download()
+        assert O.nodes_equal(nodes, onodes) is None
+        rays = B.rays_sphere_camera(mesh.bounds(), 1024, 768)
+        assert _compare_trace(ctx, tree, dm, onodes, mesh, rays) > 0.05
+    finally:
+        tree.free()
+
+
+# ------------------------------------------------------------ strands 1M --
+
+@pytest.mark.parametrize("algo,kw", [(B.ALGO_LBVH_PLUS, {"rounds": 3}), (B.ALGO_PLOC, {"radius": 100})])
+def test_strands_1m_against_oracle(ctx, algo, kw):
+    mesh = B.Mesh.strands(99, 6250, 160)
+    dm = ctx.upload(mesh)
+    tree = _build(ctx, dm, algo, **kw)
+    try:
+        nodes, idx = tree.download()
+        if algo == B.ALGO_PLOC:
+            st, onodes, oidx, _ = O.build_ploc(mesh.soa, kw["radius"])
+            assert st == 0
+        else:
+            onodes, oidx, _ = O.build_lbvh_plus(mesh.soa, kw["rounds"])
+        np.testing.assert_array_equal(idx, oidx)
+        assert O.nodes_equal(nodes, onodes) is None
+        rays = B.rays_uniform_cube(99, 65_536)
+        _compare_trace(ctx, tree, dm, onodes, mesh, rays)
+    finally:
+        tree.free()
+        dm.free()
+
+
+# ----------------------------------------------------------- strands 32M --
+
+@pytest.fixture(scope="module")
+def strands32m(ctx):
+    print("[configs] generating 32,000,000 strand triangles (seed 99)", flush=True)
+    mesh = B.Mesh.strands(99, 200_000, 160)
+    dm = ctx.upload(mesh)
+    tb = O.tri_boxes(mesh.soa)
+    rays = B.rays_uniform_cube(99, 4_194_304)
+    sample = np.arange(0, rays.shape[0], 64)
+    yield mesh, dm, tb, rays, sample
+    dm.free()
+
+
+def test_strands_32m_ploc_r100(ctx, strands32m):
+    mesh, dm, tb, rays, sample = strands32m
+    tree = _build(ctx, dm, B.ALGO_PLOC, radius=100)
+    try:
+        nodes, idx = tree.download()
+        check_tree_properties(nodes, idx, tb)
+        _compare_trace(ctx, tree, dm, nodes, mesh, rays, sample=sample)
+    finally:
+        tree.free()
+
+
+def test_strands_32m_lbvh_plus_three_rounds(ctx, strands32m):
+    mesh, dm, tb, rays, sample = strands32m
+    costs = []
+    for rounds in (0, 1, 2, 3):
+        tree = _build(ctx, dm, B.ALGO_LBVH_PLUS, rounds=rounds)
+        try:
+            nodes, idx = tree.download()
+            if rounds in (0, 3):            # ~20 s each at 32M on the host
+                check_tree_properties(nodes, idx, tb)
+            costs.append(sah64(nodes))
+            if rounds == 3:
+                _compare_trace(ctx, tree, dm, nodes, mesh, rays, sample=sample)
+        finally:
+            tree.free()
+        del nodes
+    print("[configs] 32M LBVH+ SAH by rounds:", costs, flush=True)
+    for a, b in zip(costs, costs[1:]):
+        assert b <= a * (1 + 1e-9)
+    assert costs[-1] < costs[0]

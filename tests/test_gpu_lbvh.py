@@ -4,7 +4,9 @@ import numpy as np
 import pytest
 
 import bvh_amd as B
+import morton_fixtures as MF
 import oracle as O
+from test_oracle_pins import SIGNED_ZERO_BOXES, signed_zero_soa
 
 pytestmark = pytest.mark.gpu
 
@@ -97,4 +99,55 @@ def test_builder_interface_mirror(ctx):
     assert O.nodes_equal(b.get_nodes(), onodes) is None
     np.testing.assert_array_equal(b.get_indices(), oidx)
     assert b.get_last_build_time_ms() > 0
-    assert "Sort Pass 1" in b.get_timing_breakdown()
+    text = b.get_timing_breakdown()
+    names = [line.split(":")[0].strip() for line in text.splitlines()]
+    assert names == ["Bounds/Centroids", "Morton Codes", "CUB Radix Sort", "Topology Build", "Bottom-Up Refit"]
+    assert "Sort Pass 1" in b.stages()
+
+
+@pytest.mark.parametrize("algo,bits", [(B.ALGO_LBVH, 10), (B.ALGO_PLOC, 10), (B.ALGO_LBVH_PLUS, 20)])
+@pytest.mark.parametrize("lo,ext", [((0, 0, 0), (1, 1, 1)), ((2, -1, 0), (4, 2, 0.5)), ((-64, 8, -0.25), (32, 8, 2))])
+def test_morton_known_cells_on_device(ctx, algo, bits, lo, ext):
+    """Mesh-level Morton codes (scene box, box centroid, normalisation) equal
+    closed-form cells, not only the oracle (lbvh_builder.cu:167-181)."""
+    soa, want = MF.cells_mesh(bits, seed=bits + len(lo), lo=lo, ext=ext)
+    dm = ctx.upload(B.Mesh(soa))
+    try:
+        np.testing.assert_array_equal(ctx.debug_morton(dm, algo), want)
+    finally:
+        dm.free()
+
+
+@pytest.mark.parametrize("algo,bits", [(B.ALGO_LBVH, 10), (B.ALGO_LBVH_PLUS, 20)])
+@pytest.mark.parametrize("z_top", MF.FLAT_CASES)
+def test_morton_extent_rule_on_device(ctx, algo, bits, z_top):
+    soa, want = MF.flat_mesh(bits, z_top)
+    dm = ctx.upload(B.Mesh(soa))
+    try:
+        np.testing.assert_array_equal(ctx.debug_morton(dm, algo), want)
+    finally:
+        dm.free()
+
+
+def test_signed_zero_leaf_boxes(ctx):
+    """Leaf boxes keep the hand-derived signed-zero bits of the select rule."""
+    soa = signed_zero_soa()
+    n = soa.shape[1]
+    nodes, idx = _build_download(ctx, B.Mesh(soa), B.ALGO_LBVH)
+    leaves = nodes[n - 1:]
+    for k in range(n):
+        p = int(idx[k])
+        _, lo_bits, hi_bits = SIGNED_ZERO_BOXES[p]
+        assert int(leaves["bmin"][k, 0:1].view(np.uint32)[0]) == lo_bits, p
+        assert int(leaves["bmax"][k, 0:1].view(np.uint32)[0]) == hi_bits, p
+
+
+def _build_download(ctx, mesh, algo, **kw):
+    dm = ctx.upload(mesh)
+    tree = ctx.tree()
+    try:
+        ctx.build(dm, tree, algo=algo, **kw)
+        return tree.download()
+    finally:
+        tree.free()
+        dm.free()

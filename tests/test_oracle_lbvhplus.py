@@ -112,3 +112,28 @@ def test_lbvh_plus_sah_never_worse(n):
     # 60-bit z-major codes sorted ascending with index ties.
     order = np.lexsort((np.arange(n), O.morton60(soa)))
     np.testing.assert_array_equal(idx, order.astype(np.uint32))
+
+
+def test_tree_property_checker_on_oracle_trees():
+    """The property checker used at full size accepts every builder's tree and
+    rejects a tree with one swapped child box or a duplicated primitive."""
+    import bvh_amd as B
+    from tree_props import check_tree_properties, sah64
+    m = B.Mesh.strands(99, 40, 160)
+    tb = O.tri_boxes(m.soa)
+    trees = [O.build_lbvh(m.soa)[:2], O.build_lbvh_plus(m.soa, 3)[:2], O.build_ploc(m.soa, 100)[1:3]]
+    for nodes, idx in trees:
+        check_tree_properties(nodes, idx, tb)
+    s0 = sah64(O.build_lbvh_plus(m.soa, 0)[0])
+    s3 = sah64(trees[1][0])
+    assert s3 < s0
+    nodes, idx = trees[0]
+    bad = nodes.copy()
+    bad["bmin"][3] = bad["bmin"][4]
+    with pytest.raises(AssertionError):
+        check_tree_properties(bad, idx, tb)
+    bad = nodes.copy()
+    n = idx.size
+    bad["left"][n] = bad["left"][n + 1]
+    with pytest.raises(AssertionError):
+        check_tree_properties(bad, idx, tb)

@@ -20,6 +20,7 @@ STATUS = {
     5: "no progress", 6: "over memory budget", 7: "no HIP device", 8: "invalid state", 9: "unsupported",
 }
 ALGO_LBVH, ALGO_LBVH_PLUS, ALGO_PLOC = 0, 1, 2
+NOALLOC_MAX_TRIANGLES = 262144
 BUILD_KEEP_KEYS = 1
 BUILD_TUNE = 2
 
@@ -86,6 +87,7 @@ _HIP_SIGS = {
     "bvh_ctx_destroy": (_i32, [_vp]),
     "bvh_last_error": (C.c_char_p, [_vp]),
     "bvh_status_string": (C.c_char_p, [C.c_int]),
+    "bvh_device_memory": (_i32, [_vp, C.POINTER(_u64), C.POINTER(_u64)]),
     "bvh_device_alloc": (_i32, [_vp, _sz, _pp]),
     "bvh_device_free": (_i32, [_vp, _vp]),
     "bvh_copy_h2d": (_i32, [_vp, _vp, _vp, _sz]),
@@ -100,6 +102,7 @@ _HIP_SIGS = {
     "bvh_tree_from_nodes": (_i32, [_vp, _vp, _u32, _vp]),
     "bvh_tree_set_indices": (_i32, [_vp, _vp, _vp, _u32]),
     "bvh_build": (_i32, [_vp, _vp, C.POINTER(BuildParams), _vp, C.POINTER(StageTimes)]),
+    "bvh_build_lbvh_noalloc": (_i32, [_vp, C.POINTER(_vp), _u32, _vp, _vp]),
     "bvh_debug_morton": (_i32, [_vp, _vp, _i32, _vp]),
     "bvh_debug_sorted_keys": (_i32, [_vp, _vp, _vp, _u32]),
     "bvh_sort_pairs_u64": (_i32, [_vp, _vp, _vp, _u32, C.c_int, C.c_int, C.c_int, C.POINTER(_f)]),

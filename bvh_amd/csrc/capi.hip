@@ -16,6 +16,7 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -28,6 +29,7 @@ using namespace bvhk;
 struct DevBuf {
     void* p = nullptr;
     size_t cap = 0;
+    bool fixed = false;  // a region of the no-allocation scratch: never freed or grown
 };
 
 // Device workspace of one build pipeline.
@@ -160,6 +162,9 @@ int set_err(bvh_ctx_t ctx, int code, const char* fmt, ...) {
 int ensure(bvh_ctx_t ctx, DevBuf& b, size_t bytes, bool zero = false) {
     if (bytes == 0) bytes = 16;
     if (b.cap >= bytes) return BVH_OK;
+    if (b.fixed)
+        return set_err(ctx, BVH_E_STATE, "internal error: no-allocation scratch region of %zu bytes needs %zu", b.cap,
+                       bytes);
     if (b.p) {
         HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
         HIP_TRY(ctx, hipFree(b.p));
@@ -274,24 +279,27 @@ int bit_width(uint32_t v) {  // bits needed for values 0..v
 }
 
 // Standalone build: one segment, keys morton30 << 32 | index (or morton60).
-Job single_job(const bvh_mesh_rec* mesh, bvh_tree_rec* tree, bool m60) {
+Job single_job(const SegDesc& seg, bool m60) {
+    const uint32_t n = seg.n;
     Job j;
     j.fsb.segs = nullptr;
     j.fsb.blocks = nullptr;
-    j.fsb.single = seg_of(mesh, tree, 0);
-    j.fsb.chunk = mesh->n >= kFrontLargeMin ? kFrontChunkLarge : kFrontChunk;
-    j.fsb.nblocks = cdiv(mesh->n, j.fsb.chunk);
+    j.fsb.single = seg;
+    j.fsb.chunk = n >= kFrontLargeMin ? kFrontChunkLarge : kFrontChunk;
+    j.fsb.nblocks = cdiv(n, j.fsb.chunk);
     j.hsb = j.fsb;
-    j.hsb.nblocks = cdiv(mesh->n, (uint32_t)hier_chunk());
+    j.hsb.nblocks = cdiv(n, (uint32_t)hier_chunk());
     j.k = 1;
-    j.total = mesh->n;
-    j.maxn = mesh->n;
+    j.total = n;
+    j.maxn = n;
     j.m60 = m60;
     j.kf = KeyFormat{32, 62};
     j.begin_bit = m60 ? 0 : 32;
     j.end_bit = m60 ? 60 : 62;
     return j;
 }
+
+Job single_job(const bvh_mesh_rec* mesh, bvh_tree_rec* tree, bool m60) { return single_job(seg_of(mesh, tree, 0), m60); }
 
 SortPlan plan_for(const Job& j, int radix_bits) {
     return make_sort_plan(j.total, j.begin_bit, j.end_bit, pick_bits(radix_bits), (int)j.fsb.nblocks, j.m60);
@@ -384,14 +392,17 @@ HierArgs hier_args(Workspace& w, const Job& j, const uint64_t* keys, const uint3
 }
 
 // Non-finite input and internal-error words of every segment.
-int check_input_flags(bvh_ctx_t ctx, Workspace& w, uint32_t k) {
+int check_input_flags(bvh_ctx_t ctx, Workspace& w, uint32_t k, uint32_t* host_dst = nullptr) {
     // Single builds read their flag words into a pinned buffer (a pageable
     // destination adds a staged copy to every build); multi-mesh builds,
-    // where that copy is small beside the build, use a pageable one.
+    // where that copy is small beside the build, use a pageable one.  A
+    // caller-provided destination (8k words) is used as it is.
     const size_t words = (size_t)k * 8;
     std::vector<uint32_t> pageable;
     uint32_t* dst = nullptr;
-    if (k == 1) {
+    if (host_dst) {
+        dst = host_dst;
+    } else if (k == 1) {
         if (!ctx->host_flags) HIP_TRY(ctx, hipHostMalloc((void**)&ctx->host_flags, words * 4, hipHostMallocDefault));
         dst = ctx->host_flags;
     } else {
@@ -941,6 +952,100 @@ int prepare_trace(bvh_ctx_t ctx, bvh_tree_t tree) {
 
 }  // namespace
 
+// ---- no-allocation LBVH build (bvh_build_lbvh_noalloc) ----------------------------
+//
+// Every buffer an LBVH build of up to BVH_NOALLOC_MAX_TRIANGLES triangles
+// needs (triangle boxes, the two key buffers, digit totals, look-back words,
+// scene words, handoff words, epoch, hierarchy units and status words) lives
+// at a fixed offset of this array, sized for that maximum.  Offsets do not
+// depend on the call's triangle count, so a region only ever holds its own
+// kind of data: the epoch-tagged handoff and look-back words stay valid for
+// calls of any size.  Each device holds its own copy until the process ends.
+constexpr size_t kNoAllocScratchBytes = (size_t)40 << 20;
+static __device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
+
+namespace {
+
+constexpr int kNoAllocMaxDevices = 64;
+
+struct NoAllocDevice {
+    void* ptr = nullptr;    // device address of this device's copy of `scratch`
+    bool ready = false;     // cleared and laid out
+    int status = BVH_OK;    // setup result
+    Workspace layout;       // fixed regions inside ptr
+};
+std::mutex g_noalloc_mu;
+NoAllocDevice g_noalloc[kNoAllocMaxDevices];
+
+size_t align256(size_t v) { return (v + 255) & ~(size_t)255; }
+
+// Lays the regions of an LBVH build of kNoAllocMax triangles out from base;
+// returns the bytes used.
+size_t noalloc_layout(Workspace& w, unsigned char* base) {
+    SegDesc d;
+    std::memset(&d, 0, sizeof(d));
+    d.n = BVH_NOALLOC_MAX_TRIANGLES;
+    const Job j = single_job(d, false);
+    const SortPlan plan = plan_for(j, 0);
+    const size_t n = j.total;
+    size_t off = 0;
+    auto region = [&](DevBuf& b, size_t bytes) {
+        b.p = base + off;
+        b.cap = align256(bytes);
+        b.fixed = true;
+        off += b.cap;
+    };
+    region(w.tri_boxes, n * 32);
+    region(w.keys_a, n * 8);
+    region(w.keys_b, n * 8);
+    region(w.hist_tot, (size_t)plan.passes * (1u << plan.bits) * 4);
+    region(w.lookback, zero_bytes_of(plan));
+    region(w.scene, 32);
+    region(w.handoff, n * 8);
+    region(w.epoch, 16);
+    region(w.hctl, (size_t)2 * hier_max_levels() * 4);
+    region(w.hunits[0], (size_t)j.hsb.nblocks * hier_chunk() * hier_unit_bytes());
+    region(w.hunits[1], n * (size_t)hier_unit_bytes());
+    region(w.hcounts, (size_t)j.hsb.nblocks * 4);
+    region(w.hstatus, (size_t)hier_max_levels() * (j.hsb.nblocks + 1) * 8);
+    return off;
+}
+
+// Once per device, with that device current: the address of the device's
+// copy of `scratch` (the array's name is only a host-side stand-in), a
+// zero fill of the whole array (handoff, status and epoch words must start at
+// zero) and the fixed layout.
+int noalloc_setup(bvh_ctx_t ctx) {
+    if (ctx->device >= kNoAllocMaxDevices) return BVH_OK;  // the entry point reports it
+    std::lock_guard<std::mutex> lock(g_noalloc_mu);
+    NoAllocDevice& nd = g_noalloc[ctx->device];
+    if (nd.ready || nd.status != BVH_OK) return nd.status;
+    void* ptr = nullptr;
+    hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !ptr) {
+        nd.status = set_err(ctx, BVH_E_HIP, "hipGetSymbolAddress(scratch) failed: %s", hipGetErrorString(e));
+        return nd.status;
+    }
+    const size_t used = noalloc_layout(nd.layout, static_cast<unsigned char*>(ptr));
+    if (used > kNoAllocScratchBytes) {
+        nd.status = set_err(ctx, BVH_E_STATE, "internal error: no-allocation layout needs %zu of %zu bytes", used,
+                            kNoAllocScratchBytes);
+        return nd.status;
+    }
+    launch_fill_zero(static_cast<uint32_t*>(ptr), kNoAllocScratchBytes / 4, ctx->stream);
+    e = hipGetLastError();
+    if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+    if (e != hipSuccess) {
+        nd.status = set_err(ctx, BVH_E_HIP, "clearing the no-allocation scratch failed: %s", hipGetErrorString(e));
+        return nd.status;
+    }
+    nd.ptr = ptr;
+    nd.ready = true;
+    return BVH_OK;
+}
+
+}  // namespace
+
 extern "C" {
 
 const char* bvh_status_string(int status) {
@@ -986,6 +1091,8 @@ int bvh_ctx_create(int device, bvh_ctx_t* out) {
             return BVH_E_HIP;
         }
     }
+    // Errors of the no-allocation setup are reported by that entry point.
+    (void)noalloc_setup(c);
     *out = c;
     return BVH_OK;
 }
@@ -1178,6 +1285,46 @@ int bvh_build(bvh_ctx_t ctx, bvh_mesh_t mesh, const bvh_build_params* params, bv
         case BVH_ALGO_LBVH_PLUS: return build_lbvh_plus(ctx, mesh, params, tree, times);
         default: return set_err(ctx, BVH_E_UNSUPPORTED, "algorithm %d not available", params->algo);
     }
+}
+
+int bvh_build_lbvh_noalloc(bvh_ctx_t ctx, const float* const* d_soa, uint32_t n, void* d_nodes,
+                           uint32_t* d_indices) {
+    if (!ctx) return BVH_E_ARG;
+    if (!d_soa || !d_nodes || !d_indices) return set_err(ctx, BVH_E_ARG, "null argument");
+    for (int k = 0; k < 9; ++k)
+        if (!d_soa[k]) return set_err(ctx, BVH_E_ARG, "vertex array %d is null", k);
+    if (n == 0) return set_err(ctx, BVH_E_ARG, "mesh has no triangles");
+    if (n > BVH_NOALLOC_MAX_TRIANGLES)
+        return set_err(ctx, BVH_E_ARG, "%u triangles exceed the no-allocation limit of %u", n,
+                       (uint32_t)BVH_NOALLOC_MAX_TRIANGLES);
+    if (ctx->device >= kNoAllocMaxDevices)
+        return set_err(ctx, BVH_E_UNSUPPORTED, "no-allocation builds support devices 0..%d", kNoAllocMaxDevices - 1);
+    HIP_TRY(ctx, hipSetDevice(ctx->device));
+    Workspace w;
+    {
+        std::lock_guard<std::mutex> lock(g_noalloc_mu);
+        const NoAllocDevice& nd = g_noalloc[ctx->device];
+        if (!nd.ready)
+            return set_err(ctx, nd.status != BVH_OK ? nd.status : BVH_E_STATE,
+                           "no-allocation scratch is not set up on device %d", ctx->device);
+        w = nd.layout;
+    }
+    SegDesc d;
+    for (int k = 0; k < 9; ++k) d.v[k] = d_soa[k];
+    d.n = n;
+    d.start = 0;
+    d.nodes = static_cast<DNode*>(d_nodes);
+    d.indices = d_indices;
+    const Job j = single_job(d, false);
+    const SortPlan plan = plan_for(j, 0);
+    TRY(reserve_front(ctx, w, j, plan, true));  // size checks only: every region is fixed
+    uint64_t* keys = nullptr;
+    uint32_t* vals = nullptr;
+    TRY(enqueue_front(ctx, w, j, plan, nullptr, &keys, &vals));
+    launch_hierarchy(hier_args(w, j, keys, nullptr, false), ctx->stream);
+    TRY(check_launch(ctx, "k_hierarchy"));
+    uint32_t flags[8];
+    return check_input_flags(ctx, w, 1, flags);
 }
 
 int bvh_debug_morton(bvh_ctx_t ctx, bvh_mesh_t mesh, int32_t algo, uint64_t* codes) {

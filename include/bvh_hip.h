@@ -123,6 +123,23 @@ int bvh_tree_set_indices(bvh_ctx_t ctx, bvh_tree_t tree, const uint32_t* indices
 int bvh_build(bvh_ctx_t ctx, bvh_mesh_t mesh, const bvh_build_params* params, bvh_tree_t tree,
               bvh_stage_times* times);
 
+/* ---- build without allocation ------------------------------------------- */
+/* No counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid allocation): an LBVH build (the same kernels and the same result as
+ * bvh_build with BVH_ALGO_LBVH) whose working memory is a fixed scratch array
+ * of 40 MiB that the library holds on every device it runs on, from context
+ * creation until the process ends.  It calls no hipMalloc or hipFree.
+ * d_soa: nine device pointers (v0x v0y v0z v1x v1y v1z v2x v2y v2z), n floats
+ * each; d_nodes: (2n-1) 48-byte BVHNode records; d_indices: n uint32 — all
+ * device memory owned by the caller.  1 <= n <= BVH_NOALLOC_MAX_TRIANGLES,
+ * else BVH_E_ARG.  Synchronous on the context stream.  Every call on one GPU
+ * shares that GPU's scratch: calls that overlap on one GPU (from several
+ * contexts or threads) race and must be serialised by the caller. */
+#define BVH_NOALLOC_MAX_TRIANGLES 262144u
+int bvh_build_lbvh_noalloc(bvh_ctx_t ctx, const float* const* d_soa, uint32_t n, void* d_nodes,
+                           uint32_t* d_indices);
+
 /* ---- batched builds ------------------------------------------------------ */
 /* A batch builds k resident meshes into k trees with one algorithm (LBVH or
  * PLOC) in one hipGraph-captured launch sequence.  Every tree is bit-identical

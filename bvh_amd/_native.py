@@ -54,6 +54,11 @@ class StageTimes(C.Structure):
         return {self.names[i].value.decode(): float(self.ms[i]) for i in range(self.count)}
 
 
+class Footprint(C.Structure):
+    _fields_ = [("mesh", C.c_uint64), ("tree", C.c_uint64), ("build_workspace", C.c_uint64),
+                ("trace_workspace", C.c_uint64)]
+
+
 class BatchInfo(C.Structure):
     _fields_ = [("meshes", C.c_uint32), ("triangles", C.c_uint32), ("max_triangles", C.c_uint32),
                 ("algo", C.c_int32), ("radix_bits", C.c_int32), ("sort_passes", C.c_int32), ("key_bits", C.c_int32),
@@ -103,6 +108,8 @@ _HIP_SIGS = {
     "bvh_tree_set_indices": (_i32, [_vp, _vp, _vp, _u32]),
     "bvh_build": (_i32, [_vp, _vp, C.POINTER(BuildParams), _vp, C.POINTER(StageTimes)]),
     "bvh_build_lbvh_noalloc": (_i32, [_vp, C.POINTER(_vp), _u32, _vp, _vp]),
+    "bvh_footprint_estimate": (_i32, [_i32, _u32, _u32, _u32, C.POINTER(Footprint)]),
+    "bvh_ctx_held_bytes": (_i32, [_vp, C.POINTER(_u64), C.POINTER(_u64)]),
     "bvh_debug_morton": (_i32, [_vp, _vp, _i32, _vp]),
     "bvh_debug_sorted_keys": (_i32, [_vp, _vp, _vp, _u32]),
     "bvh_sort_pairs_u64": (_i32, [_vp, _vp, _vp, _u32, C.c_int, C.c_int, C.c_int, C.POINTER(_f)]),
@@ -131,6 +138,7 @@ _HOST_SIGS = {
     "bvh_rays_uniform_cube": (_i32, [_u64, _u32, _vp]),
     "bvh_fnv1a64": (_u64, [_vp, _sz, _u64]),
     "bvh_tree_file_write": (_i32, [C.c_char_p, _vp, _u32, _vp, _u32]),
+    "bvh_tree_validate": (_i32, [_vp, _u32, _vp, _u32]),
     "bvh_tree_file_read": (_i32, [C.c_char_p, _vp, C.POINTER(_u32), _vp, C.POINTER(_u32)]),
     "bvh_tree_save": (_i32, [_vp, _vp, C.c_char_p]),
     "bvh_tree_load": (_i32, [_vp, C.c_char_p, _vp]),

@@ -29,7 +29,8 @@ using namespace bvhk;
 struct DevBuf {
     void* p = nullptr;
     size_t cap = 0;
-    bool fixed = false;  // a region of the no-allocation scratch: never freed or grown
+    bool fixed = false;    // a region of the no-allocation scratch: never freed or grown
+    bool measure = false;  // footprint estimation: record the capacity an allocation would get
 };
 
 // Device workspace of one build pipeline.
@@ -162,6 +163,10 @@ int set_err(bvh_ctx_t ctx, int code, const char* fmt, ...) {
 int ensure(bvh_ctx_t ctx, DevBuf& b, size_t bytes, bool zero = false) {
     if (bytes == 0) bytes = 16;
     if (b.cap >= bytes) return BVH_OK;
+    if (b.measure) {
+        b.cap = ((bytes + bytes / 8) + 255) & ~(size_t)255;
+        return BVH_OK;
+    }
     if (b.fixed)
         return set_err(ctx, BVH_E_STATE, "internal error: no-allocation scratch region of %zu bytes needs %zu", b.cap,
                        bytes);
@@ -952,6 +957,81 @@ int prepare_trace(bvh_ctx_t ctx, bvh_tree_t tree) {
 
 }  // namespace
 
+namespace {
+
+// Capacity (with the growth slack) of every buffer a build or trace would
+// allocate: the same reserve functions, run on buffers in measuring mode.
+size_t measured(Workspace& w) {
+    size_t t = 0;
+    w.each([&](DevBuf& b) { t += b.cap; });
+    return t;
+}
+
+uint64_t footprint_build(int algo, uint32_t ntri, uint32_t meshes) {
+    if (ntri == 0 || meshes == 0) return 0;
+    Workspace w;
+    w.each([](DevBuf& b) { b.measure = true; });
+    Job j;
+    uint64_t tables = 0;
+    if (meshes == 1) {
+        SegDesc d;
+        std::memset(&d, 0, sizeof(d));
+        d.n = ntri;
+        j = single_job(d, algo == BVH_ALGO_LBVH_PLUS);
+    } else {
+        // Upper bounds over every split of ntri triangles into `meshes` meshes.
+        j.k = meshes;
+        j.total = ntri;
+        j.maxn = ntri;
+        j.m60 = false;
+        j.fsb.nblocks = cdiv(ntri, kFrontChunk) + meshes;
+        j.fsb.chunk = kFrontChunk;
+        j.hsb = j.fsb;
+        j.hsb.nblocks = cdiv(ntri, (uint32_t)hier_chunk()) + meshes;
+        const int idx_bits = std::max(1, bit_width(ntri - 1));
+        j.kf = KeyFormat{(uint32_t)idx_bits, (uint32_t)(idx_bits + 30)};
+        j.begin_bit = idx_bits;
+        j.end_bit = std::min(64, idx_bits + 30 + bit_width(meshes - 1));
+        DevBuf seg, fb, hb;
+        seg.measure = fb.measure = hb.measure = true;
+        (void)ensure(nullptr, seg, (size_t)meshes * sizeof(SegDesc));
+        (void)ensure(nullptr, fb, (size_t)j.fsb.nblocks * sizeof(uint2));
+        (void)ensure(nullptr, hb, (size_t)j.hsb.nblocks * sizeof(uint2));
+        tables = seg.cap + fb.cap + hb.cap;
+    }
+    const SortPlan plan = plan_for(j, 0);
+    (void)reserve_front(nullptr, w, j, plan, algo != BVH_ALGO_PLOC);
+    if (algo == BVH_ALGO_LBVH_PLUS) {
+        (void)ensure(nullptr, w.parents, (size_t)(2 * (size_t)ntri) * 4);
+        (void)ensure(nullptr, w.ranges, (size_t)ntri * 8);
+        (void)ensure(nullptr, w.counters, (size_t)ntri * 4);
+    }
+    if (algo == BVH_ALGO_PLOC) (void)reserve_ploc(nullptr, w, j);
+    return measured(w) + tables;
+}
+
+uint64_t footprint_trace(uint32_t nrays) {
+    if (nrays == 0) return 0;
+    const size_t rb = (size_t)nrays * 24, tb = (size_t)nrays * 4, cb = (size_t)nrays * 12;
+    Workspace w;
+    w.each([](DevBuf& b) { b.measure = true; });
+    DevBuf io, ovf, list, k0, k1;
+    for (DevBuf* b : {&io, &ovf, &list, &k0, &k1}) b->measure = true;
+    (void)ensure(nullptr, io, rb + 2 * tb + cb + 256);
+    (void)ensure(nullptr, ovf, 64);
+    (void)ensure(nullptr, list, (size_t)nrays * 4);
+    if (nrays >= kRaySortMin) {
+        (void)ensure(nullptr, k0, (size_t)nrays * 8);
+        (void)ensure(nullptr, k1, (size_t)nrays * 8);
+        const SortPlan plan = make_sort_plan(nrays, 0, 64, 8);
+        (void)ensure(nullptr, w.hist_tot, (size_t)plan.passes * (1u << plan.bits) * 4);
+        (void)ensure(nullptr, w.lookback, zero_bytes_of(plan));
+    }
+    return io.cap + ovf.cap + list.cap + k0.cap + k1.cap + measured(w);
+}
+
+}  // namespace
+
 // ---- no-allocation LBVH build (bvh_build_lbvh_noalloc) ----------------------------
 //
 // Every buffer an LBVH build of up to BVH_NOALLOC_MAX_TRIANGLES triangles
@@ -1325,6 +1405,38 @@ int bvh_build_lbvh_noalloc(bvh_ctx_t ctx, const float* const* d_soa, uint32_t n,
     TRY(check_launch(ctx, "k_hierarchy"));
     uint32_t flags[8];
     return check_input_flags(ctx, w, 1, flags);
+}
+
+int bvh_footprint_estimate(int32_t algo, uint32_t ntri, uint32_t meshes, uint32_t nrays, bvh_footprint* out) {
+    if (!out || (algo != BVH_ALGO_LBVH && algo != BVH_ALGO_LBVH_PLUS && algo != BVH_ALGO_PLOC)) return BVH_E_ARG;
+    if (algo == BVH_ALGO_LBVH_PLUS && meshes > 1) return BVH_E_UNSUPPORTED;
+    std::memset(out, 0, sizeof(*out));
+    if (ntri) {
+        const uint64_t stride = ((uint64_t)ntri + 63) & ~(uint64_t)63;
+        out->mesh = stride * 9 * 4 + (uint64_t)ntri * 48;
+        DevBuf nodes, idx, rec;
+        nodes.measure = idx.measure = rec.measure = true;
+        (void)ensure(nullptr, nodes, (2 * (size_t)ntri - 1) * sizeof(DNode));
+        (void)ensure(nullptr, idx, (size_t)ntri * 4);
+        (void)ensure(nullptr, rec, (2 * (size_t)ntri - 1) * 64);
+        out->tree = nodes.cap + idx.cap + rec.cap;
+    }
+    out->build_workspace = footprint_build(algo, ntri, meshes ? meshes : 1);
+    out->trace_workspace = footprint_trace(nrays);
+    return BVH_OK;
+}
+
+int bvh_ctx_held_bytes(bvh_ctx_t ctx, uint64_t* build_workspace, uint64_t* trace_workspace) {
+    if (!ctx) return BVH_E_ARG;
+    uint64_t b = 0;
+    ctx->ws.each([&](DevBuf& d) { b += d.cap; });
+    for (DevBuf* d : {&ctx->tabs.segs, &ctx->tabs.fblocks, &ctx->tabs.hblocks}) b += d->cap;
+    uint64_t t = 0;
+    for (DevBuf* d : {&ctx->trace_ovf, &ctx->trace_list, &ctx->trace_io, &ctx->ray_keys[0], &ctx->ray_keys[1]})
+        t += d->cap;
+    if (build_workspace) *build_workspace = b;
+    if (trace_workspace) *trace_workspace = t;
+    return BVH_OK;
 }
 
 int bvh_debug_morton(bvh_ctx_t ctx, bvh_mesh_t mesh, int32_t algo, uint64_t* codes) {

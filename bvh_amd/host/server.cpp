@@ -19,12 +19,36 @@
 
 namespace {
 
-uint64_t mesh_bytes(uint32_t n) {
-    const uint64_t stride = ((uint64_t)n + 63) & ~(uint64_t)63;
-    return stride * 9 * 4 + (uint64_t)n * 48;  // SoA copy + traversal AoS copy
+// Device bytes of one request's mesh and tree (with traversal records), from
+// the library's own allocation sizes.
+uint64_t persistent_bytes(int32_t algo, uint32_t ntri) {
+    bvh_footprint f;
+    if (bvh_footprint_estimate(algo, ntri, 1, 0, &f) != BVH_OK) return 0;
+    return f.mesh + f.tree;
 }
 
-uint64_t tree_bytes(uint32_t n) { return (2 * (uint64_t)n - 1) * 48 + (uint64_t)n * 4; }
+// Context workspace a group needs: one build over its new meshes and the
+// largest of its traces (traces of a group run one after another on the
+// slot's context and reuse the same buffers).
+void group_workspace(int32_t algo, uint64_t tris, uint32_t builds, uint32_t max_rays, uint64_t* build,
+                     uint64_t* trace) {
+    bvh_footprint f;
+    std::memset(&f, 0, sizeof(f));
+    const uint32_t t32 = (uint32_t)std::min<uint64_t>(tris, 0xFFFFFFFFull);
+    if (bvh_footprint_estimate(algo, builds ? t32 : 0, builds ? builds : 1, max_rays, &f) != BVH_OK) {
+        *build = *trace = ~0ull / 4;
+        return;
+    }
+    *build = f.build_workspace;
+    *trace = f.trace_workspace;
+}
+
+// A resident mesh taken out of service by eviction; freed outside the lock.
+struct Evicted {
+    bvh_mesh_t mesh = nullptr;
+    bvh_tree_t tree = nullptr;
+    uint64_t bytes = 0;
+};
 
 struct Resident {
     uint64_t key = 0;
@@ -60,7 +84,11 @@ struct bvh_server_rec {
     std::vector<bvh_ctx_t> ctxs;
     std::vector<std::thread> threads;
     bool stop = false;
-    uint64_t in_use = 0, peak = 0, resident_bytes = 0, lru = 0;
+    // in_use: resident meshes/BVHs + in-flight unkeyed meshes/BVHs; held_*:
+    // what each slot's grow-only context workspace holds (or has reserved for
+    // the group it runs).  Free memory = budget - in_use - held_total.
+    uint64_t in_use = 0, peak = 0, resident_bytes = 0, lru = 0, held_total = 0;
+    std::vector<uint64_t> held_build, held_trace;
     uint32_t in_flight = 0, groups = 0, completed = 0, evictions = 0, rejected = 0, max_group_seen = 0;
     std::string err;
 
@@ -68,7 +96,8 @@ struct bvh_server_rec {
         return std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
     }
     void run_slot(int slot);
-    bool form_group(bvh_ctx_t own, std::vector<Req>& group);
+    bool form_group(int slot, std::vector<Req>& group, Evicted* ev);
+    void set_held(int slot, uint64_t build, uint64_t trace);
     void finish_one(int slot, Req& q, int status, uint32_t gsize);
     void publish_residents(int slot, std::vector<Req>& group, std::vector<int>& status);
     void complete_locked(const Req& q, int status, int slot, uint32_t gsize);
@@ -76,12 +105,11 @@ struct bvh_server_rec {
 
 extern "C" void bvh_serve_footprint(uint32_t ntri, uint32_t nrays, int32_t algo, uint64_t* transient,
                                     uint64_t* resident) {
-    // Build workspace per triangle: boxes 32 + two key buffers 16 + handoff 8;
-    // PLOC adds two cluster buffers (box, id, mesh index) 80 + neighbour 4.
-    const uint64_t build = (uint64_t)ntri * (algo == BVH_ALGO_PLOC ? 140 : 56);
-    const uint64_t rays = (uint64_t)nrays * 40;  // rays 24 + t 4 + prim 4 + overflow list 4, padded
-    if (transient) *transient = build + rays;
-    if (resident) *resident = ntri ? mesh_bytes(ntri) + tree_bytes(ntri) : 0;
+    bvh_footprint f;
+    std::memset(&f, 0, sizeof(f));
+    (void)bvh_footprint_estimate(algo, ntri, 1, nrays, &f);
+    if (transient) *transient = (ntri ? f.build_workspace : 0) + f.trace_workspace;
+    if (resident) *resident = ntri ? f.mesh + f.tree : 0;
 }
 
 extern "C" uint32_t bvh_serve_form_group(const uint64_t* bytes, const uint32_t* tris, const uint8_t* runnable,
@@ -117,10 +145,19 @@ void bvh_server_rec::complete_locked(const Req& q, int status, int slot, uint32_
     cv_done.notify_all();
 }
 
-// Called with the lock held by the slot whose context is `own`.  Takes the
-// next group off the queue (evicting idle resident meshes when memory is
-// short), or rejects requests that can never run.
-bool bvh_server_rec::form_group(bvh_ctx_t own, std::vector<Req>& group) {
+void bvh_server_rec::set_held(int slot, uint64_t build, uint64_t trace) {
+    held_total -= held_build[slot] + held_trace[slot];
+    held_build[slot] = build;
+    held_trace[slot] = trace;
+    held_total += build + trace;
+    peak = std::max(peak, in_use + held_total);
+}
+
+// Called with the lock held by `slot`.  Takes the next group off the queue,
+// or rejects requests that can never run.  When memory is short it takes the
+// least recently used idle resident mesh out of service instead and returns
+// it in *ev (the caller frees it outside the lock, then calls again).
+bool bvh_server_rec::form_group(int slot, std::vector<Req>& group, Evicted* ev) {
     while (!queue.empty()) {
         const uint32_t look = std::min<uint32_t>((uint32_t)queue.size(), (uint32_t)cfg.max_group);
         std::vector<uint64_t> bytes(look);
@@ -132,8 +169,6 @@ bool bvh_server_rec::form_group(bvh_ctx_t own, std::vector<Req>& group) {
         for (uint32_t i = 0; i < look; ++i) {
             const bvh_request& r = queue[i].r;
             const bool has_soa = r.soa[0] != nullptr;
-            uint64_t tr = 0, rs = 0;
-            bvh_serve_footprint(r.ntri, r.nrays, cfg.algo, &tr, &rs);
             Resident* rp = nullptr;
             if (r.mesh_key) {
                 auto it = residents.find(r.mesh_key);
@@ -143,13 +178,13 @@ bool bvh_server_rec::form_group(bvh_ctx_t own, std::vector<Req>& group) {
             if (rp) {  // trace against a resident (or one being built)
                 run[i] = rp->ready ? 1 : 0;
                 builds[i] = 0;
-                bytes[i] = (uint64_t)r.nrays * 40;
+                bytes[i] = 0;
                 tris[i] = 0;
             } else if (has_soa) {
                 const bool dup = r.mesh_key && new_keys.count(r.mesh_key);
                 run[i] = dup ? 0 : 1;  // a second request for a key built in this prefix waits for it
                 builds[i] = 1;
-                bytes[i] = tr + rs;
+                bytes[i] = persistent_bytes(cfg.algo, r.ntri);
                 tris[i] = r.ntri;
                 if (r.mesh_key) new_keys[r.mesh_key] = i;
             } else {
@@ -170,24 +205,44 @@ bool bvh_server_rec::form_group(bvh_ctx_t own, std::vector<Req>& group) {
         }
         for (auto& v : run)
             if (v == 2) v = 0;
-        const uint64_t free_bytes = cfg.memory_budget > in_use ? cfg.memory_budget - in_use : 0;
+        const uint64_t used = in_use + held_total;
+        const uint64_t free_bytes = cfg.memory_budget > used ? cfg.memory_budget - used : 0;
         uint32_t g = bvh_serve_form_group(bytes.data(), tris.data(), run.data(), look, free_bytes,
                                           (uint32_t)cfg.max_group, cfg.max_group_tris);
+        // The prefix's meshes must fit together with what this slot's context
+        // workspace has to grow by for the group's build and traces.
+        uint64_t need_build = 0, need_trace = 0;
+        while (g > 0) {
+            uint64_t persist = 0, t = 0;
+            uint32_t nb = 0, max_rays = 0;
+            for (uint32_t i = 0; i < g; ++i) {
+                persist += bytes[i];
+                if (builds[i]) {
+                    t += tris[i];
+                    ++nb;
+                }
+                max_rays = std::max(max_rays, queue[i].r.nrays);
+            }
+            group_workspace(cfg.algo, t, nb, max_rays, &need_build, &need_trace);
+            const uint64_t grow = (need_build > held_build[slot] ? need_build - held_build[slot] : 0) +
+                                  (need_trace > held_trace[slot] ? need_trace - held_trace[slot] : 0);
+            if (persist + grow <= free_bytes) break;
+            --g;
+        }
         if (g == 0 && run[0]) {
-            // Memory: evict the least recently used idle resident, then look again.
+            // Memory: take the least recently used idle resident out of service.
             Resident* victim = nullptr;
             for (auto& kv : residents) {
                 Resident* r = kv.second.get();
                 if (r->ready && r->users == 0 && (!victim || r->last_use < victim->last_use)) victim = r;
             }
             if (victim) {
-                bvh_tree_free(own, victim->tree);
-                bvh_mesh_free(own, victim->mesh);
-                in_use -= victim->bytes;
-                resident_bytes -= victim->bytes;
+                ev->mesh = victim->mesh;
+                ev->tree = victim->tree;
+                ev->bytes = victim->bytes;
                 ++evictions;
                 residents.erase(victim->key);
-                continue;
+                return false;
             }
             if (in_flight == 0) {
                 Req q = queue.front();
@@ -206,34 +261,33 @@ bool bvh_server_rec::form_group(bvh_ctx_t own, std::vector<Req>& group) {
             queue.pop_front();
             q.admitted = t;
             q.build = builds[i] != 0;
-            uint64_t tr = 0, rs = 0;
-            bvh_serve_footprint(q.r.ntri, q.r.nrays, cfg.algo, &tr, &rs);
             if (q.build) {
+                const uint64_t pb = bytes[i];
                 if (q.r.mesh_key) {
                     auto r = std::make_unique<Resident>();
                     r->key = q.r.mesh_key;
                     r->ntri = q.r.ntri;
-                    r->bytes = rs;
+                    r->bytes = pb;
                     r->users = 1;
                     r->last_use = ++lru;
                     q.res = r.get();
                     residents[q.r.mesh_key] = std::move(r);
-                    q.transient = tr;
-                    resident_bytes += rs;
+                    q.transient = 0;
+                    resident_bytes += pb;
                 } else {
-                    q.transient = tr + rs;
+                    q.transient = pb;
                 }
-                in_use += tr + rs;
+                in_use += pb;
             } else {
                 q.res = res[i];
                 q.res->users++;
                 q.res->last_use = ++lru;
-                q.transient = (uint64_t)q.r.nrays * 40;
-                in_use += q.transient;
+                q.transient = 0;
             }
             group.push_back(q);
         }
-        peak = std::max(peak, in_use);
+        // Reserve the workspace growth until the group reports what it holds.
+        set_held(slot, std::max(held_build[slot], need_build), std::max(held_trace[slot], need_trace));
         in_flight += g;
         ++groups;
         max_group_seen = std::max(max_group_seen, g);
@@ -297,7 +351,23 @@ void bvh_server_rec::run_slot(int slot) {
         std::vector<Req> group;
         {
             std::unique_lock<std::mutex> lk(mu);
-            while (!stop && !form_group(ctx, group)) cv_work.wait(lk);
+            while (!stop) {
+                Evicted ev;
+                if (form_group(slot, group, &ev)) break;
+                if (ev.mesh || ev.tree) {
+                    // Free the evicted mesh without holding the lock (hipFree
+                    // synchronises the device); its bytes stay counted until then.
+                    lk.unlock();
+                    if (ev.tree) bvh_tree_free(ctx, ev.tree);
+                    if (ev.mesh) bvh_mesh_free(ctx, ev.mesh);
+                    lk.lock();
+                    in_use -= ev.bytes;
+                    resident_bytes -= ev.bytes;
+                    cv_work.notify_all();
+                    continue;
+                }
+                cv_work.wait(lk);
+            }
             if (group.empty()) return;  // stopping
         }
         const uint32_t gsize = (uint32_t)group.size();
@@ -354,6 +424,13 @@ void bvh_server_rec::run_slot(int slot) {
             }
             finish_one(slot, q, status[i], gsize);
         }
+        // What the slot's context holds now replaces the group's reservation.
+        uint64_t hb = 0, ht = 0;
+        if (bvh_ctx_held_bytes(ctx, &hb, &ht) == BVH_OK) {
+            std::lock_guard<std::mutex> lk(mu);
+            set_held(slot, hb, ht);
+            cv_work.notify_all();
+        }
     }
 }
 
@@ -371,6 +448,8 @@ int bvh_server_create(const bvh_server_config* cfg, bvh_server_t* out) {
     if (s->cfg.max_group == 0) s->cfg.max_group = 64;
     if (s->cfg.max_group_tris == 0) s->cfg.max_group_tris = 1u << 24;
     s->t0 = std::chrono::steady_clock::now();
+    s->held_build.assign(s->cfg.slots, 0);
+    s->held_trace.assign(s->cfg.slots, 0);
     for (int i = 0; i < s->cfg.slots; ++i) {
         bvh_ctx_t c = nullptr;
         int st = bvh_ctx_create(cfg->device, &c);
@@ -436,7 +515,7 @@ int bvh_server_get_stats(bvh_server_t s, bvh_server_stats* st) {
     if (!s || !st) return BVH_E_ARG;
     std::lock_guard<std::mutex> lk(s->mu);
     st->budget = s->cfg.memory_budget;
-    st->in_use = s->in_use;
+    st->in_use = s->in_use + s->held_total;
     st->peak = s->peak;
     st->resident_bytes = s->resident_bytes;
     st->resident_meshes = (uint32_t)s->residents.size();

@@ -4,6 +4,7 @@
 #include <cstdio>
 #include <cstring>
 #include <memory>
+#include <new>
 #include <vector>
 
 namespace {
@@ -28,9 +29,43 @@ struct VizNode {
 };
 static_assert(sizeof(VizNode) == 32, "VizNode is 32 bytes");
 
+// Bytes left in the file from the current position (-1 when unknown).
+long long bytes_left(FILE* f) {
+    const long here = std::ftell(f);
+    if (here < 0 || std::fseek(f, 0, SEEK_END) != 0) return -1;
+    const long end = std::ftell(f);
+    if (std::fseek(f, here, SEEK_SET) != 0 || end < here) return -1;
+    return (long long)(end - here);
+}
+
 }  // namespace
 
 extern "C" {
+
+int bvh_tree_validate(const void* nodes, uint32_t num_nodes, const uint32_t* indices, uint32_t num_prims) {
+    if (!nodes || num_nodes == 0) return BVH_E_ARG;
+    if (num_prims && (uint64_t)num_nodes != 2ull * num_prims - 1) return BVH_E_STATE;
+    const unsigned char* p = static_cast<const unsigned char*>(nodes);
+    uint64_t leaves = 0;
+    for (uint32_t i = 0; i < num_nodes; ++i) {
+        int32_t l, r;
+        std::memcpy(&l, p + (size_t)i * kNodeBytes + 32, 4);
+        std::memcpy(&r, p + (size_t)i * kNodeBytes + 36, 4);
+        if (l < 0) {  // leaf: primitive in the low 31 bits
+            ++leaves;
+            if (num_prims && (uint32_t)(l & 0x7FFFFFFF) >= num_prims) return BVH_E_STATE;
+        } else if ((uint32_t)l >= num_nodes || r < 0 || (uint32_t)r >= num_nodes || l == (int32_t)i ||
+                   r == (int32_t)i) {
+            return BVH_E_STATE;
+        }
+    }
+    if (leaves != (uint64_t)(num_nodes / 2) + 1) return BVH_E_STATE;  // a binary tree has n leaves, n-1 internal
+    if (indices)
+        for (uint32_t i = 0; i < num_prims; ++i)
+            if (indices[i] >= num_prims) return BVH_E_STATE;
+    return BVH_OK;
+}
+
 
 int bvh_tree_file_write(const char* path, const void* nodes, uint32_t num_nodes, const uint32_t* indices,
                         uint32_t num_prims) {
@@ -53,6 +88,11 @@ int bvh_tree_file_read(const char* path, void* nodes, uint32_t* num_nodes, uint3
     uint32_t head[4];
     if (std::fread(magic, 1, 8, f.get()) != 8 || std::memcmp(magic, kMagic, 8) != 0) return BVH_E_STATE;
     if (std::fread(head, 4, 4, f.get()) != 4 || head[0] != kVersion || head[1] == 0) return BVH_E_STATE;
+    // The header's counts must describe exactly the bytes that follow (a
+    // truncated or foreign file is rejected before anything is allocated).
+    const long long left = bytes_left(f.get());
+    if (left < 0 || (unsigned long long)left != (unsigned long long)head[1] * kNodeBytes + 4ull * head[2])
+        return BVH_E_STATE;
     if (!nodes) {  // sizes only
         *num_nodes = head[1];
         *num_prims = head[2];
@@ -72,23 +112,32 @@ int bvh_tree_save(bvh_ctx_t ctx, bvh_tree_t tree, const char* path) {
     int st = bvh_tree_info(tree, &nn, &np);
     if (st != BVH_OK) return st;
     if (nn == 0) return BVH_E_STATE;
-    std::vector<unsigned char> nodes((size_t)nn * kNodeBytes);
-    std::vector<uint32_t> idx(np);
-    if ((st = bvh_tree_download(ctx, tree, nodes.data(), np ? idx.data() : nullptr)) != BVH_OK) return st;
-    return bvh_tree_file_write(path, nodes.data(), nn, idx.data(), np);
+    try {
+        std::vector<unsigned char> nodes((size_t)nn * kNodeBytes);
+        std::vector<uint32_t> idx(np);
+        if ((st = bvh_tree_download(ctx, tree, nodes.data(), np ? idx.data() : nullptr)) != BVH_OK) return st;
+        return bvh_tree_file_write(path, nodes.data(), nn, idx.data(), np);
+    } catch (const std::bad_alloc&) {
+        return BVH_E_NOMEM;
+    }
 }
 
 int bvh_tree_load(bvh_ctx_t ctx, const char* path, bvh_tree_t tree) {
     if (!ctx || !tree || !path) return BVH_E_ARG;
     uint32_t nn = 0, np = 0;
-    int st = bvh_tree_file_read(path, nullptr, &nn, nullptr, &np);
+    int st = bvh_tree_file_read(path, nullptr, &nn, nullptr, &np);  // counts checked against the file size
     if (st != BVH_OK) return st;
-    std::vector<unsigned char> nodes((size_t)nn * kNodeBytes);
-    std::vector<uint32_t> idx(np);
-    if ((st = bvh_tree_file_read(path, nodes.data(), &nn, idx.data(), &np)) != BVH_OK) return st;
-    if ((st = bvh_tree_from_nodes(ctx, nodes.data(), nn, tree)) != BVH_OK) return st;
-    if (np) st = bvh_tree_set_indices(ctx, tree, idx.data(), np);
-    return st;
+    try {
+        std::vector<unsigned char> nodes((size_t)nn * kNodeBytes);
+        std::vector<uint32_t> idx(np);
+        if ((st = bvh_tree_file_read(path, nodes.data(), &nn, idx.data(), &np)) != BVH_OK) return st;
+        if ((st = bvh_tree_validate(nodes.data(), nn, np ? idx.data() : nullptr, np)) != BVH_OK) return st;
+        if ((st = bvh_tree_from_nodes(ctx, nodes.data(), nn, tree)) != BVH_OK) return st;
+        if (np) st = bvh_tree_set_indices(ctx, tree, idx.data(), np);
+        return st;
+    } catch (const std::bad_alloc&) {
+        return BVH_E_NOMEM;
+    }
 }
 
 int bvh_tree_export_viz(bvh_ctx_t ctx, bvh_tree_t tree, const char* path) {
@@ -97,9 +146,15 @@ int bvh_tree_export_viz(bvh_ctx_t ctx, bvh_tree_t tree, const char* path) {
     int st = bvh_tree_info(tree, &nn, &np);
     if (st != BVH_OK) return st;
     if (nn == 0) return BVH_E_STATE;
-    std::vector<unsigned char> raw((size_t)nn * kNodeBytes);
+    std::vector<unsigned char> raw;
+    std::vector<VizNode> out;
+    try {
+        raw.resize((size_t)nn * kNodeBytes);
+        out.resize(nn);
+    } catch (const std::bad_alloc&) {
+        return BVH_E_NOMEM;
+    }
     if ((st = bvh_tree_download(ctx, tree, raw.data(), nullptr)) != BVH_OK) return st;
-    std::vector<VizNode> out(nn);
     for (uint32_t i = 0; i < nn; ++i) {
         const unsigned char* p = raw.data() + (size_t)i * kNodeBytes;
         std::memcpy(out[i].mn, p + 0, 12);   // bbox.min (x, y, z, pad)

@@ -123,6 +123,22 @@ int bvh_tree_set_indices(bvh_ctx_t ctx, bvh_tree_t tree, const uint32_t* indices
 int bvh_build(bvh_ctx_t ctx, bvh_mesh_t mesh, const bvh_build_params* params, bvh_tree_t tree,
               bvh_stage_times* times);
 
+/* ---- device-memory footprint --------------------------------------------- */
+/* No reference counterpart (the reference allocates per build and never
+ * budgets).  Device bytes the library allocates, growth slack included, for
+ * the pieces of a request: admission control sizes requests with this. */
+typedef struct bvh_footprint {
+    uint64_t mesh;             /* bvh_mesh_upload of one mesh of ntri triangles */
+    uint64_t tree;             /* its tree: nodes, indices and traversal records */
+    uint64_t build_workspace;  /* context workspace of one build over `meshes` meshes totalling ntri
+                                  triangles (an upper bound over how they are split) */
+    uint64_t trace_workspace;  /* context buffers of one bvh_trace_host of nrays rays */
+} bvh_footprint;
+int bvh_footprint_estimate(int32_t algo, uint32_t ntri, uint32_t meshes, uint32_t nrays, bvh_footprint* out);
+/* Device bytes a context holds now in its grow-only build workspace and in
+ * its trace buffers (released by bvh_ctx_destroy). */
+int bvh_ctx_held_bytes(bvh_ctx_t ctx, uint64_t* build_workspace, uint64_t* trace_workspace);
+
 /* ---- build without allocation ------------------------------------------- */
 /* No counterpart in the reference.  For callers that cannot allocate device
  * memory (code under HIP graph capture, tight serving loops, harnesses that

@@ -29,7 +29,15 @@ int bvh_tree_file_write(const char* path, const void* nodes, uint32_t num_nodes,
                         uint32_t num_prims);
 int bvh_tree_file_read(const char* path, void* nodes, uint32_t* num_nodes, uint32_t* indices, uint32_t* num_prims);
 
-/* Device trees <-> file. */
+/* Structure check of a host node array before it is uploaded: num_nodes =
+ * 2 num_prims - 1 (when num_prims > 0), internal children in range and not
+ * self-references, leaf primitives and indices below num_prims, n leaves and
+ * n - 1 internal nodes.  BVH_E_STATE when the array is not such a tree. */
+int bvh_tree_validate(const void* nodes, uint32_t num_nodes, const uint32_t* indices, uint32_t num_prims);
+
+/* Device trees <-> file.  bvh_tree_load rejects a file whose header does not
+ * match its size or whose tree fails bvh_tree_validate (BVH_E_STATE) before
+ * anything is uploaded; host allocation failure gives BVH_E_NOMEM. */
 int bvh_tree_save(bvh_ctx_t ctx, bvh_tree_t tree, const char* path);
 int bvh_tree_load(bvh_ctx_t ctx, const char* path, bvh_tree_t tree);
 

@@ -64,7 +64,8 @@ typedef struct bvh_completion {
 
 typedef struct bvh_server_stats {
     uint64_t budget;          /* bytes */
-    uint64_t in_use;          /* resident meshes/BVHs + in-flight transient bytes */
+    uint64_t in_use;          /* resident meshes/BVHs + in-flight meshes/BVHs + every slot's
+                                 context workspace (held or reserved) */
     uint64_t peak;
     uint64_t resident_bytes;
     uint32_t resident_meshes;
@@ -90,8 +91,11 @@ int bvh_server_drain(bvh_server_t s, int32_t wait_ms);
 const char* bvh_server_last_error(bvh_server_t s);
 int bvh_server_destroy(bvh_server_t s);
 
-/* Device bytes a request holds while it runs (transient) and after it
- * completes (resident: mesh + BVH when it is keyed). */
+/* Device bytes of one request from bvh_footprint_estimate: transient = the
+ * build workspace of its mesh alone + its trace buffers (held by the slot's
+ * context afterwards: context workspaces only grow); resident = its mesh +
+ * BVH with traversal records (kept when it is keyed).  Admission counts each
+ * slot's held workspace and what a group makes it grow by. */
 void bvh_serve_footprint(uint32_t ntri, uint32_t nrays, int32_t algo, uint64_t* transient, uint64_t* resident);
 
 /* The admission policy, pure (no device): given the queue in FIFO order

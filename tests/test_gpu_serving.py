@@ -73,3 +73,39 @@ def test_serving_rejections(ctx):
         got = {c["id"]: c["status"] for c in srv.poll(100)}
         assert got == {1: 6, 2: 8}
         assert srv.stats()["rejected"] == 2
+
+
+@pytest.mark.parametrize("algo,sizes,nrays", [(B.ALGO_LBVH, [300_000], 70_000), (B.ALGO_PLOC, [300_000], 20_000),
+                                              (B.ALGO_LBVH_PLUS, [250_000], 5_000),
+                                              (B.ALGO_LBVH, [1000, 50_000, 123_457, 7], 100_000),
+                                              (B.ALGO_PLOC, [30_000, 2_000, 90_001], 1_000)])
+def test_held_workspace_within_footprint(algo, sizes, nrays):
+    """After real builds and a trace, a fresh context holds no more device
+    memory than bvh_footprint_estimate says (what admission charges)."""
+    import ctypes as C
+    from bvh_amd import _native as N
+    with B.Context(0) as c:
+        meshes = [B.Mesh.blobs(40 + i, n) for i, n in enumerate(sizes)]
+        dms = [c.upload(m) for m in meshes]
+        trees = [c.tree() for _ in dms]
+        try:
+            if len(dms) == 1:
+                c.build(dms[0], trees[0], algo=algo, radius=16, rounds=2)
+            else:
+                c.build_batched(dms, trees, algo=algo, radius=16)
+            rays = B.rays_sphere_to_box(3, meshes[0].bounds(), nrays)
+            c.trace(trees[0], dms[0], rays)
+            hb, ht = C.c_uint64(), C.c_uint64()
+            assert N.hip().bvh_ctx_held_bytes(c.h, C.byref(hb), C.byref(ht)) == 0
+            f = N.Footprint()
+            assert N.hip().bvh_footprint_estimate(algo, sum(sizes), len(sizes), nrays, C.byref(f)) == 0
+            # The trace's ray sort reuses (and may grow) the build's digit
+            # table, so the two parts are compared together.
+            assert hb.value + ht.value <= f.build_workspace + f.trace_workspace, (hb.value, ht.value, f.build_workspace,
+                                                                                 f.trace_workspace)
+            assert hb.value >= 0.9 * f.build_workspace   # the estimate is not a loose bound either
+        finally:
+            for t in trees:
+                t.free()
+            for d in dms:
+                d.free()

@@ -46,3 +46,39 @@ def test_rejects_foreign_file(tmp_path):
         B.tree_file_read(p)
     with pytest.raises(B.NativeError):
         B.tree_file_read(str(tmp_path / "missing.bvh"))
+
+
+def test_rejects_truncated_and_oversized(tmp_path):
+    nodes, idx = _random_tree(100, 3)
+    p = str(tmp_path / "t.bvh")
+    B.tree_file_write(p, nodes, idx)
+    raw = open(p, "rb").read()
+    short = str(tmp_path / "short.bvh")
+    open(short, "wb").write(raw[:-4])
+    with pytest.raises(B.NativeError):
+        B.tree_file_read(short)
+    # A header claiming 2^31 nodes over a tiny file is refused before any allocation.
+    big = str(tmp_path / "big.bvh")
+    open(big, "wb").write(raw[:8] + struct.pack("<4I", 1, 1 << 31, 1 << 30, 0) + raw[24:24 + 96])
+    with pytest.raises(B.NativeError):
+        B.tree_file_read(big)
+
+
+def test_validate_tree_structure():
+    import oracle as O
+    from bvh_amd import _native as N
+    m = B.Mesh.uniform(9, 500)
+    for nodes, idx in (O.build_lbvh(m.soa)[:2], O.build_ploc(m.soa, 8)[1:3]):
+        v = N.host().bvh_tree_validate
+        assert v(nodes.ctypes.data, nodes.size, idx.ctypes.data, idx.size) == 0
+        bad = nodes.copy()
+        bad["right"][0] = nodes.size            # child out of range
+        assert v(bad.ctypes.data, bad.size, idx.ctypes.data, idx.size) == 8
+        bad = nodes.copy()
+        leaf = int(np.nonzero(nodes["left"] < 0)[0][0])
+        bad["left"][leaf] = np.int32(np.uint32(0x80000000 | 500).view(np.int32))   # primitive out of range
+        assert v(bad.ctypes.data, bad.size, idx.ctypes.data, idx.size) == 8
+        assert v(nodes.ctypes.data, nodes.size - 2, idx.ctypes.data, idx.size) == 8   # wrong node count
+        badi = idx.copy()
+        badi[3] = 500
+        assert v(nodes.ctypes.data, nodes.size, badi.ctypes.data, idx.size) == 8

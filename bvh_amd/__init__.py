@@ -98,6 +98,22 @@ def tree_file_read(path: str):
     return nodes, idx
 
 
+def render_ppm(nodes: np.ndarray, mesh: Mesh, width: int, height: int, eye, look_at, up=(0, 1, 0), fov=60.0,
+               mode: int = 0, fixed_heatmap_max: float = 0.0, path: str | None = None) -> dict:
+    """renderImage of the reference API (include/render_engine.h) through its
+    C entry point: traces on the GPU, shades on the host, writes a PPM."""
+    nodes = np.ascontiguousarray(nodes, dtype=NODE_DTYPE)
+    cam = np.array([*eye, *look_at, *up, fov], np.float32)
+    stats = np.zeros(5, np.float32)
+    st = N.host().bvh_render_ppm(nodes.ctypes.data, nodes.size, N.soa_ptrs(mesh.soa), mesh.n, width, height,
+                                 cam.ctypes.data, mode, fixed_heatmap_max, path.encode() if path else None,
+                                 stats.ctypes.data)
+    if st != 0:
+        raise NativeError(st, "render")
+    return {"render_ms": float(stats[0]), "avg_nodes_visited": float(stats[1]), "avg_aabb_tests": float(stats[2]),
+            "avg_tri_tests": float(stats[3]), "max_nodes_visited": float(stats[4])}
+
+
 def batch_counts(seed: int, meshes: int, lo: int, hi: int) -> np.ndarray:
     out = np.zeros(meshes, dtype=np.uint32)
     N.host().bvh_scene_batch_counts(seed, meshes, lo, hi, out.ctypes.data)

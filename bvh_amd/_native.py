@@ -143,6 +143,9 @@ _HOST_SIGS = {
     "bvh_tree_save": (_i32, [_vp, _vp, C.c_char_p]),
     "bvh_tree_load": (_i32, [_vp, C.c_char_p, _vp]),
     "bvh_tree_export_viz": (_i32, [_vp, _vp, C.c_char_p]),
+    "bvh_nodes_export_obj": (_i32, [_vp, _u32, C.c_char_p, _i32]),
+    "bvh_tree_export_obj": (_i32, [_vp, _vp, C.c_char_p, _i32]),
+    "bvh_render_ppm": (_i32, [_vp, _u32, C.POINTER(_vp), _u32, _i32, _i32, _vp, _i32, _f, C.c_char_p, _vp]),
 }
 
 

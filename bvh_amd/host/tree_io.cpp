@@ -3,6 +3,7 @@
 
 #include <cstdio>
 #include <cstring>
+#include <fstream>
 #include <memory>
 #include <new>
 #include <vector>
@@ -138,6 +139,57 @@ int bvh_tree_load(bvh_ctx_t ctx, const char* path, bvh_tree_t tree) {
     } catch (const std::bad_alloc&) {
         return BVH_E_NOMEM;
     }
+}
+
+// exportBVHToOBJ (reference src/main.cpp:66-99): every box as eight "v"
+// lines and six "l" polylines, written with the stream's default float
+// format.  leaves_only keeps the leaf boxes; a leaf is a node whose left
+// child has the leaf bit (the reference tests the node index against the
+// LBVH layout, leaves last, which selects the wrong nodes for PLOC's leaves-
+// first layout).
+int bvh_nodes_export_obj(const void* nodes, uint32_t num_nodes, const char* path, int32_t leaves_only) {
+    if (!nodes || num_nodes == 0 || !path) return BVH_E_ARG;
+    std::ofstream file(path);
+    if (!file) return BVH_E_ARG;
+    const unsigned char* p = static_cast<const unsigned char*>(nodes);
+    long long v_offset = 1;
+    for (uint32_t i = 0; i < num_nodes; ++i) {
+        const unsigned char* rec = p + (size_t)i * kNodeBytes;
+        float mn[3], mx[3];
+        int32_t left;
+        std::memcpy(mn, rec + 0, 12);
+        std::memcpy(mx, rec + 16, 12);
+        std::memcpy(&left, rec + 32, 4);
+        if (leaves_only && left >= 0) continue;
+        const float xs[2] = {mn[0], mx[0]}, ys[2] = {mn[1], mx[1]}, zs[2] = {mn[2], mx[2]};
+        // corners: bottom face (z = min) counter-clockwise, then the top face
+        static const int kCorner[8][3] = {{0, 0, 0}, {1, 0, 0}, {1, 1, 0}, {0, 1, 0},
+                                          {0, 0, 1}, {1, 0, 1}, {1, 1, 1}, {0, 1, 1}};
+        for (const auto& c : kCorner) file << "v " << xs[c[0]] << " " << ys[c[1]] << " " << zs[c[2]] << "\n";
+        const long long o = v_offset;
+        file << "l " << o << " " << o + 1 << " " << o + 2 << " " << o + 3 << " " << o << "\n";
+        file << "l " << o + 4 << " " << o + 5 << " " << o + 6 << " " << o + 7 << " " << o + 4 << "\n";
+        for (int e = 0; e < 4; ++e) file << "l " << o + e << " " << o + 4 + e << "\n";
+        v_offset += 8;
+    }
+    file.flush();
+    return file ? BVH_OK : BVH_E_STATE;
+}
+
+int bvh_tree_export_obj(bvh_ctx_t ctx, bvh_tree_t tree, const char* path, int32_t leaves_only) {
+    if (!ctx || !tree || !path) return BVH_E_ARG;
+    uint32_t nn = 0, np = 0;
+    int st = bvh_tree_info(tree, &nn, &np);
+    if (st != BVH_OK) return st;
+    if (nn == 0) return BVH_E_STATE;
+    std::vector<unsigned char> raw;
+    try {
+        raw.resize((size_t)nn * kNodeBytes);
+    } catch (const std::bad_alloc&) {
+        return BVH_E_NOMEM;
+    }
+    if ((st = bvh_tree_download(ctx, tree, raw.data(), nullptr)) != BVH_OK) return st;
+    return bvh_nodes_export_obj(raw.data(), nn, path, leaves_only);
 }
 
 int bvh_tree_export_viz(bvh_ctx_t ctx, bvh_tree_t tree, const char* path) {

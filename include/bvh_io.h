@@ -4,6 +4,10 @@
  *   bvh_tree_export_viz   the reference's exportBVHToBinary (src/main.cpp:24-29
  *                         VizNode, :43-64): 32 bytes per node (min xyz, max xyz,
  *                         left, right), no header, node order as built
+ *   bvh_tree_export_obj   the reference's exportBVHToOBJ (src/main.cpp:66-99):
+ *                         each box as 8 vertices and 6 polylines; leaves_only
+ *                         selects nodes by their leaf bit (the reference's index
+ *                         test assumes leaves last and is wrong for PLOC trees)
  *   bvh_tree_save/load    a resident tree to/from a file, so a server can keep
  *                         BVHs across restarts (no reference counterpart)
  *   bvh_tree_file_*       the same file format on host arrays (no device)
@@ -40,6 +44,10 @@ int bvh_tree_validate(const void* nodes, uint32_t num_nodes, const uint32_t* ind
  * anything is uploaded; host allocation failure gives BVH_E_NOMEM. */
 int bvh_tree_save(bvh_ctx_t ctx, bvh_tree_t tree, const char* path);
 int bvh_tree_load(bvh_ctx_t ctx, const char* path, bvh_tree_t tree);
+
+/* OBJ wireframe of a host node array or of a device tree. */
+int bvh_nodes_export_obj(const void* nodes, uint32_t num_nodes, const char* path, int32_t leaves_only);
+int bvh_tree_export_obj(bvh_ctx_t ctx, bvh_tree_t tree, const char* path, int32_t leaves_only);
 
 /* The reference's 32-byte VizNode export of a device tree. */
 int bvh_tree_export_viz(bvh_ctx_t ctx, bvh_tree_t tree, const char* path);

@@ -82,3 +82,54 @@ def test_validate_tree_structure():
         badi = idx.copy()
         badi[3] = 500
         assert v(nodes.ctypes.data, nodes.size, badi.ctypes.data, idx.size) == 8
+
+
+def _obj_text(boxes):
+    """The wireframe text for boxes (min, max), built by hand: 8 corners
+    (bottom face counter-clockwise, then top), two closed face loops and four
+    vertical edges per box, indices from 1, floats in the stream's default
+    (%g, 6 significant digits) format."""
+    out = []
+    o = 1
+    for lo, hi in boxes:
+        c = [(lo[0], lo[1], lo[2]), (hi[0], lo[1], lo[2]), (hi[0], hi[1], lo[2]), (lo[0], hi[1], lo[2]),
+             (lo[0], lo[1], hi[2]), (hi[0], lo[1], hi[2]), (hi[0], hi[1], hi[2]), (lo[0], hi[1], hi[2])]
+        out += ["v %s %s %s" % tuple("%g" % x for x in p) for p in c]
+        out.append(f"l {o} {o + 1} {o + 2} {o + 3} {o}")
+        out.append(f"l {o + 4} {o + 5} {o + 6} {o + 7} {o + 4}")
+        out += [f"l {o + e} {o + 4 + e}" for e in range(4)]
+        o += 8
+    return "\n".join(out) + "\n"
+
+
+def test_obj_export_matches_hand_written_text(tmp_path):
+    nodes = np.zeros(3, dtype=B.NODE_DTYPE)
+    nodes["bmin"] = [[0, 0, 0], [0.25, -1.5, 2], [1.125, 0, 3e-7]]
+    nodes["bmax"] = [[2, 1, 4], [1, 0.5, 3], [2, 1, 4]]
+    nodes["left"] = [1, np.int32(np.uint32(0x80000000).view(np.int32)), np.int32(np.uint32(0x80000001).view(np.int32))]
+    nodes["right"] = [2, -1, -1]
+    from bvh_amd import _native as N
+    p = str(tmp_path / "all.obj")
+    assert N.host().bvh_nodes_export_obj(nodes.ctypes.data, 3, p.encode(), 0) == 0
+    boxes = [(nodes["bmin"][i], nodes["bmax"][i]) for i in range(3)]
+    assert open(p).read() == _obj_text(boxes)
+    p = str(tmp_path / "leaves.obj")
+    assert N.host().bvh_nodes_export_obj(nodes.ctypes.data, 3, p.encode(), 1) == 0
+    assert open(p).read() == _obj_text(boxes[1:])
+
+
+def test_obj_export_leaves_of_ploc_tree(tmp_path):
+    """PLOC keeps leaves first (root swapped to 0): the leaf bit selects them,
+    where the reference's index test would pick internal nodes."""
+    import oracle as O
+    from bvh_amd import _native as N
+    m = B.Mesh.uniform(4, 200)
+    _, nodes, _, _ = O.build_ploc(m.soa, 8)
+    p = str(tmp_path / "ploc.obj")
+    assert N.host().bvh_nodes_export_obj(nodes.ctypes.data, nodes.size, p.encode(), 1) == 0
+    text = open(p).read()
+    leaf = np.nonzero(nodes["left"] < 0)[0]
+    assert text.count("\nl ") + text.startswith("l ") == 6 * leaf.size
+    assert text == _obj_text([(nodes["bmin"][i], nodes["bmax"][i]) for i in leaf])
+    wrong = [i for i in range(nodes.size) if i >= (nodes.size + 1) // 2 - 1]   # the reference's test
+    assert set(wrong) != set(leaf.tolist())

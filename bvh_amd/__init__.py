@@ -64,6 +64,23 @@ class Mesh:
         N.host().bvh_scene_batch_mesh(seed, index, n, N.soa_ptrs(soa))
         return cls(soa)
 
+    @classmethod
+    def from_obj(cls, path: str, threads: int = 0) -> "Mesh":
+        """Parses an OBJ file with the reference loader's rules (include/bvh_obj.h)."""
+        h = C.c_void_p()
+        st = N.host().bvh_obj_parse(path.encode(), threads, C.byref(h))
+        if st != 0:
+            raise NativeError(st, N.host().bvh_obj_last_error().decode())
+        try:
+            nt, nv = C.c_uint32(), C.c_uint64()
+            N.host().bvh_obj_info(h, C.byref(nt), C.byref(nv))
+            soa = cls._empty(nt.value)
+            if nt.value:
+                N.host().bvh_obj_fill(h, N.soa_ptrs(soa))
+            return cls(soa)
+        finally:
+            N.host().bvh_obj_free(h)
+
     def checksum(self) -> int:
         h = 14695981039346656037
         for k in range(9):
@@ -183,6 +200,16 @@ class Context:
         h = C.c_void_p()
         self._check(self._lib.bvh_mesh_upload(self.h, N.soa_ptrs(mesh.soa), mesh.n, C.byref(h)), "mesh upload")
         return DeviceMesh(self, h, mesh.n)
+
+    def upload_obj(self, path: str, threads: int = 0) -> "DeviceMesh":
+        """Parses an OBJ file into pinned staging memory and uploads it (bvh_mesh_upload_obj)."""
+        h = C.c_void_p()
+        st = N.host().bvh_mesh_upload_obj(self.h, path.encode(), threads, C.byref(h))
+        if st != N.BVH_OK:
+            raise NativeError(st, N.host().bvh_obj_last_error().decode() or self._lib.bvh_last_error(self.h).decode())
+        n = C.c_uint32()
+        self._lib.bvh_mesh_size(h, C.byref(n))
+        return DeviceMesh(self, h, n.value)
 
     def tree(self) -> "Tree":
         h = C.c_void_p()

@@ -97,6 +97,8 @@ _HIP_SIGS = {
     "bvh_device_free": (_i32, [_vp, _vp]),
     "bvh_copy_h2d": (_i32, [_vp, _vp, _vp, _sz]),
     "bvh_copy_d2h": (_i32, [_vp, _vp, _vp, _sz]),
+    "bvh_host_alloc_pinned": (_i32, [_sz, _pp]),
+    "bvh_host_free_pinned": (_i32, [_vp]),
     "bvh_mesh_upload": (_i32, [_vp, C.POINTER(_vp), _u32, _pp]),
     "bvh_mesh_free": (_i32, [_vp, _vp]),
     "bvh_mesh_size": (_i32, [_vp, C.POINTER(_u32)]),
@@ -145,6 +147,12 @@ _HOST_SIGS = {
     "bvh_tree_export_viz": (_i32, [_vp, _vp, C.c_char_p]),
     "bvh_nodes_export_obj": (_i32, [_vp, _u32, C.c_char_p, _i32]),
     "bvh_tree_export_obj": (_i32, [_vp, _vp, C.c_char_p, _i32]),
+    "bvh_obj_parse": (_i32, [C.c_char_p, _i32, _pp]),
+    "bvh_obj_info": (_i32, [_vp, C.POINTER(_u32), C.POINTER(_u64)]),
+    "bvh_obj_fill": (_i32, [_vp, C.POINTER(_vp)]),
+    "bvh_obj_free": (_i32, [_vp]),
+    "bvh_obj_last_error": (C.c_char_p, []),
+    "bvh_mesh_upload_obj": (_i32, [_vp, C.c_char_p, _i32, _pp]),
     "bvh_render_ppm": (_i32, [_vp, _u32, C.POINTER(_vp), _u32, _i32, _i32, _vp, _i32, _f, C.c_char_p, _vp]),
 }
 

@@ -1649,6 +1649,17 @@ int bvh_trace_host(bvh_ctx_t ctx, bvh_tree_t tree, bvh_mesh_t mesh, const float*
     return BVH_OK;
 }
 
+int bvh_host_alloc_pinned(size_t bytes, void** ptr) {
+    if (!ptr || bytes == 0) return BVH_E_ARG;
+    *ptr = nullptr;
+    return hipHostMalloc(ptr, bytes, hipHostMallocDefault) == hipSuccess ? BVH_OK : BVH_E_NOMEM;
+}
+
+int bvh_host_free_pinned(void* ptr) {
+    if (!ptr) return BVH_E_ARG;
+    return hipHostFree(ptr) == hipSuccess ? BVH_OK : BVH_E_HIP;
+}
+
 int bvh_debug_stream_copy(bvh_ctx_t ctx, void* d_dst, const void* d_src, size_t bytes, float* ms) {
     if (!ctx || !d_dst || !d_src || (bytes % 16) != 0) return BVH_E_ARG;
     HIP_TRY(ctx, hipEventRecord(ctx->ev[0], ctx->stream));

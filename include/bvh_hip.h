@@ -99,6 +99,10 @@ int bvh_device_alloc(bvh_ctx_t ctx, size_t bytes, void** dptr);
 int bvh_device_free(bvh_ctx_t ctx, void* dptr);
 int bvh_copy_h2d(bvh_ctx_t ctx, void* dst, const void* src, size_t bytes);
 int bvh_copy_d2h(bvh_ctx_t ctx, void* dst, const void* src, size_t bytes);
+/* Page-locked host memory (hipHostMalloc): copies from it to the device are
+ * DMA transfers with no pageable staging (the OBJ loader parses into it). */
+int bvh_host_alloc_pinned(size_t bytes, void** ptr);
+int bvh_host_free_pinned(void* ptr);
 
 /* ---- meshes ------------------------------------------------------------- */
 /* soa[0..8] = v0x v0y v0z v1x v1y v1z v2x v2y v2z, each n floats (host memory). */

@@ -293,6 +293,33 @@ def calib_copy(bytes_=256 << 20, reps=3):
                       "GBps": 2 * bytes_ / (ms.value * 1e-3) / 1e9}), flush=True)
 
 
+def run_sections(args):
+    """Selected workloads only, for profiling (kernel traces, counters)."""
+    import bvh_amd as B
+    want = {x.strip() for x in args.sections.split(",") if x.strip()}
+    unknown = want - {"companions", "batch", "large", "serve"}
+    if unknown:
+        raise SystemExit(f"unknown sections: {sorted(unknown)}")
+    out = {}
+    ctx = B.Context(0)
+    if "companions" in want:
+        log(f"[bench] generating {args.n:,} blob triangles (seed 7)")
+        mesh = B.Mesh.blobs(7, args.n)
+        dm = ctx.upload(mesh)
+        tree = ctx.tree()
+        out["config1_companions"] = extras(ctx, dm, tree, mesh, args.steps)
+        tree.free()
+        dm.free()
+    if "batch" in want:
+        out["batched_vs_plain"] = batched_vs_plain(ctx, args.steps)
+    if "large" in want:
+        out["large_32m_strands"] = large_strands(ctx, max(1, args.steps // 4))
+    ctx.close()
+    if "serve" in want:
+        out["serving"] = serving_latency()
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -306,9 +333,15 @@ def main():
     ap.add_argument("--no-large", action="store_true", help="skip the 32M-triangle strands workload")
     ap.add_argument("--calib", action="store_true",
                     help="only run the 256 MiB stream-copy kernel (reference point for HBM counters)")
+    ap.add_argument("--sections", default="",
+                    help="profiling: comma list of companions,batch,large,serve to run without the headline "
+                         "(prints their JSON only)")
     args = ap.parse_args()
     if args.calib:
         calib_copy()
+        return
+    if args.sections:
+        run_sections(args)
         return
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (this engine shards by mesh/ray "

@@ -73,6 +73,7 @@ struct bvh_context {
     Workspace ws;
     SegTables tabs;  // eager segmented builds
     DevBuf trace_ovf, trace_list, trace_io;  // traversal queue and host-call staging
+    DevBuf trace_spill;                      // spill slots of deep traversal stacks
     DevBuf ray_keys[2];                      // ray coherence keys (sorted before tracing)
     uint32_t* host_flags = nullptr;          // pinned: a single build's 8 flag words land here
     // last build's sorted keys (for bvh_debug_sorted_keys)
@@ -1015,11 +1016,12 @@ uint64_t footprint_trace(uint32_t nrays) {
     const size_t rb = (size_t)nrays * 24, tb = (size_t)nrays * 4, cb = (size_t)nrays * 12;
     Workspace w;
     w.each([](DevBuf& b) { b.measure = true; });
-    DevBuf io, ovf, list, k0, k1;
-    for (DevBuf* b : {&io, &ovf, &list, &k0, &k1}) b->measure = true;
+    DevBuf io, ovf, list, spill, k0, k1;
+    for (DevBuf* b : {&io, &ovf, &list, &spill, &k0, &k1}) b->measure = true;
     (void)ensure(nullptr, io, rb + 2 * tb + cb + 256);
     (void)ensure(nullptr, ovf, 64);
     (void)ensure(nullptr, list, (size_t)nrays * 4);
+    (void)ensure(nullptr, spill, (size_t)trace_spill_slots(nrays) * trace_spill_entries() * 8);
     if (nrays >= kRaySortMin) {
         (void)ensure(nullptr, k0, (size_t)nrays * 8);
         (void)ensure(nullptr, k1, (size_t)nrays * 8);
@@ -1027,7 +1029,7 @@ uint64_t footprint_trace(uint32_t nrays) {
         (void)ensure(nullptr, w.hist_tot, (size_t)plan.passes * (1u << plan.bits) * 4);
         (void)ensure(nullptr, w.lookback, zero_bytes_of(plan));
     }
-    return io.cap + ovf.cap + list.cap + k0.cap + k1.cap + measured(w);
+    return io.cap + ovf.cap + list.cap + spill.cap + k0.cap + k1.cap + measured(w);
 }
 
 }  // namespace
@@ -1185,6 +1187,7 @@ int bvh_ctx_destroy(bvh_ctx_t ctx) {
     ctx->tabs.release();
     free_buf(ctx->trace_ovf);
     free_buf(ctx->trace_list);
+    free_buf(ctx->trace_spill);
     free_buf(ctx->ray_keys[0]);
     free_buf(ctx->ray_keys[1]);
     free_buf(ctx->trace_io);
@@ -1432,7 +1435,8 @@ int bvh_ctx_held_bytes(bvh_ctx_t ctx, uint64_t* build_workspace, uint64_t* trace
     ctx->ws.each([&](DevBuf& d) { b += d.cap; });
     for (DevBuf* d : {&ctx->tabs.segs, &ctx->tabs.fblocks, &ctx->tabs.hblocks}) b += d->cap;
     uint64_t t = 0;
-    for (DevBuf* d : {&ctx->trace_ovf, &ctx->trace_list, &ctx->trace_io, &ctx->ray_keys[0], &ctx->ray_keys[1]})
+    for (DevBuf* d : {&ctx->trace_ovf, &ctx->trace_list, &ctx->trace_io, &ctx->trace_spill, &ctx->ray_keys[0],
+                      &ctx->ray_keys[1]})
         t += d->cap;
     if (build_workspace) *build_workspace = b;
     if (trace_workspace) *trace_workspace = t;
@@ -1573,16 +1577,18 @@ int bvh_trace(bvh_ctx_t ctx, bvh_tree_t tree, bvh_mesh_t mesh, const float* d_ra
     int st;
     if ((st = ensure(ctx, ctx->trace_ovf, 64))) return st;
     if ((st = ensure(ctx, ctx->trace_list, (size_t)nrays * 4))) return st;
+    if ((st = ensure(ctx, ctx->trace_spill, (size_t)trace_spill_slots(nrays) * trace_spill_entries() * 8))) return st;
     HIP_TRY(ctx, hipEventRecord(ctx->ev[0], ctx->stream));
     if ((st = prepare_trace(ctx, tree))) return st;
     TraceArgs a;
     a.order = nullptr;
     a.order_mask = 0;
-    // Large ray sets are traced in coherence order (direction bins, origin
-    // Morton code): neighbouring lanes then walk similar paths.
+    // Large ray sets are traced in coherence order (a 40-bit Morton code of
+    // origin and direction): neighbouring lanes then walk similar paths.
+    constexpr int kRayKeyBits = 40;
     const int idx_bits = std::max(1, bit_width(nrays - 1));
-    if (nrays >= kRaySortMin && idx_bits + 39 <= 64) {
-        SortPlan plan = make_sort_plan(nrays, idx_bits, idx_bits + 39, 8);
+    if (nrays >= kRaySortMin && idx_bits + kRayKeyBits <= 64) {
+        SortPlan plan = make_sort_plan(nrays, idx_bits, idx_bits + kRayKeyBits, 8);
         Workspace& w = ctx->ws;
         const size_t bins = 1u << plan.bits;
         TRY(ensure(ctx, ctx->ray_keys[0], (size_t)nrays * 8));
@@ -1614,6 +1620,8 @@ int bvh_trace(bvh_ctx_t ctx, bvh_tree_t tree, bvh_mesh_t mesh, const float* d_ra
     a.counters = d_counters;
     a.overflow_count = P<uint32_t>(ctx->trace_ovf);
     a.overflow_list = P<uint32_t>(ctx->trace_list);
+    a.spill = P<int2>(ctx->trace_spill);
+    a.spill_slots = trace_spill_slots(nrays);
     launch_fill_zero(a.overflow_count, 4, ctx->stream);
     launch_trace(a, ctx->stream);
     launch_trace_deep(a, ctx->stream);

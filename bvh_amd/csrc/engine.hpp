@@ -172,9 +172,15 @@ struct TraceArgs {
     float* t_out;
     int32_t* prim_out;
     int32_t* counters;          // optional: 3 per ray
-    uint32_t* overflow_count;   // [0] queued rays, [1] error flag (cleared by the caller)
+    uint32_t* overflow_count;   // [0] queued rays, [1] error flag, [2] batch counter (cleared by the caller)
     uint32_t* overflow_list;    // nrays entries
+    int2* spill;                // spill_slots x trace_spill_entries() stack entries
+    uint32_t spill_slots;
 };
+// Stack entries of one spill slot (the part of a lane's stack beyond LDS).
+int trace_spill_entries();
+// Spill slots bvh_trace provides for n rays.
+uint32_t trace_spill_slots(uint32_t nrays);
 void launch_tri_aos(const MeshView& m, float4* tri, hipStream_t s);
 // Coherence keys of rays (direction bins, origin Morton code, ray index).
 void launch_ray_keys(const float* rays, uint32_t n, const DNode* nodes, uint32_t idx_bits, uint64_t* keys,

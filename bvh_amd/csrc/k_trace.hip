@@ -19,11 +19,17 @@
 //     from the node array once per tree (k_trace_layout), so expanding a
 //     node costs one memory round trip instead of two dependent ones (node,
 //     then its children's boxes); leaf children carry their primitive;
-//   * the traversal stack is a short per-lane stack in LDS ([depth][lane],
-//     conflict-free), 24 entries of (node, tenter); a ray that would need a
-//     deeper stack is not truncated (the reference drops pushes past 128):
-//     it is queued and re-traced by k_trace_deep with a 256-entry stack (24
-//     entries in LDS, the rest in scratch);
+//   * the left child stays in registers and only right children are
+//     pushed; the stack holds (node, tenter) packed in 64 bits: 16 entries
+//     per lane in LDS ([depth][lane], conflict-free), the next 32 in the
+//     lane's own spill slot in HBM; a ray that would need a deeper stack is
+//     not truncated (the reference drops pushes past 128): it is queued and
+//     re-traced by k_trace_deep with a 256-entry stack (64 entries in LDS,
+//     the rest in scratch);
+//   * k_trace is persistent (one counter hands out 64-ray batches), so spill
+//     slots belong to resident lanes, not to rays;
+//   * the slab tests use the hardware min/max (exact for finite inputs, see
+//     slab_fast);
 //   * triangles are read from a 48-byte-per-triangle AoS copy of the mesh
 //     (three 16-byte loads instead of nine scattered 4-byte loads).
 #include "dev_common.hpp"
@@ -32,8 +38,13 @@
 namespace bvhk {
 
 constexpr int kTraceBlock = 64;
-constexpr int kShortStack = 24;
-constexpr int kDeepStack = 256;
+// Stack entries of a lane kept in LDS: 16 (8 KB per 64-lane block) measured
+// fastest against 12, 20 and 24 (fewer LDS bytes per ray let more rays be in
+// flight; fewer entries spill more often).
+constexpr int kShortStack = 16;
+constexpr int kSpill = 32;        // entries of a lane's HBM spill slot beyond LDS
+constexpr int kMaxStack = 256;    // all stack entries of a lane (k_trace_deep)
+constexpr int kDeepLds = 64;      // LDS entries of k_trace_deep (32 KB per 64-lane block)
 
 struct RayState {
     float ox, oy, oz, dx, dy, dz, ix, iy, iz;
@@ -41,36 +52,6 @@ struct RayState {
 
 __device__ __forceinline__ float inv_dir(float d) {
     return (fabsf(d) > 1e-6f) ? 1.0f / d : (d >= 0.0f ? 1e30f : -1e30f);
-}
-
-// Slab test part of intersectAABB (render_engine.cu:194-214); returns tenter,
-// sets hit when texit >= max(tenter, 0).
-__device__ __forceinline__ float slab(const RayState& r, const DNode* nodes, int32_t i, bool& hit) {
-    const float4 mn = nodes[i].mn;
-    const float4 mx = nodes[i].mx;
-    float tx1 = (mn.x - r.ox) * r.ix, tx2 = (mx.x - r.ox) * r.ix;
-    float ty1 = (mn.y - r.oy) * r.iy, ty2 = (mx.y - r.oy) * r.iy;
-    float tz1 = (mn.z - r.oz) * r.iz, tz2 = (mx.z - r.oz) * r.iz;
-    float tminx = fmin_s(tx1, tx2), tmaxx = fmax_s(tx1, tx2);
-    float tminy = fmin_s(ty1, ty2), tmaxy = fmax_s(ty1, ty2);
-    float tminz = fmin_s(tz1, tz2), tmaxz = fmax_s(tz1, tz2);
-    float tenter = fmax_s(fmax_s(tminx, tminy), tminz);
-    float texit = fmin_s(fmin_s(tmaxx, tmaxy), tmaxz);
-    hit = texit >= fmax_s(tenter, 0.0f);
-    return tenter;
-}
-
-__device__ __forceinline__ float slab_box(const RayState& r, const float4 mn, const float4 mx, bool& hit) {
-    float tx1 = (mn.x - r.ox) * r.ix, tx2 = (mx.x - r.ox) * r.ix;
-    float ty1 = (mn.y - r.oy) * r.iy, ty2 = (mx.y - r.oy) * r.iy;
-    float tz1 = (mn.z - r.oz) * r.iz, tz2 = (mx.z - r.oz) * r.iz;
-    float tminx = fmin_s(tx1, tx2), tmaxx = fmax_s(tx1, tx2);
-    float tminy = fmin_s(ty1, ty2), tmaxy = fmax_s(ty1, ty2);
-    float tminz = fmin_s(tz1, tz2), tmaxz = fmax_s(tz1, tz2);
-    float tenter = fmax_s(fmax_s(tminx, tminy), tminz);
-    float texit = fmin_s(fmin_s(tmaxx, tmaxy), tmaxz);
-    hit = texit >= fmax_s(tenter, 0.0f);
-    return tenter;
 }
 
 // Child reference in a traversal record: node index (>= 0), leaf
@@ -155,74 +136,125 @@ __device__ __forceinline__ RayState load_ray(const float* rays, uint32_t i) {
     return r;
 }
 
-// Stack accessors: LDS short stack or private deep stack.
-struct LdsStack {
-    int32_t (*idx)[kTraceBlock];
-    float (*tent)[kTraceBlock];
+// Stack entries are (node reference, entry distance) packed in 64 bits.
+//
+// LaneStack (k_trace): the first LDS entries of a lane in LDS ([depth][lane]
+// rows of 8-byte entries, conflict-free), the next kSpill in the lane's own
+// spill slot in HBM (k_trace is persistent, so a slot per resident lane
+// serves every ray that lane traces; only deep rays touch it); a ray that
+// needs more is queued for k_trace_deep.  DeepStack (k_trace_deep): the
+// first kDeepLds entries in LDS, the rest in a private array (scratch).
+template <int LDS>
+struct LaneStack {
+    int2 (*e)[kTraceBlock];
     int lane;
-    static constexpr int kCap = kShortStack;
-    __device__ void put(int sp, int32_t i, float t) { idx[sp][lane] = i; tent[sp][lane] = t; }
-    __device__ void get(int sp, int32_t& i, float& t) const { i = idx[sp][lane]; t = tent[sp][lane]; }
-};
-// Deep stack for rays that outgrew the short stack: the first kShortStack
-// entries in LDS, the rest in a private array (scratch memory).
-struct DeepStack {
-    int32_t (*idx)[kTraceBlock];
-    float (*tent)[kTraceBlock];
-    int lane;
-    int32_t deep_idx[kDeepStack - kShortStack];
-    float deep_tent[kDeepStack - kShortStack];
-    static constexpr int kCap = kDeepStack;
-    __device__ void put(int sp, int32_t i, float t) {
-        if (sp < kShortStack) {
-            idx[sp][lane] = i;
-            tent[sp][lane] = t;
-        } else {
-            deep_idx[sp - kShortStack] = i;
-            deep_tent[sp - kShortStack] = t;
-        }
+    int2* slot;  // kSpill entries
+    static constexpr int kCap = LDS + kSpill;
+    __device__ bool put(int sp, int32_t i, float t) {
+        const int2 v = make_int2(i, __float_as_int(t));
+        if (sp < LDS) e[sp][lane] = v;
+        else slot[sp - LDS] = v;
+        return true;
     }
     __device__ void get(int sp, int32_t& i, float& t) const {
-        if (sp < kShortStack) {
-            i = idx[sp][lane];
-            t = tent[sp][lane];
-        } else {
-            i = deep_idx[sp - kShortStack];
-            t = deep_tent[sp - kShortStack];
-        }
+        const int2 v = sp < LDS ? e[sp][lane] : slot[sp - LDS];
+        i = v.x;
+        t = __int_as_float(v.y);
     }
 };
 
-// Returns false when the stack capacity was exceeded (result not written).
-template <class Stack>
-__device__ __forceinline__ bool traverse(const TraceArgs& a, const RayState& r, Stack& st, float& best,
-                                         int32_t& prim, int32_t cnt[3]) {
-    const DNode* __restrict__ nodes = a.nodes;
-    const int32_t nn = (int32_t)a.num_nodes;
-    best = 1e30f;
-    prim = -1;
-    int32_t visited = 0, aabb = 0, tris = 0;
-    int sp = 0;
-    int64_t budget = 2 * (int64_t)nn + 2;  // a tree pops each node at most once
-    if (nn > 0) {
-        bool h;
-        float te = slab(r, nodes, 0, h);
-        ++visited;
-        ++aabb;
-        if (h) {
-            const int32_t l0 = nodes[0].lr.x;
-            st.put(0, l0 < 0 ? l0 : 0, te);
-            sp = 1;
+struct DeepStack {
+    int2 (*e)[kTraceBlock];
+    int lane;
+    int2 deep[kMaxStack - kDeepLds];
+    static constexpr int kCap = kMaxStack;
+    __device__ bool put(int sp, int32_t i, float t) {
+        const int2 v = make_int2(i, __float_as_int(t));
+        if (sp < kDeepLds) e[sp][lane] = v;
+        else deep[sp - kDeepLds] = v;
+        return true;
+    }
+    __device__ void get(int sp, int32_t& i, float& t) const {
+        const int2 v = sp < kDeepLds ? e[sp][lane] : deep[sp - kDeepLds];
+        i = v.x;
+        t = __int_as_float(v.y);
+    }
+};
+
+// Slab test with the hardware min/max: rays and boxes are finite (checked on
+// input; an inverse direction component is at most 1e30 in magnitude), so
+// no operand is NaN, and min/max may order -0 and +0 either way: no
+// comparison below can tell the two apart, so every decision equals the
+// reference's (its intersectAABB, render_engine.cu:194-214).
+__device__ __forceinline__ float slab_fast(const RayState& r, const float4 mn, const float4 mx, bool& hit) {
+    const float tx1 = (mn.x - r.ox) * r.ix, tx2 = (mx.x - r.ox) * r.ix;
+    const float ty1 = (mn.y - r.oy) * r.iy, ty2 = (mx.y - r.oy) * r.iy;
+    const float tz1 = (mn.z - r.oz) * r.iz, tz2 = (mx.z - r.oz) * r.iz;
+    const float tenter = fmaxf(fmaxf(fminf(tx1, tx2), fminf(ty1, ty2)), fminf(tz1, tz2));
+    const float texit = fminf(fminf(fmaxf(tx1, tx2), fmaxf(ty1, ty2)), fmaxf(tz1, tz2));
+    hit = texit >= fmaxf(tenter, 0.0f);
+    return tenter;
+}
+
+// One ray's traversal, advanced one stack pop at a time.
+//
+// The reference's depth-first order: children pushed right then left, so
+// the left one is popped next; a popped entry is skipped unless its entry
+// distance is below the current closest t.  The left child would be popped
+// right after its push with the same closest t, so it is kept in registers
+// (`cur`) and only right children go through the stack: the same decisions
+// in the same order.  Children are counted when their parent is expanded:
+// the reference pops (and counts) every child it pushes.  (A "while-while"
+// split into internal and leaf phases was measured 2.6x slower: without
+// postponing leaves, which would change the culls, each phase waits for the
+// wave's slowest lane.)
+struct Walker {
+    RayState r;
+    float best;
+    int32_t prim, visited, tris;
+    int sp;
+    bool have;       // cur holds the next entry
+    int32_t cur;
+    float cur_te;
+    int64_t budget;  // a tree pops each node at most once
+
+    __device__ void start(const TraceArgs& a, const RayState& ray) {
+        r = ray;
+        best = 1e30f;
+        prim = -1;
+        visited = 0;
+        tris = 0;
+        sp = 0;
+        have = false;
+        cur = 0;
+        cur_te = 0.0f;
+        budget = 2 * (int64_t)a.num_nodes + 2;
+        if (a.num_nodes > 0) {
+            bool h;
+            cur_te = slab_fast(r, a.nodes[0].mn, a.nodes[0].mx, h);
+            visited = 1;
+            const int32_t l0 = a.nodes[0].lr.x;
+            cur = l0 < 0 ? l0 : 0;
+            have = h;
         }
     }
-    const float4* __restrict__ rec = a.rec;
-    while (sp > 0) {
+    __device__ bool done() const { return !have && sp == 0; }
+
+    // One pop; false when the stack capacity or the pop budget ran out.
+    template <class Stack>
+    __device__ __forceinline__ bool step(const TraceArgs& a, Stack& st) {
         if (--budget < 0) return false;
-        --sp;
         int32_t ref;
         float te;
-        st.get(sp, ref, te);
-        if (!(te < best)) continue;
+        if (have) {
+            ref = cur;
+            te = cur_te;
+            have = false;
+        } else {
+            --sp;
+            st.get(sp, ref, te);
+        }
+        if (!(te < best)) return true;
         if (ref < 0) {
             ++tris;
             float t;
@@ -231,32 +263,43 @@ __device__ __forceinline__ bool traverse(const TraceArgs& a, const RayState& r, 
                 best = t;
                 prim = (int32_t)p;
             }
-            continue;
+            return true;
         }
-        const float4* q = rec + 4 * (size_t)ref;
+        const float4* q = a.rec + 4 * (size_t)ref;
         const float4 A = q[0], B = q[1], C = q[2], D = q[3];
         const int32_t lref = __float_as_int(A.w);
         const int32_t rref = __float_as_int(B.w);
-        bool hr = false, hl = false;
-        float tr = 0.0f, tl = 0.0f;
-        if (rref != kAbsent) {
-            tr = slab_box(r, C, D, hr);
-            ++visited;
-            ++aabb;
+        const bool lp = lref != kAbsent, rp = rref != kAbsent;
+        bool hl, hr;
+        const float tl = slab_fast(r, A, B, hl);
+        const float tr = slab_fast(r, C, D, hr);
+        hl = hl && lp;
+        hr = hr && rp;
+        visited += (lp ? 1 : 0) + (rp ? 1 : 0);
+        if (hr) {
+            if (sp >= Stack::kCap || !st.put(sp, rref, tr)) return false;
+            ++sp;
         }
-        if (lref != kAbsent) {
-            tl = slab_box(r, A, B, hl);
-            ++visited;
-            ++aabb;
-        }
-        const int need = (hr ? 1 : 0) + (hl ? 1 : 0);
-        if (sp + need > Stack::kCap) return false;
-        if (hr) st.put(sp++, rref, tr);
-        if (hl) st.put(sp++, lref, tl);
+        cur = lref;
+        cur_te = tl;
+        have = hl;
+        return true;
     }
-    cnt[0] = visited;
-    cnt[1] = aabb;
-    cnt[2] = tris;
+};
+
+// Whole traversal of one ray; false when the stack capacity was exceeded.
+template <class Stack>
+__device__ __forceinline__ bool traverse(const TraceArgs& a, const RayState& r, Stack& st, float& best,
+                                         int32_t& prim, int32_t cnt[3]) {
+    Walker w;
+    w.start(a, r);
+    while (!w.done())
+        if (!w.step(a, st)) return false;
+    best = w.best;
+    prim = w.prim;
+    cnt[0] = w.visited;
+    cnt[1] = w.visited;
+    cnt[2] = w.tris;
     return true;
 }
 
@@ -276,50 +319,78 @@ __device__ __forceinline__ uint32_t ray_at(const TraceArgs& a, uint32_t k) {
     return a.order ? (uint32_t)(a.order[k] & a.order_mask) : k;
 }
 
+// Persistent: each one-wave block takes batches of kTraceBlock consecutive
+// rays (in coherence order) from one counter until none are left, so the
+// rays of a batch are traced side by side and waves stay busy to the end.
+// Measured and not kept: refilling idle lanes one by one from the counter
+// (a wave then traces a sliding window of rays) and splitting the rays into
+// one contiguous part per XCD (each XCD's L2 then serves one region); both
+// were slower.
+template <int LDS>
 __global__ __launch_bounds__(kTraceBlock) void k_trace(TraceArgs a) {
-    __shared__ int32_t s_idx[kShortStack][kTraceBlock];
-    __shared__ float s_tent[kShortStack][kTraceBlock];
-    const uint32_t k = blockIdx.x * kTraceBlock + threadIdx.x;
-    if (k >= a.nrays) return;
-    const uint32_t i = ray_at(a, k);
-    const RayState r = load_ray(a.rays, i);
-    LdsStack st{s_idx, s_tent, (int)threadIdx.x};
-    float t;
-    int32_t p;
-    int32_t c[3] = {0, 0, 0};
-    if (traverse(a, r, st, t, p, c)) {
-        write_result(a, i, t, p, c);
-    } else {
-        const uint32_t slot = atomicAdd(a.overflow_count, 1u);
-        a.overflow_list[slot] = i;
+    __shared__ int2 s_stack[LDS][kTraceBlock];
+    LaneStack<LDS> st;
+    st.e = s_stack;
+    st.lane = (int)threadIdx.x;
+    st.slot = a.spill + ((size_t)blockIdx.x * kTraceBlock + threadIdx.x) * kSpill;
+    while (true) {
+        uint32_t b = 0;
+        if (threadIdx.x == 0) b = atomicAdd(a.overflow_count + 2, 1u);
+        b = (uint32_t)__builtin_amdgcn_readfirstlane((int)__shfl(b, 0));
+        if ((uint64_t)b * kTraceBlock >= a.nrays) break;
+        const uint64_t k = (uint64_t)b * kTraceBlock + threadIdx.x;
+        if (k < a.nrays) {
+            const uint32_t i = ray_at(a, (uint32_t)k);
+            const RayState r = load_ray(a.rays, i);
+            float t;
+            int32_t p;
+            int32_t c[3] = {0, 0, 0};
+            if (traverse(a, r, st, t, p, c)) {
+                write_result(a, i, t, p, c);
+            } else {
+                const uint32_t q = atomicAdd(a.overflow_count, 1u);
+                a.overflow_list[q] = i;
+            }
+        }
     }
 }
 
-// Rays whose stack outgrew the LDS short stack; grid-stride over the queue
-// (its length is read on the device, so no host round trip is needed).
+// Rays whose stack outgrew LDS and the spill slot; grid-stride over
+// the queue (its length is read on the device, so no host round trip).
 __global__ __launch_bounds__(kTraceBlock) void k_trace_deep(TraceArgs a) {
-    __shared__ int32_t s_idx[kShortStack][kTraceBlock];
-    __shared__ float s_tent[kShortStack][kTraceBlock];
+    __shared__ int2 s_stack[kDeepLds][kTraceBlock];
     const uint32_t count = *a.overflow_count;
     for (uint32_t k = blockIdx.x * kTraceBlock + threadIdx.x; k < count; k += gridDim.x * kTraceBlock) {
         const uint32_t i = a.overflow_list[k];
         const RayState r = load_ray(a.rays, i);
         DeepStack st;
-        st.idx = s_idx;
-        st.tent = s_tent;
+        st.e = s_stack;
         st.lane = (int)threadIdx.x;
         float t;
         int32_t p;
         int32_t c[3] = {0, 0, 0};
         if (traverse(a, r, st, t, p, c)) write_result(a, i, t, p, c);
-        else atomicOr(a.overflow_count + 1, 1u);  // deeper than 256 or not a tree: reported as an error
+        else atomicOr(a.overflow_count + 1, 1u);  // deeper than kMaxStack or not a tree: reported as an error
     }
 }
 
-// Coherence keys: 9 bits of direction (3 per axis, direction scaled by its
-// largest component) above a 30-bit Morton code of the origin in the root box,
-// above the ray index; sorting them makes neighbouring lanes trace similar
-// rays.  Order only: every ray's result is independent of where it runs.
+// Coherence keys: a 40-bit Morton code of the ray in five dimensions (origin
+// in the root box, 8 bits per axis; direction in octahedral coordinates, 8
+// bits per axis) above the ray index, so that sorting them puts rays with
+// nearby origins and directions in the same wave (camera rays, which share
+// their origin, come out in pixel tiles).  Order only: every ray's result is
+// independent of where it runs.
+__device__ __forceinline__ uint64_t spread5(uint32_t v) {  // 8 bits -> every fifth bit of 40
+    uint64_t r = 0;
+#pragma unroll
+    for (int b = 0; b < 8; ++b) r |= (uint64_t)((v >> b) & 1u) << (5 * b);
+    return r;
+}
+
+__device__ __forceinline__ uint32_t quant8(float x) {  // [0, 1] -> 0..255
+    return (uint32_t)fmin_s(fmax_s(x * 256.0f, 0.0f), 255.0f);
+}
+
 __global__ void k_ray_keys(const float* __restrict__ rays, uint32_t n, const DNode* __restrict__ nodes,
                            uint32_t idx_bits, uint64_t* __restrict__ keys) {
     const float4 mn = nodes[0].mn, mx = nodes[0].mx;
@@ -328,20 +399,22 @@ __global__ void k_ray_keys(const float* __restrict__ rays, uint32_t n, const DNo
         const float* r = rays + 6 * (size_t)i;
         const float ox = r[0], oy = r[1], oz = r[2], dx = r[3], dy = r[4], dz = r[5];
         const float ex = mx.x - mn.x, ey = mx.y - mn.y, ez = mx.z - mn.z;
-        float x = (ox - mn.x) / (ex > 1e-6f ? ex : 1.0f);
-        float y = (oy - mn.y) / (ey > 1e-6f ? ey : 1.0f);
-        float z = (oz - mn.z) / (ez > 1e-6f ? ez : 1.0f);
-        x = fmin_s(fmax_s(x, 0.0f), 1.0f);
-        y = fmin_s(fmax_s(y, 0.0f), 1.0f);
-        z = fmin_s(fmax_s(z, 0.0f), 1.0f);
-        const uint64_t m = morton30(x, y, z);
-        float dm = fmax_s(fabsf(dx), fmax_s(fabsf(dy), fabsf(dz)));
-        dm = dm > 0.0f ? dm : 1.0f;
-        const uint32_t qx = (uint32_t)fmin_s(fmax_s((dx / dm + 1.0f) * 4.0f, 0.0f), 7.0f);
-        const uint32_t qy = (uint32_t)fmin_s(fmax_s((dy / dm + 1.0f) * 4.0f, 0.0f), 7.0f);
-        const uint32_t qz = (uint32_t)fmin_s(fmax_s((dz / dm + 1.0f) * 4.0f, 0.0f), 7.0f);
-        const uint64_t dk = (uint64_t)((qx << 6) | (qy << 3) | qz);
-        keys[i] = (((dk << 30) | m) << idx_bits) | i;
+        const float x = (ox - mn.x) / (ex > 1e-6f ? ex : 1.0f);
+        const float y = (oy - mn.y) / (ey > 1e-6f ? ey : 1.0f);
+        const float z = (oz - mn.z) / (ez > 1e-6f ? ez : 1.0f);
+        // Octahedral map of the direction onto [-1, 1]^2, then [0, 1]^2.
+        const float l1 = fabsf(dx) + fabsf(dy) + fabsf(dz);
+        const float inv = l1 > 0.0f ? 1.0f / l1 : 0.0f;
+        float u = dx * inv, v = dy * inv;
+        if (dz * inv < 0.0f) {
+            const float uu = (1.0f - fabsf(v)) * (u >= 0.0f ? 1.0f : -1.0f);
+            const float vv = (1.0f - fabsf(u)) * (v >= 0.0f ? 1.0f : -1.0f);
+            u = uu;
+            v = vv;
+        }
+        const uint64_t m = (spread5(quant8(x)) << 4) | (spread5(quant8(y)) << 3) | (spread5(quant8(z)) << 2) |
+                           (spread5(quant8(u * 0.5f + 0.5f)) << 1) | spread5(quant8(v * 0.5f + 0.5f));
+        keys[i] = (m << idx_bits) | i;
     }
 }
 
@@ -368,10 +441,39 @@ void launch_tri_aos(const MeshView& m, float4* tri, hipStream_t s) {
     hipLaunchKernelGGL(k_tri_aos, dim3(g ? g : 1), dim3(256), 0, s, m, tri);
 }
 
-void launch_trace(const TraceArgs& a, hipStream_t s) {
-    uint32_t g = (a.nrays + kTraceBlock - 1) / kTraceBlock;
-    hipLaunchKernelGGL(k_trace, dim3(g ? g : 1), dim3(kTraceBlock), 0, s, a);
+template <int LDS>
+uint32_t resident_blocks() {
+    static const uint32_t nb = [] {
+        int dev = 0, cus = 0, per = 0;
+        (void)hipGetDevice(&dev);
+        (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+        (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, k_trace<LDS>, kTraceBlock, 0);
+        const uint32_t v = (uint32_t)(cus > 0 ? cus : 256) * (uint32_t)(per > 0 ? per : 8);
+        return v;
+    }();
+    return nb;
 }
+
+int trace_spill_entries() { return kSpill; }
+
+// One slot per lane of the grid launch_trace uses for n rays: at most one
+// 64-lane block per 64 rays, at most the resident blocks.
+uint32_t trace_spill_slots(uint32_t nrays) {
+    const uint32_t batches = (nrays + kTraceBlock - 1) / kTraceBlock;
+    const uint32_t g = resident_blocks<kShortStack>();
+    return (batches < g ? batches : g) * kTraceBlock;
+}
+
+template <int LDS>
+void launch_trace_t(const TraceArgs& a, hipStream_t s) {
+    const uint32_t batches = (a.nrays + kTraceBlock - 1) / kTraceBlock;
+    uint32_t g = resident_blocks<LDS>();
+    if (g > batches) g = batches;
+    if (g > a.spill_slots / kTraceBlock) g = a.spill_slots / kTraceBlock;
+    hipLaunchKernelGGL(k_trace<LDS>, dim3(g ? g : 1), dim3(kTraceBlock), 0, s, a);
+}
+
+void launch_trace(const TraceArgs& a, hipStream_t s) { launch_trace_t<kShortStack>(a, s); }
 
 void launch_trace_deep(const TraceArgs& a, hipStream_t s) {
     hipLaunchKernelGGL(k_trace_deep, dim3(1024), dim3(kTraceBlock), 0, s, a);

@@ -37,14 +37,14 @@ struct DevBuf {
 struct Workspace {
     DevBuf tri_boxes, keys_a, keys_b, vals_a, vals_b, hist_tot, lookback, scene, handoff, epoch;
     DevBuf hunits[2], hstatus, hctl, hcounts;  // LBVH/LBVH+ hierarchy levels
-    DevBuf parents, ranges, counters;  // LBVH+
+    DevBuf parents, tl_list, counters;  // LBVH+: parent links, ready lists + list lengths, arrival words
     DevBuf ploc_box[2], ploc_id[2], ploc_sid[2], ploc_seg[2], ploc_nn, ploc_state, ploc_status, ploc_out;
     template <class F>
     void each(F f) {
         DevBuf* all[] = {&tri_boxes,   &keys_a,      &keys_b,      &vals_a,      &vals_b,     &hist_tot,
                          &lookback,    &scene,       &handoff,     &epoch,       &hunits[0],
                          &hunits[1],   &hstatus,     &hctl,        &hcounts,     &parents,
-                         &ranges,      &counters,    &ploc_box[0], &ploc_box[1], &ploc_id[0], &ploc_id[1],
+                         &tl_list,     &counters,    &ploc_box[0], &ploc_box[1], &ploc_id[0], &ploc_id[1],
                          &ploc_sid[0], &ploc_sid[1], &ploc_seg[0], &ploc_seg[1], &ploc_nn,    &ploc_state,
                          &ploc_status, &ploc_out};
         for (DevBuf* b : all) f(*b);
@@ -381,7 +381,6 @@ HierArgs hier_args(Workspace& w, const Job& j, const uint64_t* keys, const uint3
     ha.kf = j.kf;
     ha.tri_boxes = P<float4>(w.tri_boxes);
     ha.parents = links ? P<uint32_t>(w.parents) : nullptr;
-    ha.ranges = links ? P<int2>(w.ranges) : nullptr;
     ha.handoff = P<uint64_t>(w.handoff);
     ha.epoch = P<uint32_t>(w.epoch);
     ha.err = P<uint32_t>(w.scene) + 7;
@@ -449,6 +448,15 @@ int build_lbvh(bvh_ctx_t ctx, bvh_mesh_t mesh, const bvh_build_params* prm, bvh_
     return check_input_flags(ctx, w, 1);
 }
 
+// LBVH+ round schedule: ready lists of at least kTreeletMinList nodes are
+// processed level-synchronously (one launch per list, up to kTreeletLevels),
+// the rest by climbing chains.  The lists hold each internal node once; the
+// list lengths follow at a 256-byte boundary.
+constexpr int kTreeletLevels = 16;
+constexpr uint32_t kTreeletMinList = 8192;
+size_t treelet_list_words(uint32_t n) { return ((size_t)n + 63) & ~(size_t)63; }
+size_t treelet_list_bytes(uint32_t n) { return (treelet_list_words(n) + kTreeletLevels + 2) * 4; }
+
 int build_lbvh_plus(bvh_ctx_t ctx, bvh_mesh_t mesh, const bvh_build_params* prm, bvh_tree_t tree,
                     bvh_stage_times* times) {
     const uint32_t n = mesh->n;
@@ -460,7 +468,7 @@ int build_lbvh_plus(bvh_ctx_t ctx, bvh_mesh_t mesh, const bvh_build_params* prm,
     Workspace& w = ctx->ws;
     TRY(reserve_front(ctx, w, j, plan, true));
     TRY(ensure(ctx, w.parents, (size_t)(2 * (size_t)n) * 4));
-    TRY(ensure(ctx, w.ranges, (size_t)n * 8));
+    TRY(ensure(ctx, w.tl_list, treelet_list_bytes(n)));
     TRY(ensure(ctx, w.counters, (size_t)n * 4, true));
     StageClock clk{ctx, times != nullptr};
     TRY(clk.begin());
@@ -478,9 +486,12 @@ int build_lbvh_plus(bvh_ctx_t ctx, bvh_mesh_t mesh, const bvh_build_params* prm,
         ta.n = n;
         ta.nodes = P<DNode>(tree->nodes);
         ta.parents = P<uint32_t>(w.parents);
-        ta.ranges = P<int2>(w.ranges);
         ta.counters = P<uint32_t>(w.counters);
+        ta.list = P<uint32_t>(w.tl_list);
+        ta.cnt = ta.list + treelet_list_words(n);
         ta.epoch = ++ctx->treelet_epoch;
+        ta.min_count = kTreeletMinList;
+        ta.levels = kTreeletLevels;
         launch_treelet(ta, ctx->stream);
         TRY(check_launch(ctx, "k_treelet"));
         if (r < 7 || r == rounds - 1) TRY(clk.mark(round_names[r < 7 ? r : 7]));
@@ -1004,7 +1015,7 @@ uint64_t footprint_build(int algo, uint32_t ntri, uint32_t meshes) {
     (void)reserve_front(nullptr, w, j, plan, algo != BVH_ALGO_PLOC);
     if (algo == BVH_ALGO_LBVH_PLUS) {
         (void)ensure(nullptr, w.parents, (size_t)(2 * (size_t)ntri) * 4);
-        (void)ensure(nullptr, w.ranges, (size_t)ntri * 8);
+        (void)ensure(nullptr, w.tl_list, treelet_list_bytes(ntri));
         (void)ensure(nullptr, w.counters, (size_t)ntri * 4);
     }
     if (algo == BVH_ALGO_PLOC) (void)reserve_ploc(nullptr, w, j);

@@ -136,7 +136,6 @@ struct HierArgs {
     KeyFormat kf;
     const float4* tri_boxes;
     uint32_t* parents;      // optional, 2n-1 (single segment)
-    int2* ranges;           // optional, n-1 (single segment)
     uint64_t* handoff;      // batch-wide, epoch-tagged (final climbing pass)
     const uint32_t* epoch;  // device word advanced by launch_init_build
     uint32_t* err;          // set to nonzero when a sorted key names no primitive
@@ -195,10 +194,14 @@ struct TreeletArgs {
     uint32_t n;
     DNode* nodes;        // 2n-1
     uint32_t* parents;   // 2n-1
-    int2* ranges;        // n-1: min/max sorted leaf position under each internal node
     uint32_t* counters;  // n-1 arrival words (2*epoch+1 tags, never cleared)
+    uint32_t* list;      // n-1: ready nodes, list 1, list 2, ... back to back
+    uint32_t* cnt;       // levels + 2 list lengths (cleared by launch_treelet)
     uint32_t epoch;
+    uint32_t min_count;  // lists at least this long run level-synchronously
+    int levels;          // level launches per round; the tail climbs from there
 };
+// One restructuring round: seed, `levels` level launches, the climbing tail.
 void launch_treelet(const TreeletArgs& a, hipStream_t s);
 
 // ---- PLOC (k_ploc.hip) -------------------------------------------------------

@@ -309,7 +309,6 @@ __device__ void hier_chunk_body(const HierArgs& p, HierSmem& s, uint32_t t, uint
             p.parents[lc] = idx;
             p.parents[rc] = idx;
         }
-        if (p.ranges) p.ranges[idx] = make_int2((int)l, (int)r);
     }
 
     // ---- segments: the next level's units ------------------------------------
@@ -557,7 +556,6 @@ __global__ __launch_bounds__(256) void k_hier_climb(HierArgs p, int level) {
                 p.parents[lc] = pidx;
                 p.parents[rc] = pidx;
             }
-            if (p.ranges) p.ranges[pidx] = make_int2((int)l, (int)r);
             my = pidx;
             box = pb;
         }

@@ -448,12 +448,13 @@ int build_lbvh(bvh_ctx_t ctx, bvh_mesh_t mesh, const bvh_build_params* prm, bvh_
     return check_input_flags(ctx, w, 1);
 }
 
-// LBVH+ round schedule: ready lists of at least kTreeletMinList nodes are
-// processed level-synchronously (one launch per list, up to kTreeletLevels),
-// the rest by climbing chains.  The lists hold each internal node once; the
-// list lengths follow at a 256-byte boundary.
+// LBVH+ round schedule: ready lists of at least kTreeletLevelMin nodes take
+// one launch each with one treelet per lane, up to kTreeletLevels launches;
+// the first shorter list seeds the climbing chains that finish the tree.
+// The lists hold each internal node once; the list lengths follow at a
+// 256-byte boundary.
 constexpr int kTreeletLevels = 16;
-constexpr uint32_t kTreeletMinList = 8192;
+constexpr uint32_t kTreeletLevelMin = 8192;
 size_t treelet_list_words(uint32_t n) { return ((size_t)n + 63) & ~(size_t)63; }
 size_t treelet_list_bytes(uint32_t n) { return (treelet_list_words(n) + kTreeletLevels + 2) * 4; }
 
@@ -490,7 +491,7 @@ int build_lbvh_plus(bvh_ctx_t ctx, bvh_mesh_t mesh, const bvh_build_params* prm,
         ta.list = P<uint32_t>(w.tl_list);
         ta.cnt = ta.list + treelet_list_words(n);
         ta.epoch = ++ctx->treelet_epoch;
-        ta.min_count = kTreeletMinList;
+        ta.level_min = kTreeletLevelMin;
         ta.levels = kTreeletLevels;
         launch_treelet(ta, ctx->stream);
         TRY(check_launch(ctx, "k_treelet"));

@@ -198,8 +198,8 @@ struct TreeletArgs {
     uint32_t* list;      // n-1: ready nodes, list 1, list 2, ... back to back
     uint32_t* cnt;       // levels + 2 list lengths (cleared by launch_treelet)
     uint32_t epoch;
-    uint32_t min_count;  // lists at least this long run level-synchronously
-    int levels;          // level launches per round; the tail climbs from there
+    uint32_t level_min;  // lists at least this long run level-synchronously; the first shorter seeds the tail
+    int levels;          // level launches per round
 };
 // One restructuring round: seed, `levels` level launches, the climbing tail.
 void launch_treelet(const TreeletArgs& a, hipStream_t s);

@@ -31,25 +31,24 @@
 //     maximum of an epoch tag on the node's arrival word) append the node to
 //     list 1.
 //   * k_tl_level<NM>, one launch per list L = 1..levels while the list holds
-//     at least min_count nodes: one treelet per lane.  A node of list L has
+//     at least level_min nodes: one treelet per lane.  A node of list L has
 //     at most 2^(L+1) leaves below it, so list 1 needs 4-leaf treelets and
-//     later lists 7.  Formation keeps the leaf boxes and ids in registers;
-//     the subset areas come from a depth-first walk over ascending leaf
-//     additions (the fold order) and the dynamic programme is unrolled at
-//     compile time over subsets in increasing order (every proper subset of s
-//     is smaller than s), its costs in registers and its partitions in
-//     lane-private LDS; the wave stops at the largest treelet among its
-//     lanes.  Second arrivals at the parents go to list L+1 through one
-//     atomic per wave.  Records are written with plain stores: the next
-//     launch reads them.
-//   * k_tl_tail: the first list below min_count (or list levels+1) seeds
-//     climbing chains, spread over all waves, that finish the top of the
-//     tree: ready treelets are formed into LDS slots, up to eight per wave
-//     round, optimised side by side by lane groups (partitions split across
-//     lanes, reduced in enumeration order) and rebuilt by their owner lanes.
-//     Records are written with agent-coherent stores and read with coherent
-//     loads, and every arrival waits for its own stores first, so no
-//     whole-L2 writeback or invalidate is issued.
+//     later lists 7.  Formation keeps the leaf boxes in registers and the ids
+//     in a lane-private LDS record; the subset areas come from a depth-first
+//     walk over ascending leaf additions (the fold order) and the dynamic
+//     programme is unrolled at compile time over subsets in increasing order
+//     (every proper subset of s is smaller than s), its costs in registers
+//     and its partitions in the LDS record; the wave stops at the largest
+//     treelet among its lanes.  Second arrivals at the parents go to list
+//     L+1, collected per workgroup.  Records are written with plain stores:
+//     the next launch reads them.
+//   * k_tl_tail: the first list shorter than level_min (or list levels+1)
+//     seeds climbing chains, spread over all waves, that finish the top of
+//     the tree as in the reference: ready treelets are formed into LDS slots
+//     (agent-coherent loads), up to eight per wave round, optimised and
+//     written by lane groups (agent-coherent stores), and every arrival waits
+//     for its wave's stores first, so no whole-L2 writeback or invalidate is
+//     issued.
 #include "dev_common.hpp"
 #include "engine.hpp"
 
@@ -65,7 +64,7 @@ static_assert(kWave / kGroups >= kTMax, "a treelet group needs a lane per leaf")
 constexpr float kCostInf = 1e30f;
 constexpr int kLvBlock = 256;            // level kernels: one treelet per thread
 constexpr int kTailItems = 2;            // tail chains per thread per pass
-constexpr uint32_t kTailGrid = 1024;     // tail workgroups (all resident: four per CU)
+constexpr uint32_t kTailGrid = 1024;     // tail workgroups
 
 // ---- shared helpers ------------------------------------------------------------
 
@@ -308,6 +307,22 @@ __device__ __forceinline__ int lane_form(const DNode* __restrict__ nodes, int32_
     return n;
 }
 
+// Box union with hardware min/max, for boxes used only through their area:
+// on finite coordinates (meshes with NaN or infinite vertices are rejected
+// before any build) it differs from merge() at most in the sign of a zero,
+// which changes no area's value, so every cost comparison of the dynamic
+// programme comes out as with merge().
+__device__ __forceinline__ Box merge_area(const Box& a, const Box& b) {
+    Box r;
+    r.mnx = fminf(a.mnx, b.mnx);
+    r.mny = fminf(a.mny, b.mny);
+    r.mnz = fminf(a.mnz, b.mnz);
+    r.mxx = fmaxf(a.mxx, b.mxx);
+    r.mxy = fmaxf(a.mxy, b.mxy);
+    r.mxz = fmaxf(a.mxz, b.mxz);
+    return r;
+}
+
 // Areas of every subset of the first nmax leaves with two or more leaves, at
 // c[s]: depth first, a child subset adds a leaf above the parent's highest,
 // so each box is its parent's box merged with one leaf -- the reference's
@@ -318,7 +333,7 @@ struct AreaWalk {
         if constexpr (J < NM) {
             if (J < nmax) {
                 constexpr int C = S | (1 << J);
-                const Box bc = merge(b, lane_box(t, J));
+                const Box bc = merge_area(b, lane_box(t, J));
                 if constexpr ((C & (C - 1)) != 0) c[C] = area(bc);
                 AreaWalk<NM, C, J + 1>::run(c, t, bc, nmax);
                 AreaWalk<NM, S, J + 1>::run(c, t, b, nmax);
@@ -504,42 +519,7 @@ __device__ __forceinline__ uint32_t list_base(const TreeletArgs& a, int level) {
     return b;
 }
 
-// List L in contiguous per-workgroup ranges (whole iterations of kLvBlock).
-template <int NM>
-__global__ __launch_bounds__(kLvBlock) void k_tl_level(TreeletArgs a, int level) {
-    __shared__ uint32_t s_rec[kLvBlock * lane_words<NM>()];
-    __shared__ BlockList s_list;
-    const uint32_t cnt = a.cnt[level];
-    if (cnt < a.min_count) return;  // this list and everything above go to the tail
-    const uint32_t per = ((cnt + gridDim.x - 1) / gridDim.x + kLvBlock - 1) / kLvBlock * kLvBlock;
-    const uint32_t lo = blockIdx.x * per;
-    if (lo >= cnt) return;
-    const uint32_t hi = min(cnt, lo + per);
-    if (threadIdx.x == 0) s_list.n = 0;
-    __syncthreads();
-    const uint32_t base = list_base(a, level);
-    const uint32_t tag = 2 * a.epoch + 1;
-    uint32_t* out = a.list + base + cnt;
-    LaneLds<NM>& rec = lane_rec<NM>(s_rec);
-    int it = 0;
-    for (uint32_t i0 = lo; i0 < hi; i0 += kLvBlock) {
-        const uint32_t e = i0 + threadIdx.x;
-        const bool have = e < hi;
-        const int32_t u = have ? (int32_t)a.list[base + e] : 0;
-        const int4 lr = have ? a.nodes[u].lr : make_int4(0, 0, 0, 0);
-        lane_treelet<NM>(a, have, u, lr, rec);
-        const bool up = have && u != 0;
-        const uint32_t target = up ? a.parents[u] : 0;
-        arrive_local(a, up, target, tag, s_list);
-        if (++it == kFlushIters) {
-            flush_list(s_list, a.cnt + level + 1, out);
-            it = 0;
-        }
-    }
-    if (it != 0) flush_list(s_list, a.cnt + level + 1, out);
-}
-
-// ---- tail: climbing chains, treelets in LDS slots, lane-group DP ------------------
+// ---- group path: treelets in LDS slots, lane-group DP --------------------------
 
 // One treelet of at most kTMax leaves in LDS: formed by its owner lane,
 // optimised by one lane group, rebuilt by the owner.
@@ -551,6 +531,10 @@ struct Slot {
     int32_t exn[kTMax - 1];             // expanded internal nodes and their children, in order
     int32_t exl[kTMax - 1];
     int32_t exr[kTMax - 1];
+    int32_t qn[kTMax - 1];              // reconstruction plan, in BFS order: node, leaf mask, children
+    int32_t qm[kTMax - 1];
+    int32_t ql[kTMax - 1];
+    int32_t qr[kTMax - 1];
     int32_t n;  // leaves; the treelet has n - 1 expanded (internal) nodes
 };
 struct WaveLds {
@@ -559,7 +543,7 @@ struct WaveLds {
 // An odd slot stride in words puts equal offsets of the eight slots in eight
 // different LDS banks (the groups walk identical subset sequences).
 static_assert(sizeof(Slot) % 8 == 4, "slot stride must be an odd number of words");
-static_assert(sizeof(WaveLds) * kTWaves + 127 <= 32768, "tail LDS (slots + subset table) must stay within 32 KB");
+static_assert(sizeof(WaveLds) * kTWaves + 127 <= 64 * 1024, "tail LDS (slots + subset table) must stay within 64 KB");
 
 __device__ __forceinline__ Box lds_box(const float* f) {
     Box b;
@@ -572,8 +556,8 @@ __device__ __forceinline__ void lds_put(float* f, const Box& b) {
     f[3] = b.mxx; f[4] = b.mxy; f[5] = b.mxz;
 }
 
-// Greedy formation into an LDS slot with agent-coherent loads (records of
-// this launch may come from other XCDs).  The leaves' areas and child ids
+// Greedy formation into an LDS slot with agent-coherent loads: other
+// workgroups, maybe on other XCDs, wrote the records during this launch.  The leaves' areas and child ids
 // are kept in registers as well (the argmax and the next expansion's loads
 // need no LDS round trip); LDS gets what the DP and the rebuild read.
 __device__ int slot_form(const TreeletArgs& a, int32_t root, Slot& sl, Box& rootbox) {
@@ -639,35 +623,6 @@ __device__ int slot_form(const TreeletArgs& a, int32_t root, Slot& sl, Box& root
     return n;
 }
 
-// Union of the subset's leaf boxes in ascending index order from the empty
-// box (the reference's fold).
-__device__ __forceinline__ Box subset_box(const Slot& sl, int s) {
-    Box b = empty_box();
-    for (int t = s; t != 0; t &= t - 1) b = merge(b, lds_box(sl.lb[__ffs(t) - 1]));
-    return b;
-}
-
-// Cost and best partition of subset s (reference lbvh_plus_builder.cu:131-160):
-// partitions in the reference's enumeration order, strict '<'.
-__device__ __forceinline__ void dp_subset(Slot& sl, int s) {
-    const Box b = subset_box(sl, s);
-    const float areaS = area(b);
-    float minCost = kCostInf;
-    int bestP = -1;
-    const int delta = (s - 1) & s;
-    int p = (-delta) & s;
-    do {
-        const float c = sl.cost1[p - 1] + sl.cost1[(s ^ p) - 1];
-        if (c < minCost) {
-            minCost = c;
-            bestP = p;
-        }
-        p = (p - delta) & s;
-    } while (p != 0);
-    sl.cost1[s - 1] = minCost + 1.0f * areaS;
-    sl.part[s] = (uint8_t)bestP;
-}
-
 // Subsets of 7 leaves grouped by size, increasing within a size (so the
 // subsets of n < 7 leaves are a prefix of each group); copied into LDS by
 // every workgroup.
@@ -691,69 +646,94 @@ __device__ __forceinline__ int subk7(int k) {  // sum of C(7, j) for 1 <= j < k
     return k <= 1 ? 0 : k == 2 ? 7 : k == 3 ? 28 : k == 4 ? 63 : k == 5 ? 98 : k == 6 ? 119 : k == 7 ? 126 : 127;
 }
 
+// The t-th (0-based) nonzero submask of `mask` (at most kTMax - 1 bits) in
+// increasing order, branch-free: t + 1 with its bits placed at mask's bits.
+__device__ __forceinline__ int deposit(int t, int mask) {
+    int v = t + 1, r = 0;
+#pragma unroll
+    for (int j = 0; j < kTMax - 1; ++j) {
+        r |= (v & 1) ? (mask & -mask) : 0;
+        v >>= 1;
+        mask &= mask - 1;
+    }
+    return r;
+}
+
 // The dynamic programme of one treelet on a group of G lanes (G a power of
-// two, 8..64; sub = lane in group).  Subsets of one size are independent:
-// each gets L lanes (the largest power of two with L * count <= G), which
-// split its partitions by enumeration position and reduce by (cost,
+// two, 8..64; sub = lane in group).  First every subset's area, lanes
+// striding over the subsets, each box folded from the leaf boxes in
+// ascending order from the empty box; singletons cost 0.  Then by subset
+// size: subsets of one size are independent and each gets L lanes (the
+// largest power of two with L * count <= G), which split its partitions by
+// enumeration position, four loads in flight per lane, and reduce by (cost,
 // position) -- the reference's first-minimum rule; with L = 1 a lane takes
-// whole subsets.
+// whole subsets.  cost1[s - 1] holds area(s) until cost(s) replaces it.
 __device__ void dp_group(Slot& sl, int sub, int G, const uint8_t* s_sub) {
     const int n = sl.n;
-    if (sub < n) sl.cost1[(1 << sub) - 1] = 0.0f;
+    const int full = (1 << n) - 1;
+    {
+        Box lb[kTMax];
+#pragma unroll
+        for (int i = 0; i < kTMax; ++i) lb[i] = lds_box(sl.lb[i]);
+        for (int s = sub + 1; s <= full; s += G) {
+            Box b = empty_box();
+#pragma unroll
+            for (int i = 0; i < kTMax; ++i)
+                if ((s >> i) & 1) b = merge_area(b, lb[i]);
+            sl.cost1[s - 1] = (s & (s - 1)) ? area(b) : 0.0f;
+        }
+    }
     wave_sync();
     for (int k = 2; k <= n; ++k) {
         const int cnt = binom7(n, k);
         const int subk = subk7(k);
+        const int m = (1 << (k - 1)) - 1;  // partitions of a k-leaf subset
         int L = 1;
         while (L * 2 * cnt <= G) L *= 2;
-        if (L == 1) {
-            for (int q = sub; q < cnt; q += G) dp_subset(sl, s_sub[subk + q]);
-        } else {
-            const int q = sub / L, j = sub & (L - 1);
+        const int j = sub & (L - 1);
+        for (int q = sub / L; q < cnt; q += G / L) {  // L > 1: one subset per lane group
+            const int s = s_sub[subk + q];
+            const int delta = (s - 1) & s;
             float best = kCostInf;
-            int bestP = -1, bestT = 1 << 30;
-            int s = 0;
-            float areaS = 0.0f;
-            if (q < cnt) {
-                s = s_sub[subk + q];
-                areaS = area(subset_box(sl, s));
-                const int delta = (s - 1) & s;
-                const int m = (1 << __popc(delta)) - 1;
-                for (int t = j; t < m; t += L) {
-                    const int p = submask(t, delta);
-                    const float c = sl.cost1[p - 1] + sl.cost1[(s ^ p) - 1];
-                    if (c < best) {
-                        best = c;
-                        bestP = p;
-                        bestT = t;
-                    }
+            int bestT = 1 << 30;
+            for (int t0 = j; t0 < m; t0 += 4 * L) {
+                float c[4];
+#pragma unroll
+                for (int u = 0; u < 4; ++u) {
+                    const int t = t0 + u * L;
+                    const int p = t < m ? deposit(t, delta) : delta;
+                    c[u] = sl.cost1[p - 1] + sl.cost1[(s ^ p) - 1];
                 }
+#pragma unroll
+                for (int u = 0; u < 4; ++u)
+                    if (t0 + u * L < m && c[u] < best) {
+                        best = c[u];
+                        bestT = t0 + u * L;
+                    }
             }
             for (int off = L / 2; off >= 1; off >>= 1) {
                 const float ob = __shfl_xor(best, off, L);
-                const int op = __shfl_xor(bestP, off, L);
                 const int ot = __shfl_xor(bestT, off, L);
                 if (ob < best || (ob == best && ot < bestT)) {
                     best = ob;
-                    bestP = op;
                     bestT = ot;
                 }
             }
-            if (q < cnt && j == 0) {
-                sl.cost1[s - 1] = best + 1.0f * areaS;
-                sl.part[s] = (uint8_t)bestP;
+            if (j == 0) {
+                sl.cost1[s - 1] = best + 1.0f * sl.cost1[s - 1];
+                sl.part[s] = bestT < m ? (uint8_t)deposit(bestT, delta) : (uint8_t)0xFF;
             }
         }
         wave_sync();
     }
 }
 
-// Reconstruction from the slot with agent-coherent stores: an ancestor's
-// treelet may be formed by another workgroup on another XCD later in this
-// launch.
-__device__ void slot_rebuild(const TreeletArgs& a, Slot& sl, int32_t root) {
+// Reconstruction from the slot, by the group: the owner derives the plan
+// (DFS ranks, breadth-first assignment from the partition table) into the
+// slot, then lane q of the group folds the box of plan entry q and writes
+// that node.
+__device__ __forceinline__ void slot_plan(Slot& sl, int32_t root) {
     constexpr int kI = kTMax - 1;
-    DNode* __restrict__ nodes = a.nodes;
     const int n = sl.n;
     const int nexp = n - 1;
     RegTab act = 0, en = INT32_MIN, el = 0, er = 0;
@@ -766,22 +746,74 @@ __device__ void slot_rebuild(const TreeletArgs& a, Slot& sl, int32_t root) {
         er[e] = sl.exr[e];
     }
     const RegTab avail = dfs_ranks<kI>(en, el, er, nexp, root);
+    int q = 0;
     bfs_rebuild<kI>(
         act, avail, n, [&](int mask) { return (int)sl.part[mask]; },
         [&](int32_t cur, int mask, int32_t ln, int32_t rn) {
-            Box b = empty_box();
-#pragma unroll
-            for (int i = 0; i < kTMax; ++i)
-                if (i < n && ((mask >> i) & 1)) b = merge(b, lds_box(sl.lb[i]));
-            store_node_coherent<false>(nodes, (uint32_t)cur, b, ln, rn);  // an existing internal node
-            store_u32_coherent(a.parents + ln, (uint32_t)cur);
-            store_u32_coherent(a.parents + rn, (uint32_t)cur);
+            sl.qn[q] = cur;
+            sl.qm[q] = mask;
+            sl.ql[q] = ln;
+            sl.qr[q] = rn;
+            ++q;
         });
 }
+__device__ __forceinline__ void slot_emit(const TreeletArgs& a, const Slot& sl, int q) {
+    const int mask = sl.qm[q];
+    Box b = empty_box();
+#pragma unroll
+    for (int i = 0; i < kTMax; ++i)
+        if ((mask >> i) & 1) b = merge(b, lds_box(sl.lb[i]));
+    const int32_t cur = sl.qn[q], ln = sl.ql[q], rn = sl.qr[q];
+    // Coherent stores: an ancestor's treelet may be formed by another
+    // workgroup, maybe on another XCD, later in this launch.
+    store_node_coherent<false>(a.nodes, (uint32_t)cur, b, ln, rn);  // an existing internal node
+    store_u32_coherent(a.parents + ln, (uint32_t)cur);
+    store_u32_coherent(a.parents + rn, (uint32_t)cur);
+}
 
-// Chains seeded by the first list below min_count (or list levels + 1),
-// spread over all waves of the grid; each chain processes its ready treelet
-// and climbs while it is the second arrival.
+// List L (at least level_min nodes; shorter lists go to the climbing tail):
+// one treelet per lane, in contiguous per-workgroup ranges of whole
+// iterations of kLvBlock.
+template <int NM>
+__global__ __launch_bounds__(kLvBlock) void k_tl_level(TreeletArgs a, int level) {
+    __shared__ uint32_t s_rec[kLvBlock * lane_words<NM>()];
+    __shared__ BlockList s_list;
+    const uint32_t cnt = a.cnt[level];
+    if (cnt < a.level_min) return;  // this list and everything above go to the climbing tail
+    const uint32_t per = ((cnt + gridDim.x - 1) / gridDim.x + kLvBlock - 1) / kLvBlock * kLvBlock;
+    const uint32_t lo = blockIdx.x * per;
+    if (lo >= cnt) return;
+    const uint32_t hi = min(cnt, lo + per);
+    if (threadIdx.x == 0) s_list.n = 0;
+    __syncthreads();
+    const uint32_t base = list_base(a, level);
+    const uint32_t tag = 2 * a.epoch + 1;
+    uint32_t* out = a.list + base + cnt;
+    LaneLds<NM>& rec = lane_rec<NM>(s_rec);
+    int it = 0;
+    for (uint32_t i0 = lo; i0 < hi; i0 += kLvBlock) {
+        const uint32_t e = i0 + threadIdx.x;
+        const bool have = e < hi;
+        const int32_t u = have ? (int32_t)a.list[base + e] : 0;
+        const int4 lr = have ? a.nodes[u].lr : make_int4(0, 0, 0, 0);
+        lane_treelet<NM>(a, have, u, lr, rec);
+        const bool up = have && u != 0;
+        const uint32_t target = up ? a.parents[u] : 0;
+        arrive_local(a, up, target, tag, s_list);
+        if (++it == kFlushIters) {
+            flush_list(s_list, a.cnt + level + 1, out);
+            it = 0;
+        }
+    }
+    if (it != 0) flush_list(s_list, a.cnt + level + 1, out);
+}
+
+// The top of the tree: chains seeded by the first list shorter than
+// level_min (or list levels + 1), spread over all waves of the grid, climb as
+// in the reference: each chain processes its ready treelet and moves up while
+// it is the second arrival.  A wave round takes up to kGroups ready treelets
+// (a lane offers its lowest pending chain), formed by their owners into LDS
+// slots with coherent loads, optimised and written by lane groups.
 __global__ __launch_bounds__(kTBlock) void k_tl_tail(TreeletArgs a) {
     __shared__ WaveLds s_w[kTWaves];
     __shared__ uint8_t s_sub[127];
@@ -789,9 +821,9 @@ __global__ __launch_bounds__(kTBlock) void k_tl_tail(TreeletArgs a) {
     __syncthreads();
     int level = 1;
     uint32_t base = 0;
-    while (level <= a.levels && a.cnt[level] >= a.min_count) base += a.cnt[level++];
+    while (level <= a.levels && a.cnt[level] >= a.level_min) base += a.cnt[level++];
     const uint32_t cnt = a.cnt[level];
-    const int lane = threadIdx.x % kWave;
+    const int lane = (int)lane_id();
     WaveLds& w = s_w[threadIdx.x / kWave];
     const uint32_t waves = gridDim.x * kTWaves;
     const uint32_t gwave = blockIdx.x * kTWaves + threadIdx.x / kWave;
@@ -813,9 +845,6 @@ __global__ __launch_bounds__(kTBlock) void k_tl_tail(TreeletArgs a) {
         }
         uint32_t pend = act;  // list entries are ready treelet roots
         while (__ballot(act != 0) != 0) {
-            // Ready treelets, up to kGroups per round (a lane offers its
-            // lowest pending chain): formed by their owners into LDS slots,
-            // optimised by lane groups, rebuilt by owners.
             while (true) {
                 const bool todo = pend != 0;
                 const uint64_t tmask = __ballot(todo);
@@ -839,13 +868,15 @@ __global__ __launch_bounds__(kTBlock) void k_tl_tail(TreeletArgs a) {
                 if (gi < T && w.slot[gi].n >= 2) dp_group(w.slot[gi], sub, G, s_sub);
                 wave_sync();
                 if (mine) {
-                    if (tn >= 2) slot_rebuild(a, ms, (int32_t)root);
+                    if (tn >= 2) slot_plan(ms, (int32_t)root);
                     else store_box_coherent(a.nodes, root, rootbox);  // nothing to optimise
                     pend &= ~(1u << it0);
                 }
                 wave_sync();
+                if (gi < T && sub < w.slot[gi].n - 1) slot_emit(a, w.slot[gi], sub);
+                wave_sync();
             }
-            // This lane's record stores (coherent) are complete before its
+            // The wave's record stores (coherent) are complete before its
             // arrivals are counted; a second arrival then reads them with
             // coherent loads.  No acquire/release fence, so no whole-L2
             // writeback or invalidate.
@@ -882,7 +913,7 @@ void launch_treelet(const TreeletArgs& a, hipStream_t s) {
         const uint64_t bound = (uint64_t)a.n / (uint64_t)(level + 2) + 1;
         const uint32_t grid = (uint32_t)std::min<uint64_t>((bound + kLvBlock - 1) / kLvBlock, 2048);
         if (level == 1) hipLaunchKernelGGL(k_tl_level<4>, dim3(grid), dim3(kLvBlock), 0, s, a, level);
-        else hipLaunchKernelGGL(k_tl_level<7>, dim3(grid), dim3(kLvBlock), 0, s, a, level);
+        else hipLaunchKernelGGL(k_tl_level<kTMax>, dim3(grid), dim3(kLvBlock), 0, s, a, level);
     }
     hipLaunchKernelGGL(k_tl_tail, dim3(kTailGrid), dim3(kTBlock), 0, s, a);
 }

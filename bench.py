@@ -20,6 +20,9 @@ import numpy as np
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
+# The CPU baseline's OpenMP threads stay on neighbouring cores (read when the
+# oracle's OpenMP runtime starts).
+os.environ.setdefault("OMP_PROC_BIND", "close")
 
 METRIC = "build Mtri/s @4M tris; batched vs plain meshes/s; p50/p99 build+trace latency"
 HBM_PEAK_GBS = 8000.0
@@ -37,6 +40,33 @@ STAGE_BYTES = {"Bounds/Centroids": 68, "Morton Codes": 40, "Sort Pass": 16, "Top
 # profiles/.
 TRAFFIC_FILE_REL = "profiles/traffic_lbvh_4m_blobs.json"
 TRAFFIC_FILE = os.path.join(ROOT, TRAFFIC_FILE_REL)
+
+
+# Algorithmic bytes of the companion workloads: a treelet round moves about
+# (13 node reads + 6 node writes) x 48 B per internal node; a ray 24 (ray) + 8
+# (hit) + 48 per node visited + 36 per triangle tested, counted by an
+# instrumented run; PLOC 124 B per cluster per iteration plus 48 B per new
+# node, with the clusters per iteration from profiles/ploc_iterations_*.json.
+TREELET_BYTES_PER_NODE = 912
+SURVEY_HIER_BYTES = 200  # hierarchy per triangle counted with the reference's separate refit pass
+
+
+def ploc_bytes(name, n, radius):
+    try:
+        with open(os.path.join(ROOT, "profiles", name)) as f:
+            d = json.load(f)
+    except (OSError, ValueError):
+        return None
+    if d.get("triangles") != n or d.get("radius") != radius:
+        return None
+    return int(d["algorithmic_bytes"])
+
+
+def roof(bytes_, ms):
+    if not bytes_ or not ms:
+        return None
+    gbs = bytes_ / (ms * 1e-3) / 1e9
+    return {"bytes": int(bytes_), "ms": round(ms, 4), "GBps": round(gbs, 1), "frac_hbm": round(gbs / HBM_PEAK_GBS, 4)}
 
 
 def pmc_traffic(kernel, n):
@@ -99,6 +129,12 @@ def extras(ctx, dm, tree, mesh, steps):
                      "stages_ms": {k: round(v, 4) for k, v in st.items()}}
         if iters:
             out[name]["iterations"] = iters
+        if name.startswith("lbvh_plus"):
+            tl = sum(v for k, v in st.items() if k.startswith("Treelet Round"))
+            out[name]["treelet_roofline"] = roof(TREELET_BYTES_PER_NODE * (mesh.n - 1) * 3, tl)
+        else:
+            loop = st.get("PLOC Global Iterations", 0.0) + st.get("PLOC LDS Tail", 0.0)
+            out[name]["iterations_roofline"] = roof(ploc_bytes("ploc_iterations_4m_blobs_r25.json", mesh.n, 25), loop)
     rays = B.rays_sphere_camera(mesh.bounds(), 1024, 768)
     for name, kw in (("trace_lbvh", {"algo": B.ALGO_LBVH}), ("trace_ploc_r25", {"algo": B.ALGO_PLOC, "radius": 25})):
         log(f"[bench] {name}")
@@ -108,10 +144,12 @@ def extras(ctx, dm, tree, mesh, steps):
         for _ in range(5):
             ms.append(ctx.trace(tree, dm, rays)[2])
         med = float(np.median(ms))
+        per_ray = 24 + 8 + 48 * float(cnt[:, 0].mean()) + 36 * float(cnt[:, 2].mean())
         out[name] = {"rays": int(rays.shape[0]), "device_ms": round(med, 4),
                      "Mrays_s": round(rays.shape[0] / med / 1e3, 1), "hit_fraction": round(float((prim >= 0).mean()), 4),
                      "avg_nodes_visited": round(float(cnt[:, 0].mean()), 2),
-                     "avg_tri_tests": round(float(cnt[:, 2].mean()), 2)}
+                     "avg_tri_tests": round(float(cnt[:, 2].mean()), 2),
+                     "roofline": roof(per_ray * rays.shape[0], med)}
     return out
 
 
@@ -196,11 +234,20 @@ def large_strands(ctx, steps=2):
              "Mtri_s": round(mesh.n / (wall / steps) / 1e6, 1), "stages_ms": {k: round(v, 3) for k, v in st.items()}}
         if iters:
             r["iterations"] = iters
+        if name.startswith("lbvh_plus"):
+            tl = sum(v for k, v in st.items() if k.startswith("Treelet Round"))
+            r["treelet_roofline"] = roof(TREELET_BYTES_PER_NODE * (mesh.n - 1) * 3, tl)
+        else:
+            loop = st.get("PLOC Global Iterations", 0.0) + st.get("PLOC LDS Tail", 0.0)
+            r["iterations_roofline"] = roof(ploc_bytes("ploc_iterations_32m_strands_r100.json", mesh.n, 100), loop)
         log(f"[bench] large: trace through {name}")
         t, prim, ms = ctx.trace(tree, dm, rays)
+        cnt = ctx.trace(tree, dm, rays, counters=True)[2]
+        per_ray = 24 + 8 + 48 * float(cnt[:, 0].mean()) + 36 * float(cnt[:, 2].mean())
         r["trace_device_ms"] = round(ms, 2)
         r["trace_Mrays_s"] = round(rays.shape[0] / ms / 1e3, 2)
         r["hit_fraction"] = round(float((prim >= 0).mean()), 4)
+        r["trace_roofline"] = roof(per_ray * rays.shape[0], ms)
         out[name] = r
     tree.free()
     dm.free()
@@ -252,23 +299,76 @@ def serving_latency(count=96, seed=23):
     return out
 
 
-def cpu_baseline(mesh, gpu_nodes, gpu_idx):
+def cpu_model():
+    """The host CPU model and logical CPU count (from /proc/cpuinfo)."""
+    model, count = None, 0
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    count += 1
+                    model = model or line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return {"model": model, "logical_cpus": count or os.cpu_count()}
+
+
+def _median_time(fn, reps, warmup=1):
+    for _ in range(warmup):
+        fn()
+    times, out = [], None
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        out = fn()
+        times.append(time.perf_counter() - t0)
+    return float(np.median(times)), out
+
+
+def cpu_baseline(mesh, gpu_nodes, gpu_idx, sample=1_000_000, serial_sample=250_000):
+    """The oracle on this host's cores (OMP_PROC_BIND=close): the headline
+    OpenMP LBVH build of the full mesh, median of 5 after 1 warmup; OpenMP
+    LBVH+ (3 rounds) and PLOC (r=25) on the first `sample` triangles of the
+    same (shuffled) mesh and OpenMP traversal of every 12th of the 1024x768
+    camera rays through the full-mesh LBVH, median of 5; serial figures on
+    smaller samples (every 96th ray), median of 3."""
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     import oracle as O
+    import bvh_amd as B
     threads = O.lib().orc_omp_threads()
-    O.build_lbvh_omp(mesh.soa)  # warmup
-    times = []
-    nodes = idx = None
-    for _ in range(3):
-        t0 = time.perf_counter()
-        nodes, idx, _ = O.build_lbvh_omp(mesh.soa)
-        times.append(time.perf_counter() - t0)
-    med = float(np.median(times))
+    med, (nodes, idx, _) = _median_time(lambda: O.build_lbvh_omp(mesh.soa), 5)
     parity = (O.nodes_equal(gpu_nodes, nodes) is None) and bool(np.array_equal(gpu_idx, idx))
+    sub = np.ascontiguousarray(mesh.soa[:, :sample])
+    small = np.ascontiguousarray(mesh.soa[:, :serial_sample])
+    extra = {}
+    m, _ = _median_time(lambda: O.build_lbvh_plus_omp(sub, 3), 5)
+    extra["lbvh_plus_3_rounds_omp"] = {"triangles": sample, "ms": round(m * 1e3, 1), "Mtri_s": round(sample / m / 1e6, 3)}
+    m, r = _median_time(lambda: O.build_ploc_omp(sub, 25), 5)
+    extra["ploc_r25_omp"] = {"triangles": sample, "ms": round(m * 1e3, 1), "Mtri_s": round(sample / m / 1e6, 3),
+                             "iterations": int(len(r[3]))}
+    rays = B.rays_sphere_camera(mesh.bounds(), 1024, 768)
+    ro = np.ascontiguousarray(rays[::12])  # 65,536 rays spread over the image
+    m, _ = _median_time(lambda: O.trace(nodes, mesh.soa, ro, omp=True), 5)
+    extra["trace_lbvh_omp"] = {"rays": int(ro.shape[0]), "ms": round(m * 1e3, 1),
+                               "Mrays_s": round(ro.shape[0] / m / 1e6, 4)}
+    serial = {}
+    m, _ = _median_time(lambda: O.build_lbvh(sub), 3)
+    serial["lbvh"] = {"triangles": sample, "ms": round(m * 1e3, 1), "Mtri_s": round(sample / m / 1e6, 3)}
+    m, _ = _median_time(lambda: O.build_lbvh_plus(small, 3), 3)
+    serial["lbvh_plus_3_rounds"] = {"triangles": serial_sample, "ms": round(m * 1e3, 1),
+                                    "Mtri_s": round(serial_sample / m / 1e6, 3)}
+    m, _ = _median_time(lambda: O.build_ploc(small, 25), 3)
+    serial["ploc_r25"] = {"triangles": serial_sample, "ms": round(m * 1e3, 1),
+                          "Mtri_s": round(serial_sample / m / 1e6, 3)}
+    rs = np.ascontiguousarray(rays[::96])  # 8,192 rays spread over the image
+    m, _ = _median_time(lambda: O.trace(nodes, mesh.soa, rs), 3)
+    serial["trace_lbvh"] = {"rays": int(rs.shape[0]), "ms": round(m * 1e3, 1), "Mrays_s": round(rs.shape[0] / m / 1e6, 3)}
     return {
         "value": round(mesh.n / med / 1e6, 3), "unit": "Mtri/s", "cores": int(threads), "kind": "port",
-        "sample": f"OpenMP oracle LBVH build of the same {mesh.n:,}-triangle mesh, median of 3 after 1 warmup "
-                  f"({med * 1e3:.1f} ms each)",
+        "sample": f"OpenMP oracle LBVH build of the same {mesh.n:,}-triangle mesh, median of 5 after 1 warmup "
+                  f"({med * 1e3:.1f} ms each), OMP_PROC_BIND={os.environ.get('OMP_PROC_BIND')}",
+        "cpu": cpu_model(),
+        "companions_omp": extra,
+        "serial": serial,
     }, parity
 
 
@@ -392,6 +492,12 @@ def main():
                 "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic, "kernel": dom_name,
                 "avg_launch_ms": round(avg_launch_ms, 4), "bytes_per_launch": dom["unit"] * mesh.n,
                 "traffic_source": TRAFFIC_FILE_REL if traffic else None}
+    if dom_name == "k_hier":
+        # The same launch priced at the reference's hierarchy bytes (separate
+        # Karras, leaf and refit passes: 200 B per triangle).
+        a200 = SURVEY_HIER_BYTES * mesh.n / (avg_launch_ms * 1e-3) / 1e9
+        roofline["achieved_at_200B_per_tri"] = round(a200, 1)
+        roofline["frac_at_200B_per_tri"] = round(a200 / HBM_PEAK_GBS, 4)
 
     cpu = None
     parity = None

@@ -38,6 +38,9 @@
 //                                                          Ct(n-1)+Ci*n; hand-computed 3-leaf tree at
 //                                                          (3,2) and (1,1); zero-area root
 //   depth_stats        src/evaluator.cpp:90-113              complete 2^k-leaf trees and caterpillars
+//   *_omp variants     (CPU baseline only)                   equal to the serial functions above on
+//                                                          blob, uniform and strand meshes
+//                                                          (test_oracle_omp.py)
 // The reference itself cannot be compiled here (its headers need the CUDA
 // toolkit), so no function is checked against reference binaries.
 // Known deviations from the reference, recorded as semantics (not bugs):
@@ -914,6 +917,247 @@ void orc_build_lbvh_omp(const float* const* soa, int64_t n, void* nodes_out, uin
     if (stage_ms) {
         stage_ms[0] = t1 - t0; stage_ms[1] = t2 - t1; stage_ms[2] = t3 - t2; stage_ms[3] = t4 - t3; stage_ms[4] = t5 - t4;
     }
+}
+
+
+// ---------------------------------------------------------------------------
+// OpenMP variants of LBVH+ and PLOC, for the CPU-baseline timing only.  Each
+// returns exactly what its serial counterpart returns (tests check it): the
+// sorts are stable LSD radix sorts of (code, index) by code, treelet rounds
+// follow the reference's climbing schedule on threads (second arrival
+// through an atomic counter, disjoint subtrees commute), and PLOC numbers
+// its merges by a prefix sum in ascending cluster order, as the serial loop.
+
+}  // extern "C"
+
+// Parallel triangle boxes and scene box.
+static Box omp_tri_boxes(const Soa& m, int64_t n, std::vector<Box>& tb) {
+    tb.resize(n);
+    Box scene = empty_box();
+#pragma omp parallel
+    {
+        Box loc = empty_box();
+#pragma omp for schedule(static)
+        for (int64_t i = 0; i < n; ++i) { tb[i] = tri_box(m, i); loc = merge(loc, tb[i]); }
+#pragma omp critical
+        scene = merge(scene, loc);
+    }
+    return scene;
+}
+
+// Stable LSD radix sort of (code, index) pairs by the low `bits` bits of the
+// code, 8-bit digits, one histogram per thread.
+template <class K>
+static void omp_radix_pairs(std::vector<K>& code, std::vector<uint32_t>& idx, int bits) {
+    const int64_t n = (int64_t)code.size();
+    std::vector<K> tc(n);
+    std::vector<uint32_t> ti(n);
+    const int nt = orc_omp_threads();
+    std::vector<int64_t> hist((size_t)nt * 256);
+    for (int shift = 0; shift < bits; shift += 8) {
+        std::fill(hist.begin(), hist.end(), 0);
+#pragma omp parallel num_threads(nt)
+        {
+            int tid = 0;
+#ifdef _OPENMP
+            tid = omp_get_thread_num();
+#endif
+            const int64_t lo = n * tid / nt, hi = n * (tid + 1) / nt;
+            int64_t* h = &hist[(size_t)tid * 256];
+            for (int64_t i = lo; i < hi; ++i) h[(code[i] >> shift) & 255]++;
+#pragma omp barrier
+#pragma omp single
+            {
+                int64_t run = 0;
+                for (int b = 0; b < 256; ++b)
+                    for (int t = 0; t < nt; ++t) {
+                        const int64_t c = hist[(size_t)t * 256 + b];
+                        hist[(size_t)t * 256 + b] = run;
+                        run += c;
+                    }
+            }
+            for (int64_t i = lo; i < hi; ++i) {
+                const int64_t d = h[(code[i] >> shift) & 255]++;
+                tc[d] = code[i];
+                ti[d] = idx[i];
+            }
+        }
+        std::swap(code, tc);
+        std::swap(idx, ti);
+    }
+}
+
+// Exclusive prefix sum of 0/1 flags; returns the total.
+static int64_t omp_scan_flags(const std::vector<uint8_t>& f, int64_t n, std::vector<int64_t>& pos) {
+    const int nt = orc_omp_threads();
+    std::vector<int64_t> part((size_t)nt + 1, 0);
+#pragma omp parallel num_threads(nt)
+    {
+        int tid = 0;
+#ifdef _OPENMP
+        tid = omp_get_thread_num();
+#endif
+        const int64_t lo = n * tid / nt, hi = n * (tid + 1) / nt;
+        int64_t c = 0;
+        for (int64_t i = lo; i < hi; ++i) c += f[i];
+        part[(size_t)tid + 1] = c;
+#pragma omp barrier
+#pragma omp single
+        for (int t = 0; t < nt; ++t) part[(size_t)t + 1] += part[(size_t)t];
+        int64_t run = part[(size_t)tid];
+        for (int64_t i = lo; i < hi; ++i) { pos[i] = run; run += f[i]; }
+    }
+    return part[(size_t)nt];
+}
+
+// Treelet rounds on threads: each leaf's thread climbs; the second arrival
+// at a node (acquire-release counter) refits it and optimises its treelet.
+static void refit_omp(std::vector<LNode>& nodes, int64_t n, bool optimize) {
+    std::vector<std::atomic<int>> counter(n > 1 ? n - 1 : 1);
+    for (auto& c : counter) c.store(0, std::memory_order_relaxed);
+#pragma omp parallel for schedule(dynamic, 1024)
+    for (int64_t i = 0; i < n; ++i) {
+        uint32_t idx = (uint32_t)((n - 1) + i);
+        while (idx != 0) {
+            const uint32_t parent = nodes[idx].parent;
+            if (counter[parent].fetch_add(1, std::memory_order_acq_rel) == 0) break;
+            nodes[parent].box = merge(nodes[nodes[parent].left].box, nodes[nodes[parent].right].box);
+            if (optimize) optimize_treelet(nodes, (int)parent);
+            idx = parent;
+        }
+    }
+}
+
+extern "C" {
+
+void orc_build_lbvh_plus_omp(const float* const* soa, int64_t n, int rounds, void* nodes_out, uint32_t* indices) {
+    Soa m;
+    for (int k = 0; k < 9; ++k) m.v[k] = soa[k];
+    std::vector<Box> tb;
+    const Box scene = omp_tri_boxes(m, n, tb);
+    std::vector<uint64_t> sc(n);
+    std::vector<uint32_t> si(n);
+#pragma omp parallel for schedule(static)
+    for (int64_t i = 0; i < n; ++i) {
+        float c[3];
+        norm_centroid(tb[i], scene, c);
+        sc[i] = morton60(c[0], c[1], c[2]);
+        si[i] = (uint32_t)i;
+    }
+    omp_radix_pairs<uint64_t>(sc, si, 60);
+    std::vector<LNode> nodes(2 * n - 1);
+    Delta60 d{sc.data(), si.data(), n};
+#pragma omp parallel for schedule(static)
+    for (int64_t i = 0; i < n - 1; ++i) karras_node(d, n, i, nodes);
+#pragma omp parallel for schedule(static)
+    for (int64_t i = 0; i < n; ++i) {
+        LNode& L = nodes[(n - 1) + i];
+        L.box = tb[si[i]];
+        L.left = si[i] | 0x80000000u;
+        L.right = 0xFFFFFFFFu;
+    }
+    refit_omp(nodes, n, false);
+    for (int r = 0; r < rounds; ++r) refit_omp(nodes, n, true);
+    lnodes_to_out(nodes, (ONode*)nodes_out);
+    std::memcpy(indices, si.data(), n * sizeof(uint32_t));
+}
+
+int orc_build_ploc_omp(const float* const* soa, int64_t n, int radius, void* nodes_out, uint32_t* indices,
+                       int32_t* iter_m, int32_t max_iters, int32_t* n_iters) {
+    Soa m;
+    for (int k = 0; k < 9; ++k) m.v[k] = soa[k];
+    ONode* out = (ONode*)nodes_out;
+    std::vector<Box> tb;
+    const Box scene = omp_tri_boxes(m, n, tb);
+    std::vector<uint32_t> sc(n), si(n);
+#pragma omp parallel for schedule(static)
+    for (int64_t i = 0; i < n; ++i) {
+        float c[3];
+        norm_centroid(tb[i], scene, c);
+        sc[i] = morton30(c[0], c[1], c[2]);
+        si[i] = (uint32_t)i;
+    }
+    omp_radix_pairs<uint32_t>(sc, si, 32);
+    std::memcpy(indices, si.data(), n * sizeof(uint32_t));
+    std::vector<Box> cb(n), cb2(n);
+    std::vector<int32_t> cid(n), cid2(n), nn(n);
+    std::vector<uint8_t> lf(n), vf(n);
+    std::vector<int64_t> lpos(n), vpos(n);
+#pragma omp parallel for schedule(static)
+    for (int64_t i = 0; i < n; ++i) {
+        cb[i] = tb[si[i]];
+        set_node(out[i], cb[i], (int32_t)(si[i] | 0x80000000u), -1);
+        cid[i] = (int32_t)i;
+    }
+    int64_t cur = n, next = n;
+    int32_t iters = 0;
+    while (cur > 1) {
+        if (iter_m && iters < max_iters) iter_m[iters] = (int32_t)cur;
+        iters++;
+#pragma omp parallel for schedule(dynamic, 4096)
+        for (int64_t i = 0; i < cur; ++i) {
+            float best = FLT_MAX;
+            int64_t bj = -1;
+            const int64_t lo = std::max<int64_t>(0, i - radius), hi = std::min<int64_t>(cur - 1, i + radius);
+            for (int64_t j = lo; j <= hi; ++j) {
+                if (j == i) continue;
+                const float dist = area(merge(cb[i], cb[j]));
+                if (dist < best) { best = dist; bj = j; }
+                else if (dist == best) { if (bj == -1 || j < bj) bj = j; }
+            }
+            nn[i] = (int32_t)bj;
+        }
+#pragma omp parallel for schedule(static)
+        for (int64_t i = 0; i < cur; ++i) {
+            const int32_t j = nn[i];
+            const bool mutual = (j != -1) && nn[j] == (int32_t)i;
+            lf[i] = (mutual && i < j) ? 1 : 0;
+            vf[i] = (mutual && j < i) ? 0 : 1;
+        }
+        const int64_t merges = omp_scan_flags(lf, cur, lpos);
+        if (merges == 0) {
+            if (n_iters) *n_iters = iters;
+            return 5;
+        }
+#pragma omp parallel for schedule(static)
+        for (int64_t i = 0; i < cur; ++i) {
+            if (!lf[i]) continue;
+            const int32_t j = nn[i];
+            const Box u = merge(cb[i], cb[j]);
+            const int64_t id = next + lpos[i];
+            set_node(out[id], u, cid[i], cid[j]);
+            cid[i] = (int32_t)id;
+            cb[i] = u;
+        }
+        next += merges;
+        const int64_t w = omp_scan_flags(vf, cur, vpos);
+#pragma omp parallel for schedule(static)
+        for (int64_t i = 0; i < cur; ++i)
+            if (vf[i]) { cb2[vpos[i]] = cb[i]; cid2[vpos[i]] = cid[i]; }
+        std::swap(cb, cb2);
+        std::swap(cid, cid2);
+        cur = w;
+    }
+    if (n_iters) *n_iters = iters;
+    const int64_t total = next;
+    if (total > 1) {
+        const int64_t root = total - 1;
+        std::swap(out[0], out[root]);
+#pragma omp parallel for schedule(static)
+        for (int64_t k = 0; k < total; ++k) {
+            ONode& nd = out[k];
+            if (nd.left < 0) continue;
+            auto fix = [&](int32_t& c) {
+                if (!(c & 0x80000000)) {
+                    if (c == 0) c = (int32_t)root;
+                    else if (c == root) c = 0;
+                }
+            };
+            fix(nd.left);
+            fix(nd.right);
+        }
+    }
+    return 0;
 }
 
 }  // extern "C"

@@ -51,6 +51,8 @@ def lib():
             "orc_depth_stats": (None, [_vp, _i64, _vp, _vp, _vp]),
             "orc_omp_threads": (C.c_int, []),
             "orc_build_lbvh_omp": (None, [_vp, _i64, _vp, _vp, _vp]),
+            "orc_build_lbvh_plus_omp": (None, [_vp, _i64, C.c_int, _vp, _vp]),
+            "orc_build_ploc_omp": (C.c_int, [_vp, _i64, C.c_int, _vp, _vp, _vp, C.c_int32, _vp]),
         }
         for name, (res, args) in sigs.items():
             f = getattr(L, name)
@@ -133,6 +135,30 @@ def build_lbvh_plus(m, rounds=1):
     codes = np.zeros(n, dtype=np.uint64)
     lib().orc_build_lbvh_plus(_soa(soa), n, rounds, nodes.ctypes.data, idx.ctypes.data, codes.ctypes.data)
     return nodes, idx, codes
+
+
+def build_lbvh_plus_omp(m, rounds=1):
+    """OpenMP LBVH+ (treelet rounds on threads); equal to build_lbvh_plus."""
+    soa = _as_soa(m)
+    n = soa.shape[1]
+    nodes = np.zeros(2 * n - 1, dtype=NODE_DTYPE)
+    idx = np.zeros(n, dtype=np.uint32)
+    lib().orc_build_lbvh_plus_omp(_soa(soa), n, rounds, nodes.ctypes.data, idx.ctypes.data)
+    return nodes, idx
+
+
+def build_ploc_omp(m, radius=25, max_iters=4096):
+    """OpenMP PLOC; equal to build_ploc (status, nodes, indices, clusters
+    entering each iteration)."""
+    soa = _as_soa(m)
+    n = soa.shape[1]
+    nodes = np.zeros(2 * n - 1, dtype=NODE_DTYPE)
+    idx = np.zeros(n, dtype=np.uint32)
+    iters = np.zeros(max_iters, dtype=np.int32)
+    nit = C.c_int32(0)
+    st = lib().orc_build_ploc_omp(_soa(soa), n, radius, nodes.ctypes.data, idx.ctypes.data, iters.ctypes.data,
+                                  max_iters, C.byref(nit))
+    return st, nodes, idx, iters[: min(nit.value, max_iters)]
 
 
 def optimize_treelet(nodes, parents, root):

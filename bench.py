@@ -254,17 +254,20 @@ def large_strands(ctx, steps=2):
     return out
 
 
-def serving_latency(count=96, seed=23):
+def serving_latency(count=1000, seed=23, pool=96):
     """Build+trace requests (blob meshes log-uniform 10K-2M triangles, rays
     log-uniform 65,536-1,048,576 from the bounding sphere toward the box, 20%
-    follow-up traces on resident meshes), LBVH builds.  Capacity = closed-loop
-    saturation (all requests at t=0); then Poisson arrivals at 50% and 90% of
-    the coalescing server's capacity, served by the coalescing server (4
-    slots, groups of up to 32) and by the plain path (1 slot, one mesh per
-    launch), p50/p99 of done - arrival on the server clock."""
+    follow-up traces on resident meshes), LBVH builds; `pool` distinct
+    (mesh, rays) pairs drawn at random by the `count` requests, each build
+    request under its own key.  Capacity = closed-loop saturation (all
+    requests at t=0) of each server; then Poisson arrivals at 50% and 90% of
+    each server's own capacity, and both servers at 90% of the plain path's
+    capacity (the same offered load).  Servers: coalescing (4 slots, groups
+    of up to 32) and plain (1 slot, one mesh per launch); p50/p99 of done -
+    arrival on the server clock."""
     from bvh_amd import serving as S
-    log(f"[bench] serving: generating {count} requests (seed {seed})")
-    reqs = S.make_requests(seed, count)
+    log(f"[bench] serving: generating {count} requests ({pool} distinct meshes, seed {seed})")
+    reqs = S.make_requests(seed, count, pool=pool)
     warm = S.make_requests(seed + 1, 4, tri_lo=10_000, tri_hi=50_000, ray_lo=65_536, ray_hi=100_000, follow_up=0.0)
     for q in warm:
         q["key"] = 0
@@ -282,20 +285,24 @@ def serving_latency(count=96, seed=23):
             st["peak_device_GB"] = round(ss["peak"] / 1e9, 2)
             return st
 
-    out = {"requests": count, "seed": seed, "algo": "LBVH",
+    out = {"requests": count, "distinct_meshes": pool, "seed": seed, "algo": "LBVH",
            "triangles_total": int(sum(q["ntri"] for q in reqs)),
            "rays_total": int(sum(q["rays"].shape[0] for q in reqs))}
     for name in servers:
         log(f"[bench] serving: capacity ({name})")
         out[f"capacity_{name}"] = one(name, np.zeros(count))
-    cap = out["capacity_coalesced"]["throughput_rps"]
-    for load in (0.5, 0.9):
-        arr = S.poisson_arrivals(load * cap, count, seed)
-        for name in servers:
-            log(f"[bench] serving: {int(load * 100)}% load ({name})")
-            r = one(name, arr)
+    for name in servers:
+        cap = out[f"capacity_{name}"]["throughput_rps"]
+        for load in (0.5, 0.9):
+            log(f"[bench] serving: {int(load * 100)}% of its capacity ({name})")
+            r = one(name, S.poisson_arrivals(load * cap, count, seed))
             r["offered_rps"] = round(load * cap, 2)
             out[f"load{int(load * 100)}_{name}"] = r
+    same = 0.9 * out["capacity_plain"]["throughput_rps"]
+    log("[bench] serving: coalescing at the plain path's 90% load")
+    r = one("coalesced", S.poisson_arrivals(same, count, seed))
+    r["offered_rps"] = round(same, 2)
+    out["plain90_load_coalesced"] = r
     return out
 
 

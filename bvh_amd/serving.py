@@ -203,28 +203,48 @@ def poisson_arrivals(rate_per_s: float, count: int, seed: int) -> np.ndarray:
 
 
 def make_requests(seed, count, tri_lo=10_000, tri_hi=2_000_000, ray_lo=65_536, ray_hi=1_048_576,
-                  follow_up=0.2):
+                  follow_up=0.2, pool=None):
     """Build+trace requests (blob meshes, rays from the bounding sphere toward
-    the box) and a fraction of follow-up trace requests on an earlier mesh."""
+    the box) and a fraction of follow-up trace requests on an earlier mesh.
+
+    pool: if set, only `pool` distinct (mesh, rays) pairs are generated and
+    each request takes one of them at random; every build request still
+    builds its own copy under its own key (host memory and generation time
+    stay bounded for thousands of requests)."""
     import bvh_amd as B
     rng = np.random.default_rng(seed)
-    tris = log_uniform_ints(rng, count, tri_lo, tri_hi)
-    nrays = log_uniform_ints(rng, count, ray_lo, ray_hi)
+    distinct = count if pool is None else min(pool, count)
+    tris = log_uniform_ints(rng, distinct, tri_lo, tri_hi)
+    nrays = log_uniform_ints(rng, distinct, ray_lo, ray_hi)
     follow = rng.random(count) < follow_up
+    pick = np.arange(count) if pool is None else rng.integers(0, distinct, count)
+    meshes, boxes_p, rays_p = {}, {}, {}
+
+    def entry(p):
+        if p not in meshes:
+            m = B.Mesh.blobs(seed * 7919 + int(p), int(tris[p]))
+            meshes[p] = m
+            boxes_p[p] = m.bounds()
+            rays_p[p] = B.rays_sphere_to_box(seed * 1000003 + int(p), boxes_p[p], int(nrays[p]))
+        return meshes[p], boxes_p[p], rays_p[p]
+
     reqs = []
-    boxes = {}
+    built = {}  # key -> pool entry of its mesh
     for i in range(count):
-        if follow[i] and boxes:
-            keys = sorted(boxes)
+        p = int(pick[i])
+        if follow[i] and built:
+            keys = sorted(built)
             key = keys[int(rng.integers(len(keys)))]
-            reqs.append({"id": i, "key": key, "mesh": None, "ntri": 0,
-                         "rays": B.rays_sphere_to_box(seed * 1000003 + i, boxes[key], int(nrays[i]))})
+            if pool is None:
+                rays = B.rays_sphere_to_box(seed * 1000003 + i, boxes_p[built[key]], int(nrays[p]))
+            else:
+                rays = rays_p[built[key]]
+            reqs.append({"id": i, "key": key, "mesh": None, "ntri": 0, "rays": rays})
             continue
-        m = B.Mesh.blobs(seed * 7919 + i, int(tris[i]))
+        m, _, rays = entry(p)
         key = i + 1
-        boxes[key] = m.bounds()
-        reqs.append({"id": i, "key": key, "mesh": m, "ntri": m.n,
-                     "rays": B.rays_sphere_to_box(seed * 1000003 + i, boxes[key], int(nrays[i]))})
+        built[key] = p
+        reqs.append({"id": i, "key": key, "mesh": m, "ntri": m.n, "rays": rays})
     return reqs
 
 

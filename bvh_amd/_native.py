@@ -64,14 +64,19 @@ class BatchInfo(C.Structure):
                 ("algo", C.c_int32), ("radix_bits", C.c_int32), ("sort_passes", C.c_int32), ("key_bits", C.c_int32),
                 ("ploc_captured_iterations", C.c_int32), ("ploc_iterations", C.c_int32),
                 ("ploc_nn_grid", C.c_int32), ("ploc_merge_grid", C.c_int32), ("graph_nodes", C.c_int32),
-                ("runs", C.c_int32), ("eager_continuations", C.c_int32), ("tune_count", C.c_int32),
-                ("tune_ms", C.c_float * 8), ("tune_radix_bits", C.c_int32 * 8), ("tune_nn_grid", C.c_int32 * 8)]
+                ("runs", C.c_int32), ("eager_continuations", C.c_int32), ("sort_block", C.c_int32),
+                ("front_chunk", C.c_int32), ("tune_count", C.c_int32),
+                ("tune_ms", C.c_float * 16), ("tune_radix_bits", C.c_int32 * 16), ("tune_nn_grid", C.c_int32 * 16),
+                ("tune_sort_block", C.c_int32 * 16), ("tune_front_chunk", C.c_int32 * 16),
+                ("tune_merge_grid", C.c_int32 * 16)]
+    _TUNE = ("radix_bits", "nn_grid", "sort_block", "front_chunk", "merge_grid")
 
     def as_dict(self):
-        d = {name: getattr(self, name) for name, _ in self._fields_
-             if name not in ("tune_ms", "tune_radix_bits", "tune_nn_grid")}
-        d["tune"] = [{"ms": float(self.tune_ms[i]), "radix_bits": int(self.tune_radix_bits[i]),
-                      "nn_grid": int(self.tune_nn_grid[i])} for i in range(min(self.tune_count, 8))]
+        d = {name: getattr(self, name) for name, _ in self._fields_ if not name.startswith("tune_")}
+        d["tune_count"] = int(self.tune_count)
+        d["tune"] = [dict({"ms": float(self.tune_ms[i])},
+                          **{k: int(getattr(self, "tune_" + k)[i]) for k in self._TUNE})
+                     for i in range(min(self.tune_count, 16))]
         return d
 
 

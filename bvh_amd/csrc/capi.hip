@@ -128,6 +128,8 @@ struct bvh_batch_rec {
     int radix_bits = 10;
     int ploc_iters = 0;   // global PLOC iterations captured in the graph
     uint32_t nn_grid = 0, merge_grid = 0;
+    int sort_block = 0;   // onesweep tile threads (kSortThreads or kSortThreadsSmall)
+    uint32_t front_chunk = 0;  // triangles per front-end block (kFrontChunk or kFrontChunkLarge)
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
     bvh_batch_info info;
@@ -678,8 +680,10 @@ int seg_tables(bvh_ctx_t ctx, SegTables& t, const bvh_mesh_t* meshes, const bvh_
     return BVH_OK;
 }
 
-// Job of k meshes: block tables, segment table and key format.
-int plan_job(bvh_ctx_t ctx, SegTables& t, const bvh_mesh_t* meshes, const bvh_tree_t* trees, uint32_t k, Job& j) {
+// Job of k meshes: block tables (front-end blocks of front_chunk triangles,
+// at least kFrontChunk), segment table and key format.
+int plan_job(bvh_ctx_t ctx, SegTables& t, const bvh_mesh_t* meshes, const bvh_tree_t* trees, uint32_t k, Job& j,
+             uint32_t front_chunk = kFrontChunk) {
     uint64_t total = 0;
     for (uint32_t s = 0; s < k; ++s) {
         if (!meshes[s] || !trees[s]) return set_err(ctx, BVH_E_ARG, "mesh or tree %u is null", s);
@@ -693,7 +697,7 @@ int plan_job(bvh_ctx_t ctx, SegTables& t, const bvh_mesh_t* meshes, const bvh_tr
     j.maxn = 0;
     for (uint32_t s = 0; s < k; ++s) {
         const uint32_t n = meshes[s]->n;
-        for (uint32_t c = 0; c < cdiv(n, kFrontChunk); ++c) fb.push_back(make_uint2(s, c));
+        for (uint32_t c = 0; c < cdiv(n, front_chunk); ++c) fb.push_back(make_uint2(s, c));
         for (uint32_t c = 0; c < cdiv(n, hc); ++c) hb.push_back(make_uint2(s, c));
         j.total += n;
         j.maxn = std::max(j.maxn, n);
@@ -715,7 +719,7 @@ int plan_job(bvh_ctx_t ctx, SegTables& t, const bvh_mesh_t* meshes, const bvh_tr
     j.fsb.blocks = P<uint2>(t.fblocks);
     std::memset(&j.fsb.single, 0, sizeof(j.fsb.single));
     j.fsb.nblocks = (uint32_t)fb.size();
-    j.fsb.chunk = kFrontChunk;
+    j.fsb.chunk = front_chunk;
     j.hsb = j.fsb;
     j.hsb.blocks = P<uint2>(t.hblocks);
     j.hsb.nblocks = (uint32_t)hb.size();
@@ -778,7 +782,9 @@ int batch_tables(bvh_batch_t b, bool force) {
     return seg_tables(b->ctx, b->tabs, b->meshes.data(), b->trees.data(), b->k, force);
 }
 
-int batch_plan(bvh_batch_t b) { return plan_job(b->ctx, b->tabs, b->meshes.data(), b->trees.data(), b->k, b->job); }
+int batch_plan(bvh_batch_t b) {
+    return plan_job(b->ctx, b->tabs, b->meshes.data(), b->trees.data(), b->k, b->job, b->front_chunk);
+}
 
 void batch_destroy_graph(bvh_batch_t b) {
     if (b->exec) (void)hipGraphExecDestroy(b->exec);
@@ -815,7 +821,7 @@ int batch_capture(bvh_batch_t b) {
     // Captured graphs keep the 8192-key sort tile (the wide tile has not run
     // inside a captured graph).
     SortPlan plan = make_sort_plan(b->job.total, b->job.begin_bit, b->job.end_bit, b->radix_bits,
-                                   (int)b->job.fsb.nblocks, false, false);
+                                   (int)b->job.fsb.nblocks, false, false, b->sort_block);
     if (plan.passes > 8) return set_err(ctx, BVH_E_UNSUPPORTED, "batch keys need more than 8 sort passes");
     TRY(reserve_front(ctx, b->ws, b->job, plan, b->algo == BVH_ALGO_LBVH));
     if (b->algo == BVH_ALGO_PLOC) TRY(reserve_ploc(ctx, b->ws, b->job));
@@ -839,6 +845,8 @@ int batch_capture(bvh_batch_t b) {
     b->info.ploc_captured_iterations = b->algo == BVH_ALGO_PLOC ? b->ploc_iters : 0;
     b->info.ploc_nn_grid = (int32_t)b->nn_grid;
     b->info.ploc_merge_grid = (int32_t)b->merge_grid;
+    b->info.sort_block = plan.block;
+    b->info.front_chunk = (int32_t)b->job.fsb.chunk;
     return BVH_OK;
 }
 
@@ -907,29 +915,37 @@ int batch_digest(bvh_batch_t b, uint64_t* h) {
     return BVH_OK;
 }
 
-// Times candidate configurations (digit width x persistent grid of the
-// neighbour search) and keeps the fastest; every candidate must produce the
-// same trees bit for bit.
+// Times candidate configurations and keeps the fastest, one knob at a time
+// from the defaults (coordinate descent): digit width, onesweep tile, front-end
+// chunk, and for PLOC the persistent grids of the neighbour search and of
+// merge + compaction.  Every candidate must produce the same trees bit for
+// bit (FNV-1a over all trees).
+constexpr int kTuneMax = 16;  // candidates recorded in bvh_batch_info
+
+struct TuneCfg {
+    int bits;
+    int sort_block;
+    uint32_t front_chunk, nn_grid, merge_grid;
+};
+
+int batch_apply(bvh_batch_t b, const TuneCfg& c) {
+    const bool replan = c.front_chunk != b->front_chunk;
+    b->radix_bits = c.bits;
+    b->sort_block = c.sort_block;
+    b->front_chunk = c.front_chunk;
+    b->nn_grid = c.nn_grid;
+    b->merge_grid = c.merge_grid;
+    if (replan) TRY(batch_plan(b));
+    return batch_capture(b);
+}
+
 int batch_tune(bvh_batch_t b) {
-    struct Cand {
-        int bits;
-        uint32_t nn_grid;
-    };
-    std::vector<Cand> cands;
-    for (int bits : {8, 10}) {
-        if (b->algo == BVH_ALGO_PLOC)
-            for (uint32_t g : {1024u, 2048u}) cands.push_back({bits, g});
-        else
-            cands.push_back({bits, b->nn_grid});
-    }
-    float best = 1e30f;
-    Cand pick = cands[0];
+    TuneCfg best{b->radix_bits, b->sort_block, b->front_chunk, b->nn_grid, b->merge_grid};
+    float best_ms = 1e30f;
     uint64_t ref = 0;
     int count = 0;
-    for (const Cand& c : cands) {
-        b->radix_bits = c.bits;
-        b->nn_grid = c.nn_grid;
-        TRY(batch_capture(b));
+    auto trial = [&](const TuneCfg& c) -> int {
+        TRY(batch_apply(b, c));
         float ms[3];
         for (float& m : ms) TRY(batch_launch(b, &m));
         std::sort(ms, ms + 3);
@@ -937,23 +953,40 @@ int batch_tune(bvh_batch_t b) {
         TRY(batch_digest(b, &h));
         if (count == 0) ref = h;
         else if (h != ref)
-            return set_err(b->ctx, BVH_E_STATE, "internal error: tuned configurations disagree (digit width %d)",
-                           c.bits);
-        if (count < 8) {
+            return set_err(b->ctx, BVH_E_STATE,
+                           "internal error: tuned configurations disagree (digits %d, tile %d, chunk %u, grids %u/%u)",
+                           c.bits, c.sort_block, c.front_chunk, c.nn_grid, c.merge_grid);
+        if (count < kTuneMax) {
             b->info.tune_ms[count] = ms[1];
             b->info.tune_radix_bits[count] = c.bits;
             b->info.tune_nn_grid[count] = (int32_t)c.nn_grid;
+            b->info.tune_sort_block[count] = c.sort_block;
+            b->info.tune_front_chunk[count] = (int32_t)c.front_chunk;
+            b->info.tune_merge_grid[count] = (int32_t)c.merge_grid;
         }
         ++count;
-        if (ms[1] < best) {
-            best = ms[1];
-            pick = c;
+        if (ms[1] < best_ms) {
+            best_ms = ms[1];
+            best = c;
         }
+        return BVH_OK;
+    };
+    TRY(trial(best));
+    const bool ploc = b->algo == BVH_ALGO_PLOC;
+    for (int knob = 0; knob < (ploc ? 5 : 3); ++knob) {
+        const TuneCfg base = best;
+        TuneCfg alt = base;
+        switch (knob) {
+            case 0: alt.bits = base.bits == 8 ? 10 : 8; break;
+            case 1: alt.sort_block = base.sort_block == kSortThreads ? kSortThreadsSmall : kSortThreads; break;
+            case 2: alt.front_chunk = base.front_chunk == kFrontChunk ? kFrontChunkLarge : kFrontChunk; break;
+            case 3: alt.nn_grid = base.nn_grid == 2048 ? 1024 : 2048; break;
+            default: alt.merge_grid = base.merge_grid == 1024 ? 512 : 1024; break;
+        }
+        TRY(trial(alt));
     }
     b->info.tune_count = count;
-    b->radix_bits = pick.bits;
-    b->nn_grid = pick.nn_grid;
-    return batch_capture(b);
+    return batch_apply(b, best);
 }
 
 // Builds the tree's traversal records once per build (complete on return, so
@@ -1510,6 +1543,8 @@ int bvh_batch_create(bvh_ctx_t ctx, const bvh_mesh_t* meshes, const bvh_tree_t* 
     b->radix_bits = pick_bits(params->radix_bits);
     b->nn_grid = 2048;
     b->merge_grid = 1024;
+    b->sort_block = kSortThreads;
+    b->front_chunk = kFrontChunk;
     auto fail = [&](int st) {
         bvh_batch_free(b);
         return st;

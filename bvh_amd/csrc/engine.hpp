@@ -71,6 +71,7 @@ void launch_tri_bounds(const SegBlocks& sb, float4* tri_boxes, uint32_t* seg_sce
 // on 256 CUs).
 constexpr int kSortThreads = 512;
 constexpr int kSortThreadsWide = 1024;
+constexpr int kSortThreadsSmall = 256;  // 4096-key tiles: a batch tuning option (no payload)
 constexpr int kSortItems = 16;
 constexpr uint32_t kSortWideMinKeys = 2500000;
 
@@ -86,9 +87,11 @@ struct SortPlan {
     int hist_blocks;   // blocks of the histogram producer
 };
 // hist_blocks <= 0 picks a default producer size; vals: the sort carries a
-// 32-bit payload (narrow tiles); wide_ok = false keeps the narrow tile.
+// 32-bit payload (narrow tiles); wide_ok = false keeps the narrow tile;
+// block > 0 asks for that tile (kSortThreadsSmall or kSortThreads; payload
+// sorts always use kSortThreads).
 SortPlan make_sort_plan(uint32_t n, int begin_bit, int end_bit, int bits, int hist_blocks = 0, bool vals = false,
-                        bool wide_ok = true);
+                        bool wide_ok = true, int block = 0);
 // Bytes of the zero-initialised lookback area a plan needs (status words + tile counters).
 size_t sort_zero_bytes(const SortPlan& p);
 

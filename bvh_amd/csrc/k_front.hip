@@ -144,7 +144,8 @@ void launch_tri_bounds(const SegBlocks& sb, float4* tri_boxes, uint32_t* seg_sce
     hipLaunchKernelGGL(k_tri_bounds, dim3(sb.nblocks), dim3(kFrontBlock), 0, s, sb, tri_boxes, seg_scene);
 }
 
-SortPlan make_sort_plan(uint32_t n, int begin_bit, int end_bit, int bits, int hist_blocks, bool vals, bool wide_ok) {
+SortPlan make_sort_plan(uint32_t n, int begin_bit, int end_bit, int bits, int hist_blocks, bool vals, bool wide_ok,
+                        int block) {
     SortPlan p;
     p.bits = bits;
     int span = end_bit - begin_bit;
@@ -156,6 +157,8 @@ SortPlan make_sort_plan(uint32_t n, int begin_bit, int end_bit, int bits, int hi
         p.mask[i] = (1u << w) - 1u;
     }
     p.block = (wide_ok && bits == 8 && !vals && n >= kSortWideMinKeys) ? kSortThreadsWide : kSortThreads;
+    if (block == kSortThreadsSmall && !vals) p.block = kSortThreadsSmall;
+    else if (block == kSortThreads) p.block = kSortThreads;
     p.tile = (uint32_t)p.block * kSortItems;
     p.tiles = (n + p.tile - 1) / p.tile;
     if (p.tiles < 1) p.tiles = 1;

@@ -10,7 +10,8 @@
 //     LDS histogram into one zeroed table with global atomics, and each
 //     onesweep tile turns its pass's row into bin starts with one block scan
 //     (no separate reduce or scan launch);
-//   * each onesweep tile (512 threads x 16 keys) takes a tile id from an
+//   * each onesweep tile (512 threads x 16 keys; 1024 threads for large
+//     single sorts, 256 as a batch tuning option) takes a tile id from an
 //     atomic counter (ids follow start order, so every look-back target is
 //     already resident), ranks keys into per-wave LDS histograms (stable:
 //     items are consumed in input order; lanes sharing a digit meet in a
@@ -369,6 +370,10 @@ static void onesweep_pass(int block, bool vals, const uint64_t* kin, uint64_t* k
     else if (BITS == 8 && block == kSortThreadsWide)
         hipLaunchKernelGGL((k_onesweep<kSortThreadsWide, 8, false, LDS_MATCH>), dim3(tiles), dim3(kSortThreadsWide), 0,
                            s, kin, kout, vin, vout, n, shift, mask, hist_tot, status, tile_ctr);
+    else if (block == kSortThreadsSmall)
+        hipLaunchKernelGGL((k_onesweep<kSortThreadsSmall, BITS, false, LDS_MATCH>), dim3(tiles),
+                           dim3(kSortThreadsSmall), 0, s, kin, kout, vin, vout, n, shift, mask, hist_tot, status,
+                           tile_ctr);
     else
         hipLaunchKernelGGL((k_onesweep<kSortThreads, BITS, false, LDS_MATCH>), dim3(tiles), dim3(kSortThreads), 0, s,
                            kin, kout, vin, vout, n, shift, mask, hist_tot, status, tile_ctr);

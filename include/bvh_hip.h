@@ -65,7 +65,9 @@ typedef struct bvh_batch_rec* bvh_batch_t;
 
 /* Build flags. */
 #define BVH_BUILD_KEEP_KEYS 1u /* keep the sorted keys for bvh_debug_sorted_keys */
-#define BVH_BUILD_TUNE 2u      /* batches: time candidate launch configurations at creation, keep the fastest */
+#define BVH_BUILD_TUNE 2u      /* batches: time candidate launch configurations at creation (digit width, sort
+                                  tile, front-end chunk, PLOC search and merge grids, one at a time), keep the
+                                  fastest; every candidate must give the same trees */
 
 typedef struct bvh_build_params {
     int32_t algo;           /* BVH_ALGO_* */
@@ -182,10 +184,15 @@ typedef struct bvh_batch_info {
     int32_t graph_nodes;               /* kernel nodes in the captured graph */
     int32_t runs;
     int32_t eager_continuations;       /* runs that needed iterations beyond the graph */
+    int32_t sort_block;                /* threads per onesweep tile in use (256 or 512) */
+    int32_t front_chunk;               /* triangles per front-end block in use */
     int32_t tune_count;                /* candidates timed by BVH_BUILD_TUNE */
-    float tune_ms[8];                  /* their median device times */
-    int32_t tune_radix_bits[8];
-    int32_t tune_nn_grid[8];
+    float tune_ms[16];                 /* their median device times */
+    int32_t tune_radix_bits[16];       /* and their settings */
+    int32_t tune_nn_grid[16];
+    int32_t tune_sort_block[16];
+    int32_t tune_front_chunk[16];
+    int32_t tune_merge_grid[16];
 } bvh_batch_info;
 
 /* The same segmented build launched directly on the context stream (no

@@ -105,12 +105,28 @@ def test_batch_single_mesh(ctx):
         _assert_identical(results, ref)
 
 
-def test_batch_tuning_is_result_invariant(ctx):
+@pytest.mark.parametrize("algo,knobs", [(B.ALGO_LBVH, 3), (B.ALGO_PLOC, 5)])
+def test_batch_tuning_is_result_invariant(ctx, algo, knobs):
+    """Coordinate descent over digit width, sort tile, front-end chunk (and for
+    PLOC the search and merge grids): one baseline plus one alternative per
+    knob, every candidate's trees equal (checked inside by digest), the chosen
+    configuration's trees equal to standalone builds."""
     meshes = _meshes(5, 16, 1000, 20000)
-    results, ref, info = _run_batch(ctx, meshes, B.ALGO_PLOC, tune=True, runs=1)
+    results, ref, info = _run_batch(ctx, meshes, algo, tune=True, runs=1)
     _assert_identical(results, ref)
-    assert info["tune_count"] >= 2
-    assert info["radix_bits"] in (8, 10)
+    assert info["tune_count"] == 1 + knobs
+    tried = info["tune"]
+    assert {t["radix_bits"] for t in tried} == {8, 10}
+    assert {t["sort_block"] for t in tried} == {256, 512}
+    assert {t["front_chunk"] for t in tried} == {4096, 8192}
+    if algo == B.ALGO_PLOC:
+        assert {t["nn_grid"] for t in tried} == {1024, 2048}
+        assert {t["merge_grid"] for t in tried} == {512, 1024}
+    best = min(tried, key=lambda t: t["ms"])  # the first fastest, as the tuner keeps it
+    assert (info["radix_bits"], info["sort_block"], info["front_chunk"]) == \
+        (best["radix_bits"], best["sort_block"], best["front_chunk"])
+    if algo == B.ALGO_PLOC:
+        assert (info["ploc_nn_grid"], info["ploc_merge_grid"]) == (best["nn_grid"], best["merge_grid"])
 
 
 def test_batch_rejects_nonfinite(ctx):

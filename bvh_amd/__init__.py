@@ -15,7 +15,7 @@ from . import _native as N
 from ._native import NODE_DTYPE, NativeError, ALGO_LBVH, ALGO_LBVH_PLUS, ALGO_PLOC  # noqa: F401
 
 __all__ = [
-    "Mesh", "Context", "DeviceMesh", "Tree", "LBVHBuilder", "LBVHPlusBuilder", "PLOCBuilder", "NODE_DTYPE",
+    "Mesh", "Context", "DeviceMesh", "DeviceBuffer", "Tree", "LBVHBuilder", "LBVHPlusBuilder", "PLOCBuilder", "NODE_DTYPE",
     "NativeError", "rays_sphere_camera", "rays_sphere_to_box", "rays_uniform_cube", "rays_camera", "Batch",
 ]
 
@@ -250,6 +250,30 @@ class Context:
         self._check(self._lib.bvh_build_batched(self.h, hm, ht, k, C.byref(p), C.byref(t)), "build_batched")
         return {"total_ms": float(t.total_ms), "stages": t.as_dict(), "ploc_iterations": int(t.ploc_iterations)}
 
+    def build_async(self, dmesh: "DeviceMesh", tree: "Tree", algo=ALGO_LBVH, radius=25, rounds=1, radix_bits=0):
+        """Enqueues a build of `tree` (bvh_build_async); the tree is valid after sync()."""
+        p = N.BuildParams(algo, radius, rounds, radix_bits, 0)
+        self._check(self._lib.bvh_build_async(self.h, dmesh.h, C.byref(p), tree.h), "build_async")
+
+    def trace_async(self, tree: "Tree", dmesh: "DeviceMesh", rays: "DeviceBuffer", t: "DeviceBuffer",
+                    prim: "DeviceBuffer", counters: "DeviceBuffer | None" = None):
+        """Enqueues a closest-hit trace over device buffers (bvh_trace_async):
+        rays 24 B, t 4 B, prim 4 B (and counters 12 B) per ray."""
+        n = rays.nbytes // 24
+        self._check(self._lib.bvh_trace_async(self.h, tree.h, dmesh.h, rays.ptr, n, t.ptr, prim.ptr,
+                                              counters.ptr if counters is not None else None), "trace_async")
+
+    def sync(self):
+        """Waits for the context's queued work; raises the first failure of the
+        asynchronous calls since the previous sync (bvh_sync)."""
+        self._check(self._lib.bvh_sync(self.h), "sync")
+
+    def buffer(self, nbytes: int) -> "DeviceBuffer":
+        """Device memory of nbytes bytes (bvh_device_alloc)."""
+        h = C.c_void_p()
+        self._check(self._lib.bvh_device_alloc(self.h, max(16, nbytes), C.byref(h)), "device alloc")
+        return DeviceBuffer(self, h.value, nbytes)
+
     def trace(self, tree: "Tree", dmesh: "DeviceMesh", rays: np.ndarray, counters=False):
         """Closest hit for host rays (n, 6) float32; returns (t, prim[, counters], ms)."""
         rays = np.ascontiguousarray(rays, dtype=np.float32).reshape(-1, 6)
@@ -323,6 +347,31 @@ class Batch:
         if self.h:
             self.ctx._lib.bvh_batch_free(self.h)
             self.h = None
+
+
+class DeviceBuffer:
+    """A device allocation with synchronous host copies."""
+
+    def __init__(self, ctx: Context, ptr: int, nbytes: int):
+        self.ctx, self.ptr, self.nbytes = ctx, ptr, nbytes
+
+    def write(self, a: np.ndarray):
+        a = np.ascontiguousarray(a)
+        if a.nbytes > self.nbytes:
+            raise ValueError("array larger than the buffer")
+        self.ctx._check(self.ctx._lib.bvh_copy_h2d(self.ctx.h, self.ptr, a.ctypes.data, a.nbytes), "h2d")
+
+    def read(self, dtype, count: int) -> np.ndarray:
+        out = np.empty(count, dtype=dtype)
+        if out.nbytes > self.nbytes:
+            raise ValueError("read past the buffer")
+        self.ctx._check(self.ctx._lib.bvh_copy_d2h(self.ctx.h, out.ctypes.data, self.ptr, out.nbytes), "d2h")
+        return out
+
+    def free(self):
+        if self.ptr:
+            self.ctx._lib.bvh_device_free(self.ctx.h, self.ptr)
+            self.ptr = None
 
 
 class DeviceMesh:

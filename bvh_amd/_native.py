@@ -107,6 +107,7 @@ _HIP_SIGS = {
     "bvh_mesh_upload": (_i32, [_vp, C.POINTER(_vp), _u32, _pp]),
     "bvh_mesh_free": (_i32, [_vp, _vp]),
     "bvh_mesh_size": (_i32, [_vp, C.POINTER(_u32)]),
+    "bvh_mesh_check": (_i32, [_vp]),
     "bvh_tree_create": (_i32, [_vp, _pp]),
     "bvh_tree_free": (_i32, [_vp, _vp]),
     "bvh_tree_info": (_i32, [_vp, C.POINTER(_u32), C.POINTER(_u32)]),
@@ -130,6 +131,9 @@ _HIP_SIGS = {
     "bvh_tree_prepare": (_i32, [_vp, _vp]),
     "bvh_trace": (_i32, [_vp, _vp, _vp, _vp, _u32, _vp, _vp, _vp, C.POINTER(_f)]),
     "bvh_trace_host": (_i32, [_vp, _vp, _vp, _vp, _u32, _vp, _vp, _vp, C.POINTER(_f)]),
+    "bvh_build_async": (_i32, [_vp, _vp, C.POINTER(BuildParams), _vp]),
+    "bvh_trace_async": (_i32, [_vp, _vp, _vp, _vp, _u32, _vp, _vp, _vp]),
+    "bvh_sync": (_i32, [_vp]),
 }
 
 _HOST_SIGS = {

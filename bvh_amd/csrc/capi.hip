@@ -64,6 +64,17 @@ struct SegTables {
     }
 };
 
+// An asynchronous call whose status words are still on their way to the host:
+// a stream-ordered copy lands them in a 16-word pinned slot, read by bvh_sync.
+enum PendingKind { kPendBuild, kPendPloc, kPendTrace };
+struct Pending {
+    PendingKind kind;
+    uint32_t n;  // triangles of the build (PLOC record check)
+};
+constexpr uint32_t kPinSlotWords = 16;     // build: 8 flag words + PLOC record; trace: 2 overflow words
+constexpr uint32_t kPinChunkSlots = 256;   // slots per pinned allocation (never moved while in use)
+constexpr uint32_t kMaxPending = 1u << 16; // asynchronous calls between two bvh_sync
+
 struct bvh_context {
     int device = 0;
     hipStream_t stream = nullptr;
@@ -76,6 +87,8 @@ struct bvh_context {
     DevBuf trace_spill;                      // spill slots of deep traversal stacks
     DevBuf ray_keys[2];                      // ray coherence keys (sorted before tracing)
     uint32_t* host_flags = nullptr;          // pinned: a single build's 8 flag words land here
+    std::vector<uint32_t*> pin_chunks;       // pinned status slots of asynchronous calls
+    std::vector<Pending> pending;            // asynchronous calls since the last bvh_sync, in order
     // last build's sorted keys (for bvh_debug_sorted_keys)
     uint64_t* last_keys = nullptr;
     uint32_t* last_vals = nullptr;
@@ -86,6 +99,7 @@ struct bvh_mesh_rec {
     float* base = nullptr;  // 9 x n floats, one allocation
     float4* tri = nullptr;  // 3 x float4 per triangle (traversal copy)
     uint32_t n = 0;
+    bool nonfinite = false;  // a coordinate is NaN or infinite: every build is refused
     size_t stride = 0;      // floats between arrays
     MeshView view() const {
         MeshView m;
@@ -398,6 +412,58 @@ HierArgs hier_args(Workspace& w, const Job& j, const uint64_t* keys, const uint3
     return ha;
 }
 
+// Status of a single build from its flag words (non-finite input, internal error).
+int single_flags_status(bvh_ctx_t ctx, const uint32_t* sc) {
+    if (sc[6]) return set_err(ctx, BVH_E_NONFINITE, "mesh contains NaN or infinite vertex coordinates");
+    if (sc[7]) return set_err(ctx, BVH_E_STATE, "internal error: sorted keys name no primitive");
+    return BVH_OK;
+}
+
+// Status of one mesh's PLOC record (clusters left, nodes made, iterations, flags).
+int ploc_record_status(bvh_ctx_t ctx, const uint4& o, uint32_t s, uint32_t n) {
+    if (o.w & kPlocUnfinished) return set_err(ctx, BVH_E_STATE, "internal error: PLOC tail ran before mesh %u was small", s);
+    if (o.w & kPlocNoProgress)
+        return set_err(ctx, BVH_E_NOPROGRESS, "PLOC iteration %u of mesh %u made no merge (%u clusters left)", o.z, s,
+                       o.x);
+    if (o.x != 1 || o.y != 2 * n - 1)
+        return set_err(ctx, BVH_E_STATE, "PLOC of mesh %u ended with %u clusters and %u nodes", s, o.x, o.y);
+    return BVH_OK;
+}
+
+// Pinned slot of the next asynchronous call (its index is its place in ctx->pending).
+int pending_slot(bvh_ctx_t ctx, uint32_t** slot) {
+    const size_t idx = ctx->pending.size();
+    if (idx >= kMaxPending)
+        return set_err(ctx, BVH_E_STATE, "%u asynchronous calls without bvh_sync", (unsigned)kMaxPending);
+    const size_t chunk = idx / kPinChunkSlots;
+    while (ctx->pin_chunks.size() <= chunk) {
+        uint32_t* p = nullptr;
+        HIP_TRY(ctx, hipHostMalloc((void**)&p, (size_t)kPinChunkSlots * kPinSlotWords * 4, hipHostMallocDefault));
+        ctx->pin_chunks.push_back(p);
+    }
+    *slot = ctx->pin_chunks[chunk] + (idx % kPinChunkSlots) * kPinSlotWords;
+    return BVH_OK;
+}
+
+// Enqueues the copy of a single build's status words (and, for PLOC, its
+// record) into the call's pinned slot; bvh_sync checks them.
+int defer_build_check(bvh_ctx_t ctx, Workspace& w, PendingKind kind, uint32_t n) {
+    uint32_t* slot = nullptr;
+    TRY(pending_slot(ctx, &slot));
+    HIP_TRY(ctx, hipMemcpyAsync(slot, w.scene.p, 8 * 4, hipMemcpyDeviceToHost, ctx->stream));
+    if (kind == kPendPloc) HIP_TRY(ctx, hipMemcpyAsync(slot + 8, w.ploc_out.p, 16, hipMemcpyDeviceToHost, ctx->stream));
+    ctx->pending.push_back(Pending{kind, n});
+    return BVH_OK;
+}
+
+// Meshes whose upload found a NaN or infinite coordinate are never built.
+int refuse_nonfinite(bvh_ctx_t ctx, const bvh_mesh_t* meshes, uint32_t k) {
+    for (uint32_t s = 0; s < k; ++s)
+        if (meshes[s]->nonfinite)
+            return set_err(ctx, BVH_E_NONFINITE, "mesh %u of the batch contains NaN or infinite vertex coordinates", s);
+    return BVH_OK;
+}
+
 // Non-finite input and internal-error words of every segment.
 int check_input_flags(bvh_ctx_t ctx, Workspace& w, uint32_t k, uint32_t* host_dst = nullptr) {
     // Single builds read their flag words into a pinned buffer (a pageable
@@ -429,7 +495,8 @@ int check_input_flags(bvh_ctx_t ctx, Workspace& w, uint32_t k, uint32_t* host_ds
     return BVH_OK;
 }
 
-int build_lbvh(bvh_ctx_t ctx, bvh_mesh_t mesh, const bvh_build_params* prm, bvh_tree_t tree, bvh_stage_times* times) {
+int build_lbvh(bvh_ctx_t ctx, bvh_mesh_t mesh, const bvh_build_params* prm, bvh_tree_t tree, bvh_stage_times* times,
+               bool async) {
     TRY(ensure_tree(ctx, tree, mesh->n));
     Job j = single_job(mesh, tree, false);
     SortPlan plan = plan_for(j, prm->radix_bits);
@@ -447,7 +514,7 @@ int build_lbvh(bvh_ctx_t ctx, bvh_mesh_t mesh, const bvh_build_params* prm, bvh_
     ctx->last_keys = keys;
     ctx->last_vals = nullptr;
     ctx->last_n = mesh->n;
-    return check_input_flags(ctx, w, 1);
+    return async ? defer_build_check(ctx, w, kPendBuild, mesh->n) : check_input_flags(ctx, w, 1);
 }
 
 // LBVH+ round schedule: ready lists of at least kTreeletLevelMin nodes take
@@ -461,7 +528,7 @@ size_t treelet_list_words(uint32_t n) { return ((size_t)n + 63) & ~(size_t)63; }
 size_t treelet_list_bytes(uint32_t n) { return (treelet_list_words(n) + kTreeletLevels + 2) * 4; }
 
 int build_lbvh_plus(bvh_ctx_t ctx, bvh_mesh_t mesh, const bvh_build_params* prm, bvh_tree_t tree,
-                    bvh_stage_times* times) {
+                    bvh_stage_times* times, bool async) {
     const uint32_t n = mesh->n;
     const int rounds = prm->treelet_rounds;
     if (rounds < 0 || rounds > 64) return set_err(ctx, BVH_E_ARG, "treelet_rounds %d out of range [0, 64]", rounds);
@@ -503,7 +570,7 @@ int build_lbvh_plus(bvh_ctx_t ctx, bvh_mesh_t mesh, const bvh_build_params* prm,
     ctx->last_keys = keys;
     ctx->last_vals = vals;
     ctx->last_n = n;
-    return check_input_flags(ctx, w, 1);
+    return async ? defer_build_check(ctx, w, kPendBuild, n) : check_input_flags(ctx, w, 1);
 }
 
 // ---- PLOC ---------------------------------------------------------------------
@@ -607,19 +674,18 @@ int ploc_check(bvh_ctx_t ctx, Workspace& w, const Job& j, const std::vector<uint
             *unfinished = true;
             continue;
         }
-        if (o.w & kPlocNoProgress)
-            return set_err(ctx, BVH_E_NOPROGRESS, "PLOC iteration %u of mesh %u made no merge (%u clusters left)", o.z,
-                           s, o.x);
-        const uint32_t n = ns[s];
-        if (o.x != 1 || o.y != 2 * n - 1)
-            return set_err(ctx, BVH_E_STATE, "PLOC of mesh %u ended with %u clusters and %u nodes", s, o.x, o.y);
+        TRY(ploc_record_status(ctx, o, s, ns[s]));
         mi = std::max(mi, o.z);
     }
     if (max_iters) *max_iters = mi;
     return BVH_OK;
 }
 
-int build_ploc(bvh_ctx_t ctx, bvh_mesh_t mesh, const bvh_build_params* prm, bvh_tree_t tree, bvh_stage_times* times) {
+// PLOC's global iterations are driven from the host (one 32-byte state read
+// per batch of launches), so an asynchronous PLOC build returns once they are
+// enqueued and done; the LDS tail and the checks stay deferred.
+int build_ploc(bvh_ctx_t ctx, bvh_mesh_t mesh, const bvh_build_params* prm, bvh_tree_t tree, bvh_stage_times* times,
+               bool async) {
     const uint32_t n = mesh->n;
     const int radius = prm->ploc_radius;
     if (radius < 1 || radius > 1024) return set_err(ctx, BVH_E_ARG, "PLOC radius %d out of range [1, 1024]", radius);
@@ -648,6 +714,7 @@ int build_ploc(bvh_ctx_t ctx, bvh_mesh_t mesh, const bvh_build_params* prm, bvh_
     ctx->last_keys = keys;
     ctx->last_vals = nullptr;
     ctx->last_n = n;
+    if (async) return defer_build_check(ctx, w, kPendPloc, n);
     TRY(check_input_flags(ctx, w, 1));
     bool unfinished = false;
     uint32_t iters = 0;
@@ -1001,6 +1068,66 @@ int prepare_trace(bvh_ctx_t ctx, bvh_tree_t tree) {
     return BVH_OK;
 }
 
+// Enqueues one traversal on the context stream (ray ordering, traversal,
+// deep re-trace); *ovf_dev = the device words whose second one flags a ray
+// that needed more stack than the deep path has.
+int trace_enqueue(bvh_ctx_t ctx, bvh_tree_t tree, bvh_mesh_t mesh, const float* d_rays, uint32_t nrays, float* d_t,
+                  int32_t* d_prim, int32_t* d_counters, const uint32_t** ovf_dev) {
+    int st;
+    if ((st = ensure(ctx, ctx->trace_ovf, 64))) return st;
+    if ((st = ensure(ctx, ctx->trace_list, (size_t)nrays * 4))) return st;
+    if ((st = ensure(ctx, ctx->trace_spill, (size_t)trace_spill_slots(nrays) * trace_spill_entries() * 8))) return st;
+    if ((st = prepare_trace(ctx, tree))) return st;
+    TraceArgs a;
+    a.order = nullptr;
+    a.order_mask = 0;
+    // Large ray sets are traced in coherence order (a 40-bit Morton code of
+    // origin and direction): neighbouring lanes then walk similar paths.
+    constexpr int kRayKeyBits = 40;
+    const int idx_bits = std::max(1, bit_width(nrays - 1));
+    if (nrays >= kRaySortMin && idx_bits + kRayKeyBits <= 64) {
+        SortPlan plan = make_sort_plan(nrays, idx_bits, idx_bits + kRayKeyBits, 8);
+        Workspace& w = ctx->ws;
+        const size_t bins = 1u << plan.bits;
+        TRY(ensure(ctx, ctx->ray_keys[0], (size_t)nrays * 8));
+        TRY(ensure(ctx, ctx->ray_keys[1], (size_t)nrays * 8));
+        TRY(ensure(ctx, w.hist_tot, (size_t)plan.passes * bins * 4));
+        TRY(ensure(ctx, w.lookback, zero_bytes_of(plan)));
+        launch_ray_keys(d_rays, nrays, P<DNode>(tree->nodes), (uint32_t)idx_bits, P<uint64_t>(ctx->ray_keys[0]),
+                        ctx->stream);
+        launch_sort_histogram(P<uint64_t>(ctx->ray_keys[0]), nrays, plan, P<uint32_t>(w.hist_tot), ctx->stream);
+        launch_fill_zero(P<uint32_t>(w.lookback), zero_bytes_of(plan) / 4, ctx->stream);
+        uint64_t* kout = nullptr;
+        uint32_t* vout = nullptr;
+        if (!launch_onesweep(plan, nrays, P<uint64_t>(ctx->ray_keys[0]), P<uint64_t>(ctx->ray_keys[1]), nullptr,
+                             nullptr, P<uint32_t>(w.hist_tot), P<uint32_t>(w.lookback), &kout, &vout, ctx->stream))
+            return set_err(ctx, BVH_E_ARG, "sort plan made for keys only");
+        TRY(check_launch(ctx, "ray ordering"));
+        a.order = kout;
+        a.order_mask = (1ull << idx_bits) - 1;
+    }
+    a.nodes = P<DNode>(tree->nodes);
+    a.rec = P<float4>(tree->rec);
+    a.num_nodes = tree->num_nodes;
+    a.num_prims = mesh->n;
+    a.tri = mesh->tri;
+    a.rays = d_rays;
+    a.nrays = nrays;
+    a.t_out = d_t;
+    a.prim_out = d_prim;
+    a.counters = d_counters;
+    a.overflow_count = P<uint32_t>(ctx->trace_ovf);
+    a.overflow_list = P<uint32_t>(ctx->trace_list);
+    a.spill = P<int2>(ctx->trace_spill);
+    a.spill_slots = trace_spill_slots(nrays);
+    launch_fill_zero(a.overflow_count, 4, ctx->stream);
+    launch_trace(a, ctx->stream);
+    launch_trace_deep(a, ctx->stream);
+    if ((st = check_launch(ctx, "k_trace"))) return st;
+    *ovf_dev = a.overflow_count;
+    return BVH_OK;
+}
+
 }  // namespace
 
 namespace {
@@ -1237,6 +1364,7 @@ int bvh_ctx_destroy(bvh_ctx_t ctx) {
     free_buf(ctx->ray_keys[1]);
     free_buf(ctx->trace_io);
     if (ctx->host_flags) (void)hipHostFree(ctx->host_flags);
+    for (uint32_t* p : ctx->pin_chunks) (void)hipHostFree(p);
     for (auto& e : ctx->ev)
         if (e) (void)hipEventDestroy(e);
     (void)hipStreamDestroy(ctx->stream);
@@ -1307,19 +1435,26 @@ int bvh_mesh_upload(bvh_ctx_t ctx, const float* const* soa, uint32_t n, bvh_mesh
             return set_err(ctx, BVH_E_HIP, "mesh upload failed: %s", hipGetErrorString(e));
         }
     }
-    e = hipMalloc(&m->tri, (size_t)n * 48);
+    // The traversal copy is followed by one word that the copy kernel sets
+    // when any coordinate is NaN or infinite.
+    e = hipMalloc(&m->tri, (size_t)n * 48 + 64);
     if (e != hipSuccess) {
         (void)hipFree(m->base);
         delete m;
         return set_err(ctx, BVH_E_NOMEM, "mesh allocation failed: %s", hipGetErrorString(e));
     }
-    launch_tri_aos(m->view(), m->tri, ctx->stream);
-    if (hipStreamSynchronize(ctx->stream) != hipSuccess) {
+    uint32_t* bad = reinterpret_cast<uint32_t*>(m->tri + 3 * (size_t)n);
+    launch_fill_zero(bad, 1, ctx->stream);
+    launch_tri_aos(m->view(), m->tri, bad, ctx->stream);
+    uint32_t hbad = 0;
+    if (hipMemcpyAsync(&hbad, bad, 4, hipMemcpyDeviceToHost, ctx->stream) != hipSuccess ||
+        hipStreamSynchronize(ctx->stream) != hipSuccess) {
         (void)hipFree(m->base);
         (void)hipFree(m->tri);
         delete m;
         return set_err(ctx, BVH_E_HIP, "mesh upload sync failed");
     }
+    m->nonfinite = hbad != 0;
     *out = m;
     return BVH_OK;
 }
@@ -1331,6 +1466,11 @@ int bvh_mesh_free(bvh_ctx_t ctx, bvh_mesh_t mesh) {
     if (mesh->tri) HIP_TRY(ctx, hipFree(mesh->tri));
     delete mesh;
     return BVH_OK;
+}
+
+int bvh_mesh_check(bvh_mesh_t mesh) {
+    if (!mesh) return BVH_E_ARG;
+    return mesh->nonfinite ? BVH_E_NONFINITE : BVH_OK;
 }
 
 int bvh_mesh_size(bvh_mesh_t mesh, uint32_t* n) {
@@ -1404,15 +1544,55 @@ int bvh_build(bvh_ctx_t ctx, bvh_mesh_t mesh, const bvh_build_params* params, bv
               bvh_stage_times* times) {
     if (!ctx || !mesh || !params || !tree) return ctx ? set_err(ctx, BVH_E_ARG, "null argument") : BVH_E_ARG;
     if (mesh->n == 0) return set_err(ctx, BVH_E_ARG, "mesh has no triangles");
+    if (mesh->nonfinite) return set_err(ctx, BVH_E_NONFINITE, "mesh contains NaN or infinite vertex coordinates");
     if (params->radix_bits != 0 && params->radix_bits != 8 && params->radix_bits != 10)
         return set_err(ctx, BVH_E_UNSUPPORTED, "radix_bits must be 0, 8 or 10");
     HIP_TRY(ctx, hipSetDevice(ctx->device));
     switch (params->algo) {
-        case BVH_ALGO_LBVH: return build_lbvh(ctx, mesh, params, tree, times);
-        case BVH_ALGO_PLOC: return build_ploc(ctx, mesh, params, tree, times);
-        case BVH_ALGO_LBVH_PLUS: return build_lbvh_plus(ctx, mesh, params, tree, times);
+        case BVH_ALGO_LBVH: return build_lbvh(ctx, mesh, params, tree, times, false);
+        case BVH_ALGO_PLOC: return build_ploc(ctx, mesh, params, tree, times, false);
+        case BVH_ALGO_LBVH_PLUS: return build_lbvh_plus(ctx, mesh, params, tree, times, false);
         default: return set_err(ctx, BVH_E_UNSUPPORTED, "algorithm %d not available", params->algo);
     }
+}
+
+int bvh_build_async(bvh_ctx_t ctx, bvh_mesh_t mesh, const bvh_build_params* params, bvh_tree_t tree) {
+    if (!ctx || !mesh || !params || !tree) return ctx ? set_err(ctx, BVH_E_ARG, "null argument") : BVH_E_ARG;
+    if (mesh->n == 0) return set_err(ctx, BVH_E_ARG, "mesh has no triangles");
+    if (mesh->nonfinite) return set_err(ctx, BVH_E_NONFINITE, "mesh contains NaN or infinite vertex coordinates");
+    if (params->radix_bits != 0 && params->radix_bits != 8 && params->radix_bits != 10)
+        return set_err(ctx, BVH_E_UNSUPPORTED, "radix_bits must be 0, 8 or 10");
+    HIP_TRY(ctx, hipSetDevice(ctx->device));
+    switch (params->algo) {
+        case BVH_ALGO_LBVH: return build_lbvh(ctx, mesh, params, tree, nullptr, true);
+        case BVH_ALGO_PLOC: return build_ploc(ctx, mesh, params, tree, nullptr, true);
+        case BVH_ALGO_LBVH_PLUS: return build_lbvh_plus(ctx, mesh, params, tree, nullptr, true);
+        default: return set_err(ctx, BVH_E_UNSUPPORTED, "algorithm %d not available", params->algo);
+    }
+}
+
+int bvh_sync(bvh_ctx_t ctx) {
+    if (!ctx) return BVH_E_ARG;
+    HIP_TRY(ctx, hipSetDevice(ctx->device));
+    const hipError_t e = hipStreamSynchronize(ctx->stream);
+    std::vector<Pending> pending;
+    pending.swap(ctx->pending);
+    if (e != hipSuccess) return set_err(ctx, BVH_E_HIP, "hipStreamSynchronize failed: %s", hipGetErrorString(e));
+    for (size_t i = 0; i < pending.size(); ++i) {
+        const uint32_t* sc = ctx->pin_chunks[i / kPinChunkSlots] + (i % kPinChunkSlots) * kPinSlotWords;
+        const Pending& p = pending[i];
+        int st = BVH_OK;
+        if (p.kind == kPendTrace) {
+            if (sc[1])
+                st = set_err(ctx, BVH_E_STATE,
+                             "traversal needed more than 256 stack entries (or the nodes are not a tree)");
+        } else {
+            st = single_flags_status(ctx, sc);
+            if (st == BVH_OK && p.kind == kPendPloc) st = ploc_record_status(ctx, make_uint4(sc[8], sc[9], sc[10], sc[11]), 0, p.n);
+        }
+        if (st != BVH_OK) return st;  // the first failure in call order
+    }
+    return BVH_OK;
 }
 
 int bvh_build_lbvh_noalloc(bvh_ctx_t ctx, const float* const* d_soa, uint32_t n, void* d_nodes,
@@ -1531,6 +1711,7 @@ int bvh_batch_create(bvh_ctx_t ctx, const bvh_mesh_t* meshes, const bvh_tree_t* 
         return set_err(ctx, BVH_E_UNSUPPORTED, "radix_bits must be 0, 8 or 10");
     for (uint32_t s = 0; s < k; ++s)
         if (!meshes[s] || !trees[s]) return set_err(ctx, BVH_E_ARG, "mesh or tree %u is null", s);
+    TRY(refuse_nonfinite(ctx, meshes, k));
     HIP_TRY(ctx, hipSetDevice(ctx->device));
     bvh_batch_rec* b = new bvh_batch_rec();
     std::memset(&b->info, 0, sizeof(b->info));
@@ -1581,6 +1762,9 @@ int bvh_build_batched(bvh_ctx_t ctx, const bvh_mesh_t* meshes, const bvh_tree_t*
         return set_err(ctx, BVH_E_ARG, "PLOC radius %d out of range [1, 1024]", params->ploc_radius);
     if (params->radix_bits != 0 && params->radix_bits != 8 && params->radix_bits != 10)
         return set_err(ctx, BVH_E_UNSUPPORTED, "radix_bits must be 0, 8 or 10");
+    for (uint32_t s = 0; s < k; ++s)
+        if (!meshes[s] || !trees[s]) return set_err(ctx, BVH_E_ARG, "mesh or tree %u is null", s);
+    TRY(refuse_nonfinite(ctx, meshes, k));
     HIP_TRY(ctx, hipSetDevice(ctx->device));
     return build_batched_eager(ctx, meshes, trees, k, params, times);
 }
@@ -1621,64 +1805,30 @@ int bvh_trace(bvh_ctx_t ctx, bvh_tree_t tree, bvh_mesh_t mesh, const float* d_ra
     if (tree->num_nodes == 0) return set_err(ctx, BVH_E_STATE, "tree is empty");
     if (nrays == 0) return BVH_OK;
     HIP_TRY(ctx, hipSetDevice(ctx->device));
-    int st;
-    if ((st = ensure(ctx, ctx->trace_ovf, 64))) return st;
-    if ((st = ensure(ctx, ctx->trace_list, (size_t)nrays * 4))) return st;
-    if ((st = ensure(ctx, ctx->trace_spill, (size_t)trace_spill_slots(nrays) * trace_spill_entries() * 8))) return st;
     HIP_TRY(ctx, hipEventRecord(ctx->ev[0], ctx->stream));
-    if ((st = prepare_trace(ctx, tree))) return st;
-    TraceArgs a;
-    a.order = nullptr;
-    a.order_mask = 0;
-    // Large ray sets are traced in coherence order (a 40-bit Morton code of
-    // origin and direction): neighbouring lanes then walk similar paths.
-    constexpr int kRayKeyBits = 40;
-    const int idx_bits = std::max(1, bit_width(nrays - 1));
-    if (nrays >= kRaySortMin && idx_bits + kRayKeyBits <= 64) {
-        SortPlan plan = make_sort_plan(nrays, idx_bits, idx_bits + kRayKeyBits, 8);
-        Workspace& w = ctx->ws;
-        const size_t bins = 1u << plan.bits;
-        TRY(ensure(ctx, ctx->ray_keys[0], (size_t)nrays * 8));
-        TRY(ensure(ctx, ctx->ray_keys[1], (size_t)nrays * 8));
-        TRY(ensure(ctx, w.hist_tot, (size_t)plan.passes * bins * 4));
-        TRY(ensure(ctx, w.lookback, zero_bytes_of(plan)));
-        launch_ray_keys(d_rays, nrays, P<DNode>(tree->nodes), (uint32_t)idx_bits, P<uint64_t>(ctx->ray_keys[0]),
-                        ctx->stream);
-        launch_sort_histogram(P<uint64_t>(ctx->ray_keys[0]), nrays, plan, P<uint32_t>(w.hist_tot), ctx->stream);
-        launch_fill_zero(P<uint32_t>(w.lookback), zero_bytes_of(plan) / 4, ctx->stream);
-        uint64_t* kout = nullptr;
-        uint32_t* vout = nullptr;
-        if (!launch_onesweep(plan, nrays, P<uint64_t>(ctx->ray_keys[0]), P<uint64_t>(ctx->ray_keys[1]), nullptr,
-                             nullptr, P<uint32_t>(w.hist_tot), P<uint32_t>(w.lookback), &kout, &vout, ctx->stream))
-            return set_err(ctx, BVH_E_ARG, "sort plan made for keys only");
-        TRY(check_launch(ctx, "ray ordering"));
-        a.order = kout;
-        a.order_mask = (1ull << idx_bits) - 1;
-    }
-    a.nodes = P<DNode>(tree->nodes);
-    a.rec = P<float4>(tree->rec);
-    a.num_nodes = tree->num_nodes;
-    a.num_prims = mesh->n;
-    a.tri = mesh->tri;
-    a.rays = d_rays;
-    a.nrays = nrays;
-    a.t_out = d_t;
-    a.prim_out = d_prim;
-    a.counters = d_counters;
-    a.overflow_count = P<uint32_t>(ctx->trace_ovf);
-    a.overflow_list = P<uint32_t>(ctx->trace_list);
-    a.spill = P<int2>(ctx->trace_spill);
-    a.spill_slots = trace_spill_slots(nrays);
-    launch_fill_zero(a.overflow_count, 4, ctx->stream);
-    launch_trace(a, ctx->stream);
-    launch_trace_deep(a, ctx->stream);
-    if ((st = check_launch(ctx, "k_trace"))) return st;
+    const uint32_t* ovf_dev = nullptr;
+    TRY(trace_enqueue(ctx, tree, mesh, d_rays, nrays, d_t, d_prim, d_counters, &ovf_dev));
     HIP_TRY(ctx, hipEventRecord(ctx->ev[1], ctx->stream));
     uint32_t ovf[2] = {0, 0};
-    HIP_TRY(ctx, hipMemcpyAsync(ovf, a.overflow_count, 8, hipMemcpyDeviceToHost, ctx->stream));
+    HIP_TRY(ctx, hipMemcpyAsync(ovf, ovf_dev, 8, hipMemcpyDeviceToHost, ctx->stream));
     HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
     if (ms) HIP_TRY(ctx, hipEventElapsedTime(ms, ctx->ev[0], ctx->ev[1]));
     if (ovf[1]) return set_err(ctx, BVH_E_STATE, "traversal needed more than 256 stack entries (or the nodes are not a tree)");
+    return BVH_OK;
+}
+
+int bvh_trace_async(bvh_ctx_t ctx, bvh_tree_t tree, bvh_mesh_t mesh, const float* d_rays, uint32_t nrays, float* d_t,
+                    int32_t* d_prim, int32_t* d_counters) {
+    if (!ctx || !tree || !mesh || (nrays && (!d_rays || !d_t || !d_prim))) return BVH_E_ARG;
+    if (tree->num_nodes == 0) return set_err(ctx, BVH_E_STATE, "tree is empty");
+    if (nrays == 0) return BVH_OK;
+    HIP_TRY(ctx, hipSetDevice(ctx->device));
+    uint32_t* slot = nullptr;
+    TRY(pending_slot(ctx, &slot));
+    const uint32_t* ovf_dev = nullptr;
+    TRY(trace_enqueue(ctx, tree, mesh, d_rays, nrays, d_t, d_prim, d_counters, &ovf_dev));
+    HIP_TRY(ctx, hipMemcpyAsync(slot, ovf_dev, 8, hipMemcpyDeviceToHost, ctx->stream));
+    ctx->pending.push_back(Pending{kPendTrace, nrays});
     return BVH_OK;
 }
 

@@ -183,7 +183,7 @@ struct TraceArgs {
 int trace_spill_entries();
 // Spill slots bvh_trace provides for n rays.
 uint32_t trace_spill_slots(uint32_t nrays);
-void launch_tri_aos(const MeshView& m, float4* tri, hipStream_t s);
+void launch_tri_aos(const MeshView& m, float4* tri, uint32_t* bad, hipStream_t s);
 // Coherence keys of rays (direction bins, origin Morton code, ray index).
 void launch_ray_keys(const float* rays, uint32_t n, const DNode* nodes, uint32_t idx_bits, uint64_t* keys,
                      hipStream_t s);

@@ -425,20 +425,29 @@ void launch_ray_keys(const float* rays, uint32_t n, const DNode* nodes, uint32_t
     hipLaunchKernelGGL(k_ray_keys, dim3(g ? g : 1), dim3(256), 0, s, rays, n, nodes, idx_bits, keys);
 }
 
-// Mesh SoA -> 3 x float4 per triangle (v0, v1, v2; w = 0).
-__global__ void k_tri_aos(MeshView m, float4* __restrict__ tri) {
+// Mesh SoA -> 3 x float4 per triangle (v0, v1, v2; w = 0).  *bad (cleared
+// by the caller) becomes nonzero when any coordinate is NaN or infinite.
+__global__ void k_tri_aos(MeshView m, float4* __restrict__ tri, uint32_t* bad) {
     const uint32_t stride = gridDim.x * blockDim.x;
+    uint32_t any = 0;
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < m.n; i += stride) {
-        tri[3 * (size_t)i + 0] = make_float4(m.v[0][i], m.v[1][i], m.v[2][i], 0.0f);
-        tri[3 * (size_t)i + 1] = make_float4(m.v[3][i], m.v[4][i], m.v[5][i], 0.0f);
-        tri[3 * (size_t)i + 2] = make_float4(m.v[6][i], m.v[7][i], m.v[8][i], 0.0f);
+        float v[9];
+#pragma unroll
+        for (int k = 0; k < 9; ++k) {
+            v[k] = m.v[k][i];
+            any |= (__float_as_uint(v[k]) & 0x7f800000u) == 0x7f800000u ? 1u : 0u;
+        }
+        tri[3 * (size_t)i + 0] = make_float4(v[0], v[1], v[2], 0.0f);
+        tri[3 * (size_t)i + 1] = make_float4(v[3], v[4], v[5], 0.0f);
+        tri[3 * (size_t)i + 2] = make_float4(v[6], v[7], v[8], 0.0f);
     }
+    if (__ballot(any != 0) != 0 && (threadIdx.x & (kWave - 1)) == 0) atomicOr(bad, 1u);
 }
 
-void launch_tri_aos(const MeshView& m, float4* tri, hipStream_t s) {
+void launch_tri_aos(const MeshView& m, float4* tri, uint32_t* bad, hipStream_t s) {
     uint32_t g = (m.n + 255) / 256;
     if (g > 4096) g = 4096;
-    hipLaunchKernelGGL(k_tri_aos, dim3(g ? g : 1), dim3(256), 0, s, m, tri);
+    hipLaunchKernelGGL(k_tri_aos, dim3(g ? g : 1), dim3(256), 0, s, m, tri, bad);
 }
 
 template <int LDS>

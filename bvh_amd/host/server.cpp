@@ -380,6 +380,7 @@ void bvh_server_rec::run_slot(int slot) {
             Req& q = group[i];
             if (!q.build) continue;
             int st = bvh_mesh_upload(ctx, q.r.soa, q.r.ntri, &q.mesh);
+            if (st == BVH_OK) st = bvh_mesh_check(q.mesh);  // a non-finite mesh fails alone
             if (st == BVH_OK) st = bvh_tree_create(ctx, &q.tree);
             if (st != BVH_OK) {
                 status[i] = st;

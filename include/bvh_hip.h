@@ -111,6 +111,10 @@ int bvh_host_free_pinned(void* ptr);
 int bvh_mesh_upload(bvh_ctx_t ctx, const float* const* soa, uint32_t n, bvh_mesh_t* out);
 int bvh_mesh_free(bvh_ctx_t ctx, bvh_mesh_t mesh);
 int bvh_mesh_size(bvh_mesh_t mesh, uint32_t* n);
+/* BVH_E_NONFINITE when a vertex coordinate of the mesh is NaN or infinite
+ * (found by the upload; every build of such a mesh is refused before any
+ * launch with the same code), else BVH_OK.  The reference checks nothing. */
+int bvh_mesh_check(bvh_mesh_t mesh);
 
 /* ---- trees -------------------------------------------------------------- */
 int bvh_tree_create(bvh_ctx_t ctx, bvh_tree_t* out);
@@ -128,6 +132,34 @@ int bvh_tree_set_indices(bvh_ctx_t ctx, bvh_tree_t tree, const uint32_t* indices
 /* Synchronous: returns once the tree is complete; times may be NULL. */
 int bvh_build(bvh_ctx_t ctx, bvh_mesh_t mesh, const bvh_build_params* params, bvh_tree_t tree,
               bvh_stage_times* times);
+
+/* ---- asynchronous build and trace ---------------------------------------- */
+/* The reference's build and render calls are synchronous
+ * (LBVHBuilderCUDA::build, src/cuda/lbvh_builder.cu:484-488, ends in
+ * cudaEventSynchronize at :454; renderImage waits at src/render_engine.cu:789).
+ * These enqueue the same work on the context's stream and return without
+ * waiting for the device, so a host thread can queue several builds and traces
+ * back to back; results are valid, and failures are reported, only after
+ * bvh_sync.  Work on one context runs in call order.
+ *   bvh_build_async  the work of bvh_build (no stage times).  Argument errors
+ *                    and allocation failures return at once; non-finite input,
+ *                    PLOC failures and internal errors are reported by
+ *                    bvh_sync.  A call that has to grow the context's
+ *                    workspace or the tree's storage first waits for the work
+ *                    already queued; PLOC returns once its host-driven global
+ *                    iterations are done (their loop reads one 32-byte state
+ *                    per batch of launches), with its LDS tail still queued.
+ *   bvh_trace_async  the work of bvh_trace on device buffers (no time); a ray
+ *                    needing more than 256 stack entries is reported by bvh_sync.
+ *   bvh_sync         waits for everything queued on the context and returns the
+ *                    first failure among the asynchronous calls since the last
+ *                    bvh_sync, in call order (BVH_OK if none); it forgets them
+ *                    either way.  At most 65,536 asynchronous calls may be
+ *                    pending (then BVH_E_STATE until bvh_sync). */
+int bvh_build_async(bvh_ctx_t ctx, bvh_mesh_t mesh, const bvh_build_params* params, bvh_tree_t tree);
+int bvh_trace_async(bvh_ctx_t ctx, bvh_tree_t tree, bvh_mesh_t mesh, const float* d_rays, uint32_t nrays, float* d_t,
+                    int32_t* d_prim, int32_t* d_counters);
+int bvh_sync(bvh_ctx_t ctx);
 
 /* ---- device-memory footprint --------------------------------------------- */
 /* No reference counterpart (the reference allocates per build and never

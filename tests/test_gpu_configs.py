@@ -13,6 +13,12 @@
   its children, SAH non-increasing over treelet rounds), and a 65,536-ray
   sample of the 4,194,304 incoherent rays traced by the oracle through the
   same trees.
+* 512 meshes of 1,000-200,000 triangles (seed 11), LBVH and PLOC r = 16 in
+  one tuned captured launch sequence: every tree bit-identical to its own
+  standalone build, the smallest ones also to the oracle.
+* Serving requests drawn from the serving workload's ranges (10,000 to
+  2,000,000 triangles, 65,536 to 1,048,576 rays, seed 23): each result in the
+  mix equal to the request alone.
 
 Oracle references: lbvh_plus_builder.cu:341-366,483-495 (rounds),
 ploc_builder.cu:489-548 (iteration loop), render_engine.cu:289-356 (traversal).
@@ -168,3 +174,86 @@ def test_strands_32m_lbvh_plus_three_rounds(ctx, strands32m):
     for a, b in zip(costs, costs[1:]):
         assert b <= a * (1 + 1e-9)
     assert costs[-1] < costs[0]
+
+
+# ------------------------------------------------------- 512-mesh batch --
+
+@pytest.fixture(scope="module")
+def batch512(ctx):
+    """The batched workload: 512 meshes, log-uniform 1,000-200,000 triangles
+    (seed 11), each a uniform-cube mesh with its own seed, scale and offset."""
+    counts = B.batch_counts(11, 512, 1000, 200_000)
+    print(f"[configs] generating 512 meshes, {int(counts.sum()):,} triangles", flush=True)
+    meshes = [B.Mesh.batch_mesh(11, i, int(c)) for i, c in enumerate(counts)]
+    dms = [ctx.upload(m) for m in meshes]
+    yield meshes, dms
+    for dm in dms:
+        dm.free()
+
+
+@pytest.mark.parametrize("algo,radius", [(B.ALGO_LBVH, 25), (B.ALGO_PLOC, 16)])
+def test_batch_512_meshes_one_tuned_graph(ctx, batch512, algo, radius):
+    """One tuned, captured launch sequence over all 512 meshes: every tree is
+    bit-identical to its own standalone build, and the smallest meshes'
+    trees equal the oracle's."""
+    meshes, dms = batch512
+    trees = [ctx.tree() for _ in dms]
+    one = ctx.tree()
+    try:
+        batch = ctx.batch(dms, trees, algo=algo, radius=radius, tune=True)
+        try:
+            batch.run()
+            info = batch.info()
+        finally:
+            batch.free()
+        assert info["meshes"] == 512 and info["tune_count"] >= 4
+        for i, (dm, tree) in enumerate(zip(dms, trees)):
+            nodes, idx = tree.download()
+            ctx.build(dm, one, algo=algo, radius=radius, timing=False)
+            snodes, sidx = one.download()
+            assert nodes.tobytes() == snodes.tobytes(), f"mesh {i}"
+            np.testing.assert_array_equal(idx, sidx, err_msg=f"mesh {i}")
+        small = np.argsort([m.n for m in meshes])[:6]
+        for i in small:
+            nodes, idx = trees[i].download()
+            if algo == B.ALGO_PLOC:
+                st, onodes, oidx, _ = O.build_ploc(meshes[i].soa, radius)
+                assert st == 0
+            else:
+                onodes, oidx, _ = O.build_lbvh(meshes[i].soa)
+            assert O.nodes_equal(nodes, onodes) is None, f"mesh {i}"
+            np.testing.assert_array_equal(idx, oidx)
+    finally:
+        one.free()
+        for t in trees:
+            t.free()
+
+
+# ------------------------------------------------- serving request ranges --
+
+def test_serving_request_ranges_same_alone_and_in_mix(ctx):
+    """Requests drawn from the serving workload's ranges (blob meshes of
+    10,000-2,000,000 triangles, 65,536-1,048,576 rays, 20% follow-up traces,
+    seed 23), all submitted at once to four slots: each result equals a
+    plain build + trace of that request alone."""
+    from bvh_amd import serving as S
+    reqs = S.make_requests(23, 40, follow_up=0.2)
+    with S.Server(algo=B.ALGO_LBVH, slots=4, max_group=32) as srv:
+        done, outs, _ = S.run_open_loop(srv, reqs, np.zeros(len(reqs)))
+        stats = srv.stats()
+    assert [c["status"] for c in done] == [0] * len(reqs), done
+    assert stats["max_group_seen"] > 1, stats
+    mesh_of = {q["key"]: q["mesh"] for q in reqs if q["mesh"] is not None}
+    for q in reqs:
+        m = mesh_of[q["key"]]
+        dm = ctx.upload(m)
+        tree = ctx.tree()
+        try:
+            ctx.build(dm, tree, algo=B.ALGO_LBVH, timing=False)
+            t, prim, _ = ctx.trace(tree, dm, q["rays"])
+        finally:
+            tree.free()
+            dm.free()
+        got_t, got_prim = outs[q["id"]]
+        np.testing.assert_array_equal(got_prim, prim, err_msg=f"request {q['id']}")
+        assert got_t.tobytes() == t.tobytes(), f"request {q['id']}"

@@ -98,22 +98,25 @@ void launch_sort_histogram(const uint64_t* keys, uint32_t n, const SortPlan& pla
                            hist_tot);
 }
 
-// Exclusive sum over a block of T threads; returns the prefix, *total the sum.
+// Exclusive sums over a block of T threads of two 32-bit values at once
+// (packed in one 64-bit word: the low sum stays below 2^32, so no carry
+// crosses into the high one); returns the packed prefixes.
 template <int T>
-__device__ __forceinline__ uint32_t block_excl_sum(uint32_t v, uint32_t* s_w, uint32_t* total) {
+__device__ __forceinline__ uint64_t block_excl_sum2(uint32_t hi, uint32_t lo, uint64_t* s_w) {
     const int wave = threadIdx.x / kWave, lane = threadIdx.x % kWave;
-    uint32_t incl = wave_incl_sum(v);
+    const uint64_t v = ((uint64_t)hi << 32) | lo;
+    uint64_t incl = v;
+#pragma unroll
+    for (int o = 1; o < kWave; o <<= 1) {
+        const uint32_t a = __shfl_up((uint32_t)incl, o, kWave), b = __shfl_up((uint32_t)(incl >> 32), o, kWave);
+        if (lane >= o) incl += ((uint64_t)b << 32) | a;
+    }
     if (lane == kWave - 1) s_w[wave] = incl;
     __syncthreads();
-    uint32_t before = 0, all = 0;
+    uint64_t before = 0;
 #pragma unroll
-    for (int w = 0; w < T / kWave; ++w) {
-        uint32_t x = s_w[w];
-        before += (w < wave) ? x : 0u;
-        all += x;
-    }
+    for (int w = 0; w < T / kWave; ++w) before += (w < wave) ? s_w[w] : 0ull;
     __syncthreads();
-    *total = all;
     return before + incl - v;
 }
 
@@ -134,7 +137,7 @@ __global__ __launch_bounds__(BLOCK) void k_onesweep(const uint64_t* __restrict__
     __shared__ uint64_t s_keys[TILE];
     __shared__ uint32_t s_vals[VALS ? TILE : 1];
     __shared__ uint32_t s_tile;
-    __shared__ uint32_t s_w[WAVES];
+    __shared__ uint64_t s_w[WAVES];
 
     // Per-wave digit match masks for the ranking, in the key staging array
     // (unused until the reorder).
@@ -166,8 +169,10 @@ __global__ __launch_bounds__(BLOCK) void k_onesweep(const uint64_t* __restrict__
         key[it] = valid ? kin[idx] : ~0ull;
         if (VALS) val[it] = valid ? vin[idx] : 0u;
     }
-    // Global start of this thread's bins: exclusive scan of the pass's digit
-    // totals (their loads overlap the key loads above).
+    // The pass's digit totals of this thread's bins (their loads overlap the
+    // key loads above); their exclusive scan, the global start of the bins,
+    // is taken together with the tile's own counts after the ranking, so each
+    // wave starts ranking as soon as its own keys arrive.
     uint32_t tot[BPT];
     uint32_t cnt[BPT];
     uint32_t tot_sum = 0, tsum = 0;
@@ -178,8 +183,6 @@ __global__ __launch_bounds__(BLOCK) void k_onesweep(const uint64_t* __restrict__
         tot[j] = owner ? hist_tot[b] : 0u;
         tot_sum += tot[j];
     }
-    uint32_t all_keys;
-    uint32_t gstart = block_excl_sum<BLOCK>(tot_sum, s_w, &all_keys);
 
     // Stable ranking inside each wave (items consumed in input order) into
     // per-wave LDS histograms; the lanes holding one digit find each other
@@ -254,8 +257,9 @@ __global__ __launch_bounds__(BLOCK) void k_onesweep(const uint64_t* __restrict__
         __hip_atomic_store(status + (size_t)tile * BINS + b, (tile == 0 ? kFlagP : kFlagA) | run, __ATOMIC_RELAXED,
                            __HIP_MEMORY_SCOPE_AGENT);
     }
-    uint32_t total;
-    const uint32_t lrun0 = block_excl_sum<BLOCK>(tsum, s_w, &total);
+    const uint64_t starts = block_excl_sum2<BLOCK>(tot_sum, tsum, s_w);
+    uint32_t gstart = (uint32_t)(starts >> 32);
+    const uint32_t lrun0 = (uint32_t)starts;
     // The tile's local bin starts: all the reorder into LDS needs.
     {
         uint32_t lr = lrun0;

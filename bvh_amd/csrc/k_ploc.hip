@@ -99,6 +99,18 @@ __device__ __forceinline__ const SegDesc& seg_desc(const SegBlocks& sb, uint32_t
     return sb.blocks ? sb.segs[s] : sb.single;
 }
 
+// Union of two finite boxes with the hardware min/max, for areas only.
+__device__ __forceinline__ Box merge_hw(const Box& a, const Box& b) {
+    Box r;
+    r.mnx = fminf(a.mnx, b.mnx);
+    r.mny = fminf(a.mny, b.mny);
+    r.mnz = fminf(a.mnz, b.mnz);
+    r.mxx = fmaxf(a.mxx, b.mxx);
+    r.mxy = fmaxf(a.mxy, b.mxy);
+    r.mxz = fmaxf(a.mxz, b.mxz);
+    return r;
+}
+
 // Packed neighbour candidate; non-finite distances are never candidates.
 __device__ __forceinline__ uint64_t nn_key(float dist, uint32_t j) {
     uint32_t bits = __float_as_uint(dist);
@@ -218,21 +230,49 @@ __device__ __forceinline__ void nn_tile_symmetric(const PlocArgs& a, const float
         int64_t jhi = hi[0];
 #pragma unroll
         for (int c = 1; c < kNnPer; ++c) jhi = hi[c] > jhi ? hi[c] : jhi;
-        for (int64_t j = i0 + 1; j <= jhi; ++j) {
+        // Running minima as (distance bits, index): distances are never
+        // negative, so their bits order like the values; starting from the
+        // bits of +inf keeps every non-finite distance out, as the packed
+        // keys do; a strict < over ascending indices keeps the first minimum.
+        // Area-only unions use the hardware min/max: on finite boxes they
+        // differ from the selects at most in the sign of a zero extent, which
+        // changes no area's bits (an area of -0 would need all three products
+        // -0, i.e. an odd number of -0 extents in each of three pairs).
+        constexpr uint32_t kInfBits = 0x7f800000u;
+        uint32_t fd[kNnPer], fj[kNnPer];
+#pragma unroll
+        for (int c = 0; c < kNnPer; ++c) {
+            fd[c] = kInfBits;
+            fj[c] = 0;
+        }
+        auto visit = [&](int64_t j, bool all) {
             const Box bj = box_at(j);
-            uint64_t back = ~0ull;
+            uint32_t bd = kInfBits, bi_ = 0;
 #pragma unroll
             for (int c = 0; c < kNnPer; ++c) {
-                if (j > i0 + c && j <= hi[c]) {
-                    const float d = area(merge(bi[c], bj));
-                    const uint64_t kf = nn_key(d, (uint32_t)j);
-                    best[c] = kf < best[c] ? kf : best[c];
-                    const uint64_t kb = nn_key(d, (uint32_t)(i0 + c));
-                    back = kb < back ? kb : back;
+                if (all || (j > i0 + c && j <= hi[c])) {
+                    const Box u = merge_hw(bi[c], bj);
+                    const uint32_t d = __float_as_uint(area(u));
+                    if (d < fd[c]) {
+                        fd[c] = d;
+                        fj[c] = (uint32_t)j;
+                    }
+                    if (d < bd) {
+                        bd = d;
+                        bi_ = (uint32_t)(i0 + c);
+                    }
                 }
             }
-            if (back != ~0ull && j < tend) back_min(j, back);
-        }
+            if (bd < kInfBits && j < tend) back_min(j, ((uint64_t)bd << 32) | bi_);
+        };
+        // j = i0 + 1, i0 + 2: the windows start; then every window holds j up
+        // to hi[0] (hi is non-decreasing in c); past it the windows end.
+        const int64_t a_hi = (i0 + 2 < jhi) ? i0 + 2 : jhi;
+        for (int64_t j = i0 + 1; j <= a_hi; ++j) visit(j, false);
+        for (int64_t j = i0 + 3; j <= hi[0]; ++j) visit(j, true);
+        for (int64_t j = (hi[0] + 1 > i0 + 3 ? hi[0] + 1 : i0 + 3); j <= jhi; ++j) visit(j, false);
+#pragma unroll
+        for (int c = 0; c < kNnPer; ++c) best[c] = fd[c] < kInfBits ? (((uint64_t)fd[c] << 32) | fj[c]) : ~0ull;
     }
     // Pairs (i, j) with i in the r clusters before the tile and j in it:
     // i = t0 - u, j = t0 + v for 1 <= u <= r, 0 <= v <= r - u.

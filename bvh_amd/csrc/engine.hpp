@@ -240,7 +240,6 @@ struct PlocArgs {
     uint4* out;         // per mesh, written by the tail: count, next node id, iterations, flags
 };
 int ploc_small();
-int ploc_nn_block();
 int ploc_merge_tile();
 void launch_ploc_init(const PlocArgs& a, const uint64_t* keys, const KeyFormat& kf, const float4* tri_boxes,
                       uint32_t* err, hipStream_t s);

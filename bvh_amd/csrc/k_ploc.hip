@@ -52,8 +52,15 @@
 namespace bvhk {
 
 constexpr int kNnBlock = 256;
-constexpr int kNnPer = 3;                     // consecutive clusters per lane (odd: conflict-free LDS)
-constexpr int kNnTile = kNnBlock * kNnPer;
+// Consecutive clusters per lane (odd: conflict-free LDS): every box read
+// from LDS serves that many candidates and one LDS minimum per candidate j
+// covers them all, against more registers per lane.  Measured: three at
+// r = 25 (five: 3.77 against 3.73 ms per 4M build), five at r = 100 (32M
+// strands 27.1 against 29.3 ms).
+constexpr int kNnPer = 3;
+constexpr int kNnPerWide = 5;
+constexpr int kNnWideR = 64;                 // radius from which lanes take kNnPerWide clusters
+
 constexpr uint32_t kNnFill = 1u << 18;        // lanes that keep every CU busy
 constexpr int kNnSymMaxR = 128;               // symmetric search up to this radius (its halo pairs grow as r^2)
 constexpr int kMergeBlock = 256;
@@ -72,7 +79,6 @@ constexpr uint64_t kVMask = (1ull << 31) - 1;
 constexpr uint64_t kMMask = ((1ull << 30) - 1) << 31;
 
 int ploc_small() { return kPlocSmall; }
-int ploc_nn_block() { return kNnTile; }
 int ploc_merge_tile() { return kMergeBlock; }  // smallest merge tile (sizes the look-back words)
 
 // Merge tile of an iteration with m live clusters.
@@ -188,12 +194,12 @@ __device__ __forceinline__ bool ploc_global_step(const PlocState& st) {
 // j lies past the tile belong to the next tile's backward half.  The minimum
 // of (distance bits, index) over both halves is the reference's first minimum
 // of its ascending strict-< scan.
-template <bool SEG>
+template <bool SEG, int PER>
 __device__ __forceinline__ void nn_tile_symmetric(const PlocArgs& a, const float* s_nn, int span, int64_t t0,
                                                   uint32_t m, int r, uint32_t par) {
     uint64_t* s_back = reinterpret_cast<uint64_t*>(const_cast<float*>(s_nn) + 6 * span);
     const uint4* __restrict__ segs = a.seg[par];
-    const int64_t tend = t0 + kNnTile;  // first cluster past the tile
+    const int64_t tend = t0 + (kNnBlock * PER);  // first cluster past the tile
     auto box_at = [&](int64_t j) {
         const int q = (int)(j - t0) + r;
         Box b;
@@ -205,17 +211,17 @@ __device__ __forceinline__ void nn_tile_symmetric(const PlocArgs& a, const float
         __hip_atomic_fetch_min(reinterpret_cast<unsigned long long*>(s_back + (j - t0)), (unsigned long long)key,
                                __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
     };
-    const int64_t i0 = t0 + (int64_t)kNnPer * threadIdx.x;
-    uint64_t best[kNnPer];
+    const int64_t i0 = t0 + (int64_t)PER * threadIdx.x;
+    uint64_t best[PER];
 #pragma unroll
-    for (int c = 0; c < kNnPer; ++c) best[c] = ~0ull;
+    for (int c = 0; c < PER; ++c) best[c] = ~0ull;
     if (i0 < (int64_t)m) {
         // Forward windows (i, hi[c]] clipped to the mesh; hi is non-decreasing
         // in c, so one pass over (i0, max hi] serves all three.
-        Box bi[kNnPer];
-        int64_t hi[kNnPer];
+        Box bi[PER];
+        int64_t hi[PER];
 #pragma unroll
-        for (int c = 0; c < kNnPer; ++c) {
+        for (int c = 0; c < PER; ++c) {
             const int64_t i = i0 + c;
             if (i < (int64_t)m) {
                 const uint4 sg = segs[SEG ? a.sid[par][i] : 0u];
@@ -229,7 +235,7 @@ __device__ __forceinline__ void nn_tile_symmetric(const PlocArgs& a, const float
         }
         int64_t jhi = hi[0];
 #pragma unroll
-        for (int c = 1; c < kNnPer; ++c) jhi = hi[c] > jhi ? hi[c] : jhi;
+        for (int c = 1; c < PER; ++c) jhi = hi[c] > jhi ? hi[c] : jhi;
         // Running minima as (distance bits, index): distances are never
         // negative, so their bits order like the values; starting from the
         // bits of +inf keeps every non-finite distance out, as the packed
@@ -239,9 +245,9 @@ __device__ __forceinline__ void nn_tile_symmetric(const PlocArgs& a, const float
         // changes no area's bits (an area of -0 would need all three products
         // -0, i.e. an odd number of -0 extents in each of three pairs).
         constexpr uint32_t kInfBits = 0x7f800000u;
-        uint32_t fd[kNnPer], fj[kNnPer];
+        uint32_t fd[PER], fj[PER];
 #pragma unroll
-        for (int c = 0; c < kNnPer; ++c) {
+        for (int c = 0; c < PER; ++c) {
             fd[c] = kInfBits;
             fj[c] = 0;
         }
@@ -249,7 +255,7 @@ __device__ __forceinline__ void nn_tile_symmetric(const PlocArgs& a, const float
             const Box bj = box_at(j);
             uint32_t bd = kInfBits, bi_ = 0;
 #pragma unroll
-            for (int c = 0; c < kNnPer; ++c) {
+            for (int c = 0; c < PER; ++c) {
                 if (all || (j > i0 + c && j <= hi[c])) {
                     const Box u = merge_hw(bi[c], bj);
                     const uint32_t d = __float_as_uint(area(u));
@@ -265,14 +271,14 @@ __device__ __forceinline__ void nn_tile_symmetric(const PlocArgs& a, const float
             }
             if (bd < kInfBits && j < tend) back_min(j, ((uint64_t)bd << 32) | bi_);
         };
-        // j = i0 + 1, i0 + 2: the windows start; then every window holds j up
-        // to hi[0] (hi is non-decreasing in c); past it the windows end.
-        const int64_t a_hi = (i0 + 2 < jhi) ? i0 + 2 : jhi;
+        // j = i0 + 1 .. i0 + PER - 1: the windows start; then every window
+        // holds j up to hi[0] (hi is non-decreasing in c); past it they end.
+        const int64_t a_hi = (i0 + PER - 1 < jhi) ? i0 + PER - 1 : jhi;
         for (int64_t j = i0 + 1; j <= a_hi; ++j) visit(j, false);
-        for (int64_t j = i0 + 3; j <= hi[0]; ++j) visit(j, true);
-        for (int64_t j = (hi[0] + 1 > i0 + 3 ? hi[0] + 1 : i0 + 3); j <= jhi; ++j) visit(j, false);
+        for (int64_t j = i0 + PER; j <= hi[0]; ++j) visit(j, true);
+        for (int64_t j = (hi[0] + 1 > i0 + PER ? hi[0] + 1 : i0 + PER); j <= jhi; ++j) visit(j, false);
 #pragma unroll
-        for (int c = 0; c < kNnPer; ++c) best[c] = fd[c] < kInfBits ? (((uint64_t)fd[c] << 32) | fj[c]) : ~0ull;
+        for (int c = 0; c < PER; ++c) best[c] = fd[c] < kInfBits ? (((uint64_t)fd[c] << 32) | fj[c]) : ~0ull;
     }
     // Pairs (i, j) with i in the r clusters before the tile and j in it:
     // i = t0 - u, j = t0 + v for 1 <= u <= r, 0 <= v <= r - u.
@@ -287,7 +293,7 @@ __device__ __forceinline__ void nn_tile_symmetric(const PlocArgs& a, const float
     }
     __syncthreads();
 #pragma unroll
-    for (int c = 0; c < kNnPer; ++c) {
+    for (int c = 0; c < PER; ++c) {
         const int64_t i = i0 + c;
         if (i < (int64_t)m) {
             const uint64_t b = s_back[i - t0];
@@ -297,15 +303,15 @@ __device__ __forceinline__ void nn_tile_symmetric(const PlocArgs& a, const float
     }
 }
 
-template <bool SEG>
+template <bool SEG, int PER>
 __global__ __launch_bounds__(kNnBlock) void k_ploc_nn(PlocArgs a, int k) {
-    extern __shared__ float s_nn[];  // 6 SoA arrays of (kNnTile + 2r) floats
+    extern __shared__ float s_nn[];  // 6 SoA arrays of ((kNnBlock * PER) + 2r) floats
     const PlocState st = a.state[k];
     if (!ploc_global_step(st)) return;
     const uint32_t m = st.m;
     const uint32_t par = st.parity;
     const int G = nn_lanes(m);
-    const uint32_t tc = G == 0 ? (uint32_t)kNnTile : (uint32_t)(kNnBlock / G);  // clusters per tile
+    const uint32_t tc = G == 0 ? (uint32_t)(kNnBlock * PER) : (uint32_t)(kNnBlock / G);  // clusters per tile
     const uint32_t tiles = (m + tc - 1) / tc;
     const uint32_t mtiles = (m + merge_tile(m) - 1) / merge_tile(m);
     // Reset the merge kernel's look-back words, tile counter and next state.
@@ -330,7 +336,7 @@ __global__ __launch_bounds__(kNnBlock) void k_ploc_nn(PlocArgs a, int k) {
         const int64_t t0 = (int64_t)tile * tc;
         if (G == 0 && r <= kNnSymMaxR) {
             uint64_t* s_back = reinterpret_cast<uint64_t*>(s_nn + 6 * span);
-            for (int q = threadIdx.x; q < kNnTile; q += kNnBlock) s_back[q] = ~0ull;
+            for (int q = threadIdx.x; q < (kNnBlock * PER); q += kNnBlock) s_back[q] = ~0ull;
         }
         for (int q = threadIdx.x; q < span; q += kNnBlock) {
             int64_t p = t0 - r + q;
@@ -378,20 +384,20 @@ __global__ __launch_bounds__(kNnBlock) void k_ploc_nn(PlocArgs a, int k) {
             continue;
         }
         if (r <= kNnSymMaxR) {
-            nn_tile_symmetric<SEG>(a, s_nn, span, t0, m, r, par);
+            nn_tile_symmetric<SEG, PER>(a, s_nn, span, t0, m, r, par);
             continue;
         }
         // This lane's clusters i0, i0+1, i0+2 and their windows, clipped to
         // their meshes.  Window bounds are non-decreasing in i (meshes are in
-        // order), so one pass over [lo[0], hi[kNnPer-1]] covers all three and
+        // order), so one pass over [lo[0], hi[PER-1]] covers all three and
         // every box read from LDS serves up to three candidates.
-        const int64_t i0 = t0 + (int64_t)kNnPer * threadIdx.x;
+        const int64_t i0 = t0 + (int64_t)PER * threadIdx.x;
         if (i0 >= (int64_t)m) continue;
-        Box bi[kNnPer];
-        int64_t lo[kNnPer], hi[kNnPer];
-        uint64_t best[kNnPer];
+        Box bi[PER];
+        int64_t lo[PER], hi[PER];
+        uint64_t best[PER];
 #pragma unroll
-        for (int c = 0; c < kNnPer; ++c) {
+        for (int c = 0; c < PER; ++c) {
             const int64_t i = i0 + c;
             best[c] = ~0ull;
             if (i < (int64_t)m) {
@@ -412,14 +418,14 @@ __global__ __launch_bounds__(kNnBlock) void k_ploc_nn(PlocArgs a, int k) {
         const int64_t jlo = lo[0];
         int64_t jhi = hi[0];
 #pragma unroll
-        for (int c = 1; c < kNnPer; ++c) jhi = hi[c] > jhi ? hi[c] : jhi;
+        for (int c = 1; c < PER; ++c) jhi = hi[c] > jhi ? hi[c] : jhi;
         for (int64_t j = jlo; j <= jhi; ++j) {
             const int q = (int)(j - t0) + r;
             Box bj;
             bj.mnx = s_nn[0 * span + q]; bj.mny = s_nn[1 * span + q]; bj.mnz = s_nn[2 * span + q];
             bj.mxx = s_nn[3 * span + q]; bj.mxy = s_nn[4 * span + q]; bj.mxz = s_nn[5 * span + q];
 #pragma unroll
-            for (int c = 0; c < kNnPer; ++c) {
+            for (int c = 0; c < PER; ++c) {
                 if (j >= lo[c] && j <= hi[c] && j != i0 + c) {
                     const uint64_t key = nn_key(area(merge(bi[c], bj)), (uint32_t)j);
                     best[c] = key < best[c] ? key : best[c];
@@ -427,7 +433,7 @@ __global__ __launch_bounds__(kNnBlock) void k_ploc_nn(PlocArgs a, int k) {
             }
         }
 #pragma unroll
-        for (int c = 0; c < kNnPer; ++c) {
+        for (int c = 0; c < PER; ++c) {
             const int64_t i = i0 + c;
             if (i < (int64_t)m) a.nn[i] = (best[c] == ~0ull) ? -1 : (int32_t)(uint32_t)best[c];
         }
@@ -924,9 +930,16 @@ void launch_ploc_nn(const PlocArgs& a, int k, uint32_t m_bound, uint32_t max_gri
     g = g < 1 ? 1 : (g > kNnGridMax ? kNnGridMax : g);
     // Six SoA float arrays of the staged span, then the tile's backward
     // minima (symmetric search; 6 * span floats keep them 8-byte aligned).
-    size_t lds = (size_t)6 * (kNnTile + 2 * a.radius) * sizeof(float) + (size_t)kNnTile * sizeof(uint64_t);
-    if (a.sid[0]) hipLaunchKernelGGL(k_ploc_nn<true>, dim3(g), dim3(kNnBlock), lds, s, a, k);
-    else hipLaunchKernelGGL(k_ploc_nn<false>, dim3(g), dim3(kNnBlock), lds, s, a, k);
+    const int per = a.radius >= kNnWideR ? kNnPerWide : kNnPer;
+    const size_t tile = (size_t)kNnBlock * per;
+    size_t lds = (size_t)6 * (tile + 2 * a.radius) * sizeof(float) + tile * sizeof(uint64_t);
+    if (per == kNnPerWide) {
+        if (a.sid[0]) hipLaunchKernelGGL((k_ploc_nn<true, kNnPerWide>), dim3(g), dim3(kNnBlock), lds, s, a, k);
+        else hipLaunchKernelGGL((k_ploc_nn<false, kNnPerWide>), dim3(g), dim3(kNnBlock), lds, s, a, k);
+    } else {
+        if (a.sid[0]) hipLaunchKernelGGL((k_ploc_nn<true, kNnPer>), dim3(g), dim3(kNnBlock), lds, s, a, k);
+        else hipLaunchKernelGGL((k_ploc_nn<false, kNnPer>), dim3(g), dim3(kNnBlock), lds, s, a, k);
+    }
 }
 
 void launch_ploc_merge(const PlocArgs& a, int k, uint32_t m_bound, uint32_t max_grid, hipStream_t s) {

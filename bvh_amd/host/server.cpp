@@ -159,69 +159,81 @@ void bvh_server_rec::set_held(int slot, uint64_t build, uint64_t trace) {
 // it in *ev (the caller frees it outside the lock, then calls again).
 bool bvh_server_rec::form_group(int slot, std::vector<Req>& group, Evicted* ev) {
     while (!queue.empty()) {
-        const uint32_t look = std::min<uint32_t>((uint32_t)queue.size(), (uint32_t)cfg.max_group);
-        std::vector<uint64_t> bytes(look);
-        std::vector<uint32_t> tris(look);
-        std::vector<uint8_t> run(look);
-        std::vector<Resident*> res(look, nullptr);
-        std::vector<uint8_t> builds(look);
-        std::unordered_map<uint64_t, uint32_t> new_keys;  // keys first built inside this prefix
+        {  // a head that names no mesh (no vertices, key neither resident nor being built) is rejected
+            const bvh_request& r = queue.front().r;
+            if (r.soa[0] == nullptr && !(r.mesh_key && residents.count(r.mesh_key))) {
+                Req q = queue.front();
+                queue.pop_front();
+                q.admitted = now();
+                err = "request names a mesh_key that is not resident";
+                ++rejected;
+                complete_locked(q, BVH_E_STATE, -1, 0);
+                continue;
+            }
+        }
+        // Candidates, in queue order: the requests that can run now.  A
+        // follow-up trace whose mesh is still being built, a second build of
+        // a key first built earlier in the window, and a request that names
+        // no mesh stay queued (the first two run once the mesh is resident;
+        // the last is rejected when it reaches the head), so none of them
+        // holds up the requests behind it.
+        const uint32_t look = std::min<uint32_t>((uint32_t)queue.size(), 4u * (uint32_t)cfg.max_group);
+        std::vector<uint32_t> idx;
+        std::vector<uint64_t> bytes;
+        std::vector<uint32_t> tris;
+        std::vector<Resident*> res;
+        std::vector<uint8_t> builds;
+        std::unordered_map<uint64_t, uint32_t> new_keys;  // keys first built inside the window
         for (uint32_t i = 0; i < look; ++i) {
             const bvh_request& r = queue[i].r;
-            const bool has_soa = r.soa[0] != nullptr;
             Resident* rp = nullptr;
             if (r.mesh_key) {
                 auto it = residents.find(r.mesh_key);
                 if (it != residents.end()) rp = it->second.get();
             }
-            res[i] = rp;
-            if (rp) {  // trace against a resident (or one being built)
-                run[i] = rp->ready ? 1 : 0;
-                builds[i] = 0;
-                bytes[i] = 0;
-                tris[i] = 0;
-            } else if (has_soa) {
-                const bool dup = r.mesh_key && new_keys.count(r.mesh_key);
-                run[i] = dup ? 0 : 1;  // a second request for a key built in this prefix waits for it
-                builds[i] = 1;
-                bytes[i] = persistent_bytes(cfg.algo, r.ntri);
-                tris[i] = r.ntri;
+            if (rp) {  // trace against a resident, once it is ready
+                if (!rp->ready) continue;
+                idx.push_back(i);
+                bytes.push_back(0);
+                tris.push_back(0);
+                res.push_back(rp);
+                builds.push_back(0);
+            } else if (r.soa[0] != nullptr) {
+                if (r.mesh_key && new_keys.count(r.mesh_key)) continue;  // waits for the first build of its key
                 if (r.mesh_key) new_keys[r.mesh_key] = i;
-            } else {
-                run[i] = 2;  // names no mesh: rejected below if at the head
-                builds[i] = 0;
-                bytes[i] = 0;
-                tris[i] = 0;
+                idx.push_back(i);
+                bytes.push_back(persistent_bytes(cfg.algo, r.ntri));
+                tris.push_back(r.ntri);
+                res.push_back(nullptr);
+                builds.push_back(1);
             }
         }
-        if (run[0] == 2) {
-            Req q = queue.front();
-            queue.pop_front();
-            q.admitted = now();
-            err = "request names a mesh_key that is not resident";
-            ++rejected;
-            complete_locked(q, BVH_E_STATE, -1, 0);
-            continue;
-        }
-        for (auto& v : run)
-            if (v == 2) v = 0;
+        const uint32_t ncand = (uint32_t)idx.size();
+        if (ncand == 0) return false;  // everything queued waits for a mesh being built
+        const std::vector<uint8_t> run(ncand, 1);
         const uint64_t used = in_use + held_total;
         const uint64_t free_bytes = cfg.memory_budget > used ? cfg.memory_budget - used : 0;
-        uint32_t g = bvh_serve_form_group(bytes.data(), tris.data(), run.data(), look, free_bytes,
-                                          (uint32_t)cfg.max_group, cfg.max_group_tris);
-        // The prefix's meshes must fit together with what this slot's context
+        // A slot answers its group's requests one after another, so it takes
+        // at most its share of the queue (all slots' worth when the queue is
+        // deep): the rest stays for the next slot that frees up.
+        const uint32_t slots = (uint32_t)std::max(1, cfg.slots);
+        const uint32_t share = ((uint32_t)queue.size() + slots - 1) / slots;
+        const uint32_t cap = std::min<uint32_t>((uint32_t)cfg.max_group, std::max<uint32_t>(1, share));
+        uint32_t g = bvh_serve_form_group(bytes.data(), tris.data(), run.data(), ncand, free_bytes, cap,
+                                          cfg.max_group_tris);
+        // The group's meshes must fit together with what this slot's context
         // workspace has to grow by for the group's build and traces.
         uint64_t need_build = 0, need_trace = 0;
         while (g > 0) {
             uint64_t persist = 0, t = 0;
             uint32_t nb = 0, max_rays = 0;
-            for (uint32_t i = 0; i < g; ++i) {
-                persist += bytes[i];
-                if (builds[i]) {
-                    t += tris[i];
+            for (uint32_t k = 0; k < g; ++k) {
+                persist += bytes[k];
+                if (builds[k]) {
+                    t += tris[k];
                     ++nb;
                 }
-                max_rays = std::max(max_rays, queue[i].r.nrays);
+                max_rays = std::max(max_rays, queue[idx[k]].r.nrays);
             }
             group_workspace(cfg.algo, t, nb, max_rays, &need_build, &need_trace);
             const uint64_t grow = (need_build > held_build[slot] ? need_build - held_build[slot] : 0) +
@@ -229,7 +241,7 @@ bool bvh_server_rec::form_group(int slot, std::vector<Req>& group, Evicted* ev) 
             if (persist + grow <= free_bytes) break;
             --g;
         }
-        if (g == 0 && run[0]) {
+        if (g == 0) {
             // Memory: take the least recently used idle resident out of service.
             Resident* victim = nullptr;
             for (auto& kv : residents) {
@@ -244,25 +256,24 @@ bool bvh_server_rec::form_group(int slot, std::vector<Req>& group, Evicted* ev) 
                 residents.erase(victim->key);
                 return false;
             }
-            if (in_flight == 0) {
-                Req q = queue.front();
-                queue.pop_front();
+            if (in_flight == 0) {  // the first candidate can never fit
+                Req q = queue[idx[0]];
+                queue.erase(queue.begin() + idx[0]);
                 q.admitted = now();
                 err = "request needs more device memory than the budget";
                 ++rejected;
                 complete_locked(q, BVH_E_BUDGET, -1, 0);
                 continue;
             }
+            return false;
         }
-        if (g == 0) return false;
         const double t = now();
-        for (uint32_t i = 0; i < g; ++i) {
-            Req q = queue.front();
-            queue.pop_front();
+        for (uint32_t k = 0; k < g; ++k) {
+            Req q = queue[idx[k]];
             q.admitted = t;
-            q.build = builds[i] != 0;
+            q.build = builds[k] != 0;
             if (q.build) {
-                const uint64_t pb = bytes[i];
+                const uint64_t pb = bytes[k];
                 if (q.r.mesh_key) {
                     auto r = std::make_unique<Resident>();
                     r->key = q.r.mesh_key;
@@ -279,13 +290,14 @@ bool bvh_server_rec::form_group(int slot, std::vector<Req>& group, Evicted* ev) 
                 }
                 in_use += pb;
             } else {
-                q.res = res[i];
+                q.res = res[k];
                 q.res->users++;
                 q.res->last_use = ++lru;
                 q.transient = 0;
             }
             group.push_back(q);
         }
+        for (uint32_t k = g; k-- > 0;) queue.erase(queue.begin() + idx[k]);  // highest index first
         // Reserve the workspace growth until the group reports what it holds.
         set_held(slot, std::max(held_build[slot], need_build), std::max(held_trace[slot], need_trace));
         in_flight += g;

@@ -157,7 +157,16 @@ struct LaneStack {
         return true;
     }
     __device__ void get(int sp, int32_t& i, float& t) const {
-        const int2 v = sp < LDS ? e[sp][lane] : slot[sp - LDS];
+        // Two loads of different types, so the compiler keeps an LDS read
+        // and a global read on their own paths instead of merging them into
+        // one generic (flat) load through a selected address.
+        int2 v;
+        if (sp < LDS) {
+            v = e[sp][lane];
+        } else {
+            const uint64_t w = *reinterpret_cast<const uint64_t*>(slot + (sp - LDS));
+            v = make_int2((int32_t)(uint32_t)w, (int32_t)(uint32_t)(w >> 32));
+        }
         i = v.x;
         t = __int_as_float(v.y);
     }

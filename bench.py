@@ -144,79 +144,38 @@ def extras(ctx, dm, tree, mesh, steps):
     for name, kw in (("trace_lbvh", {"algo": B.ALGO_LBVH}), ("trace_ploc_r25", {"algo": B.ALGO_PLOC, "radius": 25})):
         log(f"[bench] {name}")
         ctx.build(dm, tree, **kw)
-        t, prim, cnt, _ = ctx.trace(tree, dm, rays, counters=True)
-        ms = []
-        for _ in range(5):
-            ms.append(ctx.trace(tree, dm, rays)[2])
-        med = float(np.median(ms))
-        per_ray = 24 + 8 + 48 * float(cnt[:, 0].mean()) + 36 * float(cnt[:, 2].mean())
-        out[name] = {"rays": int(rays.shape[0]), "device_ms": round(med, 4),
-                     "Mrays_s": round(rays.shape[0] / med / 1e3, 1), "hit_fraction": round(float((prim >= 0).mean()), 4),
-                     "avg_nodes_visited": round(float(cnt[:, 0].mean()), 2),
-                     "avg_tri_tests": round(float(cnt[:, 2].mean()), 2),
-                     "roofline": roof(per_ray * rays.shape[0], med)}
+        out[name], ref = trace_stats(ctx, tree, dm, rays, 5)
+        log(f"[bench] {name} (nearest-first order)")
+        ctx.set_trace_order(B.TRACE_NEAREST)
+        try:
+            out[name + "_nearest"], near = trace_stats(ctx, tree, dm, rays, 5)
+        finally:
+            ctx.set_trace_order(B.TRACE_REFERENCE)
+        out[name + "_nearest"].update(same_hits(ref, near))
     return out
 
 
-def batched_vs_plain(ctx, steps, k=512, lo=1000, hi=200_000, seed=11):
-    """512 uneven meshes (log-uniform 1,000-200,000 triangles, seed 11, each a
-    uniform-cube mesh with its own seed, scale and offset): one captured
-    launch sequence per algorithm against 512 plain per-mesh builds, meshes/s
-    of each, and per-mesh bit-identity of the trees."""
-    import bvh_amd as B
-    counts = B.batch_counts(seed, k, lo, hi)
-    log(f"[bench] batched: generating {k} meshes, {int(counts.sum()):,} triangles")
-    meshes = [B.Mesh.batch_mesh(seed, i, int(c)) for i, c in enumerate(counts)]
-    dms = [ctx.upload(m) for m in meshes]
-    del meshes
-    out = {"meshes": k, "triangles": int(counts.sum()), "seed": seed}
-    for name, algo, radius in (("lbvh", B.ALGO_LBVH, 25), ("ploc_r16", B.ALGO_PLOC, 16)):
-        log(f"[bench] batched {name}: plain per-mesh builds")
-        plain_trees = [ctx.tree() for _ in dms]
-        for dm, t in zip(dms, plain_trees):  # warm (allocations)
-            ctx.build(dm, t, algo=algo, radius=radius)
-        reps = max(1, min(3, steps // 5))
-        t0 = time.perf_counter()
-        for _ in range(reps):
-            for dm, t in zip(dms, plain_trees):
-                ctx.build(dm, t, algo=algo, radius=radius)
-        plain_s = (time.perf_counter() - t0) / reps
-        log(f"[bench] batched {name}: capture + tune")
-        trees = [ctx.tree() for _ in dms]
-        batch = ctx.batch(dms, trees, algo=algo, radius=radius, tune=True)
-        batch.run()
-        dev = []
-        t0 = time.perf_counter()
-        for _ in range(steps):
-            dev.append(batch.run())
-        batch_s = (time.perf_counter() - t0) / steps
-        info = batch.info()
-        log(f"[bench] batched {name}: bit-identity check")
-        identical = True
-        for a, b in zip(trees, plain_trees):
-            na, ia = a.download()
-            nb, ib = b.download()
-            if na.tobytes() != nb.tobytes() or not np.array_equal(ia, ib):
-                identical = False
-                break
-        batch.free()
-        for t in trees + plain_trees:
-            t.free()
-        out[name] = {
-            "plain_ms": round(plain_s * 1e3, 2), "plain_meshes_s": round(k / plain_s, 1),
-            "batched_ms": round(batch_s * 1e3, 3), "batched_device_ms": round(float(np.median(dev)), 3),
-            "batched_meshes_s": round(k / batch_s, 1), "speedup": round(plain_s / batch_s, 2),
-            "batched_Mtri_s": round(int(counts.sum()) / batch_s / 1e6, 1),
-            "bit_identical_all_meshes": identical,
-            "graph_nodes": info["graph_nodes"], "radix_bits": info["radix_bits"], "sort_passes": info["sort_passes"],
-            "key_bits": info["key_bits"], "tune": info["tune"],
-        }
-        if algo == B.ALGO_PLOC:
-            out[name]["global_iterations"] = info["ploc_iterations"]
-            out[name]["nn_grid"] = info["ploc_nn_grid"]
-    for d in dms:
-        d.free()
-    return out
+def trace_stats(ctx, tree, dm, rays, reps):
+    """One counted trace, then the median device time of `reps` traces; the
+    algorithmic bytes per ray are 24 (ray) + 8 (hit) + 48 per node visited +
+    36 per triangle tested."""
+    t, prim, cnt, _ = ctx.trace(tree, dm, rays, counters=True)
+    ms = float(np.median([ctx.trace(tree, dm, rays)[2] for _ in range(reps)]))
+    per_ray = 24 + 8 + 48 * float(cnt[:, 0].mean()) + 36 * float(cnt[:, 2].mean())
+    return ({"rays": int(rays.shape[0]), "device_ms": round(ms, 4),
+             "Mrays_s": round(rays.shape[0] / ms / 1e3, 1), "hit_fraction": round(float((prim >= 0).mean()), 4),
+             "avg_nodes_visited": round(float(cnt[:, 0].mean()), 2),
+             "avg_tri_tests": round(float(cnt[:, 2].mean()), 2),
+             "roofline": roof(per_ray * rays.shape[0], ms)}, (t, prim))
+
+
+def same_hits(ref, near):
+    """How the nearest-first hits compare with the reference order's."""
+    (rt, rp), (nt, np_) = ref, near
+    hit = rp >= 0
+    rel = np.abs(nt[hit].astype(np.float64) - rt[hit]) / np.maximum(np.abs(rt[hit].astype(np.float64)), 1e-30)
+    return {"rays_with_other_hit_than_reference_order": int((rp != np_).sum()),
+            "max_rel_t_difference": float(rel.max()) if rel.size else 0.0}
 
 
 def large_strands(ctx, steps=2):
@@ -246,13 +205,19 @@ def large_strands(ctx, steps=2):
             loop = st.get("PLOC Global Iterations", 0.0) + st.get("PLOC LDS Tail", 0.0)
             r["iterations_roofline"] = roof(ploc_bytes("ploc_iterations_32m_strands_r100.json", mesh.n, 100), loop)
         log(f"[bench] large: trace through {name}")
-        t, prim, ms = ctx.trace(tree, dm, rays)
-        cnt = ctx.trace(tree, dm, rays, counters=True)[2]
-        per_ray = 24 + 8 + 48 * float(cnt[:, 0].mean()) + 36 * float(cnt[:, 2].mean())
-        r["trace_device_ms"] = round(ms, 2)
-        r["trace_Mrays_s"] = round(rays.shape[0] / ms / 1e3, 2)
-        r["hit_fraction"] = round(float((prim >= 0).mean()), 4)
-        r["trace_roofline"] = roof(per_ray * rays.shape[0], ms)
+        st_ref, ref = trace_stats(ctx, tree, dm, rays, 1)
+        r["trace_device_ms"] = round(st_ref["device_ms"], 2)
+        r["trace_Mrays_s"] = st_ref["Mrays_s"]
+        r["hit_fraction"] = st_ref["hit_fraction"]
+        r["trace_roofline"] = st_ref["roofline"]
+        log(f"[bench] large: trace through {name} (nearest-first order)")
+        ctx.set_trace_order(B.TRACE_NEAREST)
+        try:
+            st_near, near = trace_stats(ctx, tree, dm, rays, 1)
+        finally:
+            ctx.set_trace_order(B.TRACE_REFERENCE)
+        st_near.update(same_hits(ref, near))
+        r["trace_nearest"] = st_near
         out[name] = r
     tree.free()
     dm.free()

@@ -13,10 +13,11 @@ import numpy as np
 
 from . import _native as N
 from ._native import NODE_DTYPE, NativeError, ALGO_LBVH, ALGO_LBVH_PLUS, ALGO_PLOC  # noqa: F401
+from ._native import TRACE_REFERENCE, TRACE_NEAREST  # noqa: F401
 
 __all__ = [
     "Mesh", "Context", "DeviceMesh", "DeviceBuffer", "Tree", "LBVHBuilder", "LBVHPlusBuilder", "PLOCBuilder", "NODE_DTYPE",
-    "NativeError", "rays_sphere_camera", "rays_sphere_to_box", "rays_uniform_cube", "rays_camera", "Batch",
+    "NativeError", "TRACE_REFERENCE", "TRACE_NEAREST", "rays_sphere_camera", "rays_sphere_to_box", "rays_uniform_cube", "rays_camera", "Batch",
 ]
 
 
@@ -262,6 +263,12 @@ class Context:
         n = rays.nbytes // 24
         self._check(self._lib.bvh_trace_async(self.h, tree.h, dmesh.h, rays.ptr, n, t.ptr, prim.ptr,
                                               counters.ptr if counters is not None else None), "trace_async")
+
+    def set_trace_order(self, order: int):
+        """Visit order of this context's traces (bvh_ctx_set_trace_order):
+        TRACE_REFERENCE (default: the reference renderer's order, ids, t and
+        counters) or TRACE_NEAREST (nearer child first, least (t, index) hit)."""
+        self._check(self._lib.bvh_ctx_set_trace_order(self.h, int(order)), "set_trace_order")
 
     def sync(self):
         """Waits for the context's queued work; raises the first failure of the

@@ -20,6 +20,7 @@ STATUS = {
     5: "no progress", 6: "over memory budget", 7: "no HIP device", 8: "invalid state", 9: "unsupported",
 }
 ALGO_LBVH, ALGO_LBVH_PLUS, ALGO_PLOC = 0, 1, 2
+TRACE_REFERENCE, TRACE_NEAREST = 0, 1
 NOALLOC_MAX_TRIANGLES = 262144
 BUILD_KEEP_KEYS = 1
 BUILD_TUNE = 2
@@ -129,6 +130,7 @@ _HIP_SIGS = {
     "bvh_batch_get_info": (_i32, [_vp, C.POINTER(BatchInfo)]),
     "bvh_batch_free": (_i32, [_vp]),
     "bvh_tree_prepare": (_i32, [_vp, _vp]),
+    "bvh_ctx_set_trace_order": (_i32, [_vp, _i32]),
     "bvh_trace": (_i32, [_vp, _vp, _vp, _vp, _u32, _vp, _vp, _vp, C.POINTER(_f)]),
     "bvh_trace_host": (_i32, [_vp, _vp, _vp, _vp, _u32, _vp, _vp, _vp, C.POINTER(_f)]),
     "bvh_build_async": (_i32, [_vp, _vp, C.POINTER(BuildParams), _vp]),

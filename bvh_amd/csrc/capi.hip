@@ -86,6 +86,7 @@ struct bvh_context {
     DevBuf trace_ovf, trace_list, trace_io;  // traversal queue and host-call staging
     DevBuf trace_spill;                      // spill slots of deep traversal stacks
     DevBuf ray_keys[2];                      // ray coherence keys (sorted before tracing)
+    int32_t trace_order = BVH_TRACE_REFERENCE;  // visit order of this context's traces
     uint32_t* host_flags = nullptr;          // pinned: a single build's 8 flag words land here
     std::vector<uint32_t*> pin_chunks;       // pinned status slots of asynchronous calls
     std::vector<Pending> pending;            // asynchronous calls since the last bvh_sync, in order
@@ -1120,6 +1121,7 @@ int trace_enqueue(bvh_ctx_t ctx, bvh_tree_t tree, bvh_mesh_t mesh, const float* 
     a.overflow_list = P<uint32_t>(ctx->trace_list);
     a.spill = P<int2>(ctx->trace_spill);
     a.spill_slots = trace_spill_slots(nrays);
+    a.nearest = ctx->trace_order == BVH_TRACE_NEAREST ? 1 : 0;
     launch_fill_zero(a.overflow_count, 4, ctx->stream);
     launch_trace(a, ctx->stream);
     launch_trace_deep(a, ctx->stream);
@@ -1789,6 +1791,14 @@ int bvh_batch_free(bvh_batch_t batch) {
     batch->ws.each([](DevBuf& b) { free_buf(b); });
     batch->tabs.release();
     delete batch;
+    return BVH_OK;
+}
+
+int bvh_ctx_set_trace_order(bvh_ctx_t ctx, int32_t order) {
+    if (!ctx) return BVH_E_ARG;
+    if (order != BVH_TRACE_REFERENCE && order != BVH_TRACE_NEAREST)
+        return set_err(ctx, BVH_E_ARG, "unknown trace order");
+    ctx->trace_order = order;
     return BVH_OK;
 }
 

@@ -178,6 +178,7 @@ struct TraceArgs {
     uint32_t* overflow_list;    // nrays entries
     int2* spill;                // spill_slots x trace_spill_entries() stack entries
     uint32_t spill_slots;
+    int32_t nearest;            // 0: the reference's visit order; 1: nearer child first, least (t, index) hit
 };
 // Stack entries of one spill slot (the part of a lane's stack beyond LDS).
 int trace_spill_entries();

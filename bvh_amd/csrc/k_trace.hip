@@ -207,16 +207,26 @@ __device__ __forceinline__ float slab_fast(const RayState& r, const float4 mn, c
 
 // One ray's traversal, advanced one stack pop at a time.
 //
-// The reference's depth-first order: children pushed right then left, so
-// the left one is popped next; a popped entry is skipped unless its entry
-// distance is below the current closest t.  The left child would be popped
-// right after its push with the same closest t, so it is kept in registers
-// (`cur`) and only right children go through the stack: the same decisions
-// in the same order.  Children are counted when their parent is expanded:
-// the reference pops (and counts) every child it pushes.  (A "while-while"
-// split into internal and leaf phases was measured 2.6x slower: without
-// postponing leaves, which would change the culls, each phase waits for the
-// wave's slowest lane.)
+// Reference order (NEAREST = false), the reference's depth-first order:
+// children pushed right then left, so the left one is popped next; a popped
+// entry is skipped unless its entry distance is below the current closest t;
+// a hit replaces the closest one when its t is strictly smaller.  The child
+// popped next is kept in registers (`cur`) and only the other one goes
+// through the stack: the same decisions in the same order.  Children are
+// counted when their parent is expanded: the reference pops (and counts)
+// every child it pushes.  A child whose entry distance is already not below
+// the closest t is not pushed: the closest t only falls, so its pop would be
+// skipped anyway.  (A "while-while" split into internal and leaf phases was
+// measured 2.6x slower: without postponing leaves, which would change the
+// culls, each phase waits for the wave's slowest lane.)
+//
+// Nearest order (NEAREST = true): of two children that pass, the one with
+// the smaller entry distance is taken first (the left one on a tie); an
+// entry is skipped when its entry distance exceeds the closest t; the
+// closest hit is the least (t, triangle index) pair, so it does not depend
+// on the order triangles are met in.  Counted as the reference order
+// counts.
+template <bool NEAREST>
 struct Walker {
     RayState r;
     float best;
@@ -226,6 +236,8 @@ struct Walker {
     int32_t cur;
     float cur_te;
     int64_t budget;  // a tree pops each node at most once
+
+    __device__ __forceinline__ bool keep(float te) const { return NEAREST ? te <= best : te < best; }
 
     __device__ void start(const TraceArgs& a, const RayState& ray) {
         r = ray;
@@ -263,12 +275,13 @@ struct Walker {
             --sp;
             st.get(sp, ref, te);
         }
-        if (!(te < best)) return true;
+        if (!keep(te)) return true;
         if (ref < 0) {
             ++tris;
             float t;
             const uint32_t p = (uint32_t)ref & 0x7FFFFFFFu;
-            if (p < a.num_prims && hit_tri(r, a.tri, p, t) && t < best) {
+            if (p < a.num_prims && hit_tri(r, a.tri, p, t) &&
+                (t < best || (NEAREST && t == best && (int32_t)p < prim))) {
                 best = t;
                 prim = (int32_t)p;
             }
@@ -282,25 +295,29 @@ struct Walker {
         bool hl, hr;
         const float tl = slab_fast(r, A, B, hl);
         const float tr = slab_fast(r, C, D, hr);
-        hl = hl && lp;
-        hr = hr && rp;
         visited += (lp ? 1 : 0) + (rp ? 1 : 0);
-        if (hr) {
-            if (sp >= Stack::kCap || !st.put(sp, rref, tr)) return false;
+        hl = hl && lp && keep(tl);
+        hr = hr && rp && keep(tr);
+        if (hl && hr) {
+            const bool sw = NEAREST && tr < tl;  // right child first
+            if (sp >= Stack::kCap || !st.put(sp, sw ? lref : rref, sw ? tl : tr)) return false;
             ++sp;
+            cur = sw ? rref : lref;
+            cur_te = sw ? tr : tl;
+        } else {
+            cur = hl ? lref : rref;
+            cur_te = hl ? tl : tr;
         }
-        cur = lref;
-        cur_te = tl;
-        have = hl;
+        have = hl || hr;
         return true;
     }
 };
 
 // Whole traversal of one ray; false when the stack capacity was exceeded.
-template <class Stack>
+template <bool NEAREST, class Stack>
 __device__ __forceinline__ bool traverse(const TraceArgs& a, const RayState& r, Stack& st, float& best,
                                          int32_t& prim, int32_t cnt[3]) {
-    Walker w;
+    Walker<NEAREST> w;
     w.start(a, r);
     while (!w.done())
         if (!w.step(a, st)) return false;
@@ -335,7 +352,7 @@ __device__ __forceinline__ uint32_t ray_at(const TraceArgs& a, uint32_t k) {
 // (a wave then traces a sliding window of rays) and splitting the rays into
 // one contiguous part per XCD (each XCD's L2 then serves one region); both
 // were slower.
-template <int LDS>
+template <int LDS, bool NEAREST>
 __global__ __launch_bounds__(kTraceBlock) void k_trace(TraceArgs a) {
     __shared__ int2 s_stack[LDS][kTraceBlock];
     LaneStack<LDS> st;
@@ -354,7 +371,7 @@ __global__ __launch_bounds__(kTraceBlock) void k_trace(TraceArgs a) {
             float t;
             int32_t p;
             int32_t c[3] = {0, 0, 0};
-            if (traverse(a, r, st, t, p, c)) {
+            if (traverse<NEAREST>(a, r, st, t, p, c)) {
                 write_result(a, i, t, p, c);
             } else {
                 const uint32_t q = atomicAdd(a.overflow_count, 1u);
@@ -366,6 +383,7 @@ __global__ __launch_bounds__(kTraceBlock) void k_trace(TraceArgs a) {
 
 // Rays whose stack outgrew LDS and the spill slot; grid-stride over
 // the queue (its length is read on the device, so no host round trip).
+template <bool NEAREST>
 __global__ __launch_bounds__(kTraceBlock) void k_trace_deep(TraceArgs a) {
     __shared__ int2 s_stack[kDeepLds][kTraceBlock];
     const uint32_t count = *a.overflow_count;
@@ -378,7 +396,7 @@ __global__ __launch_bounds__(kTraceBlock) void k_trace_deep(TraceArgs a) {
         float t;
         int32_t p;
         int32_t c[3] = {0, 0, 0};
-        if (traverse(a, r, st, t, p, c)) write_result(a, i, t, p, c);
+        if (traverse<NEAREST>(a, r, st, t, p, c)) write_result(a, i, t, p, c);
         else atomicOr(a.overflow_count + 1, 1u);  // deeper than kMaxStack or not a tree: reported as an error
     }
 }
@@ -459,13 +477,13 @@ void launch_tri_aos(const MeshView& m, float4* tri, uint32_t* bad, hipStream_t s
     hipLaunchKernelGGL(k_tri_aos, dim3(g ? g : 1), dim3(256), 0, s, m, tri, bad);
 }
 
-template <int LDS>
+template <int LDS, bool NEAREST>
 uint32_t resident_blocks() {
     static const uint32_t nb = [] {
         int dev = 0, cus = 0, per = 0;
         (void)hipGetDevice(&dev);
         (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
-        (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, k_trace<LDS>, kTraceBlock, 0);
+        (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, k_trace<LDS, NEAREST>, kTraceBlock, 0);
         const uint32_t v = (uint32_t)(cus > 0 ? cus : 256) * (uint32_t)(per > 0 ? per : 8);
         return v;
     }();
@@ -474,27 +492,32 @@ uint32_t resident_blocks() {
 
 int trace_spill_entries() { return kSpill; }
 
-// One slot per lane of the grid launch_trace uses for n rays: at most one
-// 64-lane block per 64 rays, at most the resident blocks.
+// One slot per lane of the grid launch_trace uses for n rays (either
+// order): at most one 64-lane block per 64 rays, at most the resident blocks.
 uint32_t trace_spill_slots(uint32_t nrays) {
     const uint32_t batches = (nrays + kTraceBlock - 1) / kTraceBlock;
-    const uint32_t g = resident_blocks<kShortStack>();
+    const uint32_t g0 = resident_blocks<kShortStack, false>(), g1 = resident_blocks<kShortStack, true>();
+    const uint32_t g = g0 > g1 ? g0 : g1;
     return (batches < g ? batches : g) * kTraceBlock;
 }
 
-template <int LDS>
+template <int LDS, bool NEAREST>
 void launch_trace_t(const TraceArgs& a, hipStream_t s) {
     const uint32_t batches = (a.nrays + kTraceBlock - 1) / kTraceBlock;
-    uint32_t g = resident_blocks<LDS>();
+    uint32_t g = resident_blocks<LDS, NEAREST>();
     if (g > batches) g = batches;
     if (g > a.spill_slots / kTraceBlock) g = a.spill_slots / kTraceBlock;
-    hipLaunchKernelGGL(k_trace<LDS>, dim3(g ? g : 1), dim3(kTraceBlock), 0, s, a);
+    hipLaunchKernelGGL((k_trace<LDS, NEAREST>), dim3(g ? g : 1), dim3(kTraceBlock), 0, s, a);
 }
 
-void launch_trace(const TraceArgs& a, hipStream_t s) { launch_trace_t<kShortStack>(a, s); }
+void launch_trace(const TraceArgs& a, hipStream_t s) {
+    if (a.nearest) launch_trace_t<kShortStack, true>(a, s);
+    else launch_trace_t<kShortStack, false>(a, s);
+}
 
 void launch_trace_deep(const TraceArgs& a, hipStream_t s) {
-    hipLaunchKernelGGL(k_trace_deep, dim3(1024), dim3(kTraceBlock), 0, s, a);
+    if (a.nearest) hipLaunchKernelGGL(k_trace_deep<true>, dim3(1024), dim3(kTraceBlock), 0, s, a);
+    else hipLaunchKernelGGL(k_trace_deep<false>, dim3(1024), dim3(kTraceBlock), 0, s, a);
 }
 
 }  // namespace bvhk

@@ -241,6 +241,21 @@ int bvh_batch_get_info(bvh_batch_t batch, bvh_batch_info* info);
 int bvh_batch_free(bvh_batch_t batch);
 
 /* ---- traversal ----------------------------------------------------------- */
+/* Visit order of a context's traces (bvh_trace, bvh_trace_host,
+ * bvh_trace_async), set by bvh_ctx_set_trace_order.
+ *   BVH_TRACE_REFERENCE (the default): traverseBVH's order and predicates
+ *     (render_engine.cu:289-356): left child first, a node entered when its
+ *     slab entry distance is below the closest t, a hit taken when its t is
+ *     strictly smaller; ids, t and the three counters are the reference's.
+ *   BVH_TRACE_NEAREST: no reference counterpart, for callers that want the
+ *     closest hit and not the reference's counters: of two children the one
+ *     with the smaller entry distance is visited first (left on a tie), a
+ *     node is entered when its entry distance is at most the closest t, and
+ *     the closest hit is the least (t, triangle index) pair, so it does not
+ *     depend on the visit order.  Counters are those of this order. */
+#define BVH_TRACE_REFERENCE 0
+#define BVH_TRACE_NEAREST 1
+int bvh_ctx_set_trace_order(bvh_ctx_t ctx, int32_t order);
 /* Builds the tree's 64-byte-per-node traversal records now (the first trace
  * after a build does it otherwise).  A tree may be traced from several
  * contexts at once only after this (or one trace) has returned. */

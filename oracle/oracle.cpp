@@ -34,6 +34,9 @@
 //   ploc               src/cuda/ploc_builder.cu:173-264,403-586  naive O(m^2) window transcription in
 //                                                          Python on small n, progress invariant
 //   trace              src/render_engine.cu:194-356          brute-force closest hit over all triangles
+//   trace_nearest      (no reference counterpart; predicates  brute-force least (t, index) hit over all
+//                      of render_engine.cu:194-273)           triangles; visit counts against a
+//                                                          pure-Python transcription on small trees
 //   sah                src/evaluator.cpp:38-88               closed forms: n identical boxes give
 //                                                          Ct(n-1)+Ci*n; hand-computed 3-leaf tree at
 //                                                          (3,2) and (1,1); zero-area root
@@ -495,6 +498,81 @@ inline void trace_one(const ONode* nodes, int64_t num_nodes, const Soa& m, const
     if (counters) { counters[0] = visited; counters[1] = aabb; counters[2] = tris; }
 }
 
+// Slab test of intersectAABB (render_engine.cu:194-214) without the closest-t
+// cull: true when the ray's interval through the box is non-empty; te = the
+// entry distance.
+inline bool slab_te(const Ray& ray, const ONode& nd, float& te) {
+    float tminv[3], tmaxv[3];
+    for (int k = 0; k < 3; ++k) {
+        float t1 = (nd.mn[k] - ray.o[k]) * ray.inv[k];
+        float t2 = (nd.mx[k] - ray.o[k]) * ray.inv[k];
+        tminv[k] = fmin_s(t1, t2);
+        tmaxv[k] = fmax_s(t1, t2);
+    }
+    te = fmax_s(fmax_s(tminv[0], tminv[1]), tminv[2]);
+    float texit = fmin_s(fmin_s(tmaxv[0], tmaxv[1]), tmaxv[2]);
+    return texit >= fmax_s(te, 0.0f);
+}
+
+// Nearest-first closest hit (no reference counterpart; the engine's
+// BVH_TRACE_NEAREST order): the same slab and triangle predicates as
+// traverseBVH, but of two children that pass, the one with the smaller entry
+// distance is visited first (left on a tie); an entry whose entry distance
+// exceeds the closest t is skipped (kept when equal); the hit is the least
+// (t, triangle index) pair.  Counters as traverseBVH counts: every child in
+// range is a node visited and an AABB test when its parent is expanded (the
+// root once), every leaf entered a triangle test.
+inline void trace_one_nearest(const ONode* nodes, int64_t num_nodes, const Soa& m, const float* r, float& out_t,
+                              int32_t& out_prim, int32_t* counters, std::vector<std::pair<int32_t, float>>& stack) {
+    Ray ray = make_ray(r);
+    float best = 1e30f;
+    int32_t prim = -1;
+    int32_t visited = 0, aabb = 0, tris = 0;
+    stack.clear();
+    if (num_nodes > 0) {
+        visited++;
+        aabb++;
+        float te;
+        if (slab_te(ray, nodes[0], te)) stack.push_back({0, te});
+    }
+    while (!stack.empty()) {
+        const std::pair<int32_t, float> e = stack.back();
+        stack.pop_back();
+        if (e.second > best) continue;
+        const ONode& nd = nodes[e.first];
+        if (nd.left < 0) {
+            uint32_t p = (uint32_t)nd.left & 0x7FFFFFFFu;
+            tris++;
+            float t;
+            if (hit_tri(ray, m, p, t) && (t < best || (t == best && (int32_t)p < prim))) {
+                best = t;
+                prim = (int32_t)p;
+            }
+            continue;
+        }
+        const int32_t c[2] = {nd.left, nd.right};
+        bool keep[2] = {false, false};
+        float te[2] = {0.0f, 0.0f};
+        for (int k = 0; k < 2; ++k) {
+            if (c[k] < 0 || c[k] >= num_nodes) continue;
+            visited++;
+            aabb++;
+            keep[k] = slab_te(ray, nodes[c[k]], te[k]) && te[k] <= best;
+        }
+        if (keep[0] && keep[1]) {
+            const int nearer = te[1] < te[0] ? 1 : 0;
+            stack.push_back({c[1 - nearer], te[1 - nearer]});
+            stack.push_back({c[nearer], te[nearer]});
+        } else if (keep[0] || keep[1]) {
+            const int k = keep[0] ? 0 : 1;
+            stack.push_back({c[k], te[k]});
+        }
+    }
+    out_t = best;
+    out_prim = prim;
+    if (counters) { counters[0] = visited; counters[1] = aabb; counters[2] = tris; }
+}
+
 // computeSAHRecursive (evaluator.cpp:54-88) with parameterised costs.
 float sah_rec(const ONode* nodes, int64_t num, int64_t idx, float rootArea, float ct, float ci) {
     if (idx < 0 || idx >= num) return 0.0f;
@@ -773,6 +851,23 @@ void orc_trace_omp(const void* nodes, int64_t num_nodes, const float* const* soa
         for (int64_t r = 0; r < nrays; ++r)
             trace_one(nd, num_nodes, m, rays + 6 * r, t_out[r], prim_out[r],
                       counters ? counters + 3 * r : nullptr, stack);
+    }
+}
+
+// Nearest-first traversal (trace_one_nearest) of rays; counters optional.
+void orc_trace_nearest(const void* nodes, int64_t num_nodes, const float* const* soa, const float* rays,
+                       int64_t nrays, float* t_out, int32_t* prim_out, int32_t* counters) {
+    Soa m;
+    for (int k = 0; k < 9; ++k) m.v[k] = soa[k];
+    const ONode* nd = (const ONode*)nodes;
+#pragma omp parallel
+    {
+        std::vector<std::pair<int32_t, float>> stack;
+        stack.reserve(256);
+#pragma omp for schedule(dynamic, 1024)
+        for (int64_t r = 0; r < nrays; ++r)
+            trace_one_nearest(nd, num_nodes, m, rays + 6 * r, t_out[r], prim_out[r],
+                              counters ? counters + 3 * r : nullptr, stack);
     }
 }
 

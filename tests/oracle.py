@@ -46,6 +46,7 @@ def lib():
             "orc_build_ploc": (C.c_int, [_vp, _i64, C.c_int, _vp, _vp, _vp, C.c_int32, _vp]),
             "orc_trace": (None, [_vp, _i64, _vp, _vp, _i64, _vp, _vp, _vp]),
             "orc_trace_omp": (None, [_vp, _i64, _vp, _vp, _i64, _vp, _vp, _vp]),
+            "orc_trace_nearest": (None, [_vp, _i64, _vp, _vp, _i64, _vp, _vp, _vp]),
             "orc_trace_brute": (None, [_vp, _i64, _vp, _i64, _vp, _vp]),
             "orc_sah": (C.c_float, [_vp, _i64, C.c_float, C.c_float]),
             "orc_depth_stats": (None, [_vp, _i64, _vp, _vp, _vp]),
@@ -180,14 +181,16 @@ def build_ploc(m, radius=25, max_iters=4096):
     return st, nodes, idx, iters[: min(nit.value, max_iters)]
 
 
-def trace(nodes, m, rays, omp=False):
+def trace(nodes, m, rays, omp=False, nearest=False):
+    """Closest hit per ray (t, prim, counters): the reference order, or the
+    nearest-first order (least (t, index) hit) when nearest."""
     soa = _as_soa(m)
     rays = np.ascontiguousarray(rays, np.float32)
     nr = rays.shape[0]
     t = np.zeros(nr, np.float32)
     prim = np.zeros(nr, np.int32)
     cnt = np.zeros((nr, 3), np.int32)
-    f = lib().orc_trace_omp if omp else lib().orc_trace
+    f = lib().orc_trace_nearest if nearest else (lib().orc_trace_omp if omp else lib().orc_trace)
     f(nodes.ctypes.data, nodes.size, _soa(soa), rays.ctypes.data, nr, t.ctypes.data, prim.ctypes.data,
       cnt.ctypes.data)
     return t, prim, cnt

@@ -224,6 +224,38 @@ def large_strands(ctx, steps=2):
     return out
 
 
+SERVERS = {"coalesced": {"slots": 4, "max_group": 32}, "plain": {"slots": 1, "max_group": 1},
+           "coalesced_nearest": {"slots": 4, "max_group": 32, "trace_order": 1}}
+
+
+def serve_child(spec):
+    """One server, one measurement (run in a process of its own): the request
+    stream, closed-loop (rate None) or Poisson at `rate` requests/s; prints
+    one JSON line."""
+    from bvh_amd import serving as S
+    count, seed, pool = spec["count"], spec["seed"], spec["pool"]
+    reqs = S.make_requests(seed, count, pool=pool)
+    warm = S.make_requests(seed + 1, 4, tri_lo=10_000, tri_hi=50_000, ray_lo=65_536, ray_hi=100_000, follow_up=0.0)
+    for q in warm:
+        q["key"] = 0
+    rate = spec["rate"]
+    offsets = np.zeros(count) if rate is None else S.poisson_arrivals(rate, count, seed)
+    with S.Server(algo=S.ALGO_LBVH, **SERVERS[spec["server"]]) as srv:
+        S.run_open_loop(srv, warm, np.zeros(len(warm)))
+        done, _, t0 = S.run_open_loop(srv, reqs, offsets)
+        st = S.latency_stats(done, t0)
+        ss = srv.stats()
+    st["groups"] = ss["groups"]
+    st["max_group"] = ss["max_group_seen"]
+    st["evictions"] = ss["evictions"]
+    st["peak_device_GB"] = round(ss["peak"] / 1e9, 2)
+    if rate is not None:
+        st["offered_rps"] = round(rate, 2)
+    st["triangles_total"] = int(sum(q["ntri"] for q in reqs))
+    st["rays_total"] = int(sum(q["rays"].shape[0] for q in reqs))
+    print(json.dumps(st), flush=True)
+
+
 def serving_latency(count=1000, seed=23, pool=96):
     """Build+trace requests (blob meshes log-uniform 10K-2M triangles, rays
     log-uniform 65,536-1,048,576 from the bounding sphere toward the box, 20%
@@ -231,47 +263,44 @@ def serving_latency(count=1000, seed=23, pool=96):
     (mesh, rays) pairs drawn at random by the `count` requests, each build
     request under its own key.  Capacity = closed-loop saturation (all
     requests at t=0) of each server; then Poisson arrivals at 50% and 90% of
-    each server's own capacity, and both servers at 90% of the plain path's
+    each server's own capacity, and the first two at 90% of the plain path's
     capacity (the same offered load).  Servers: coalescing (4 slots, groups
-    of up to 32) and plain (1 slot, one mesh per launch); p50/p99 of done -
-    arrival on the server clock."""
-    from bvh_amd import serving as S
-    log(f"[bench] serving: generating {count} requests ({pool} distinct meshes, seed {seed})")
-    reqs = S.make_requests(seed, count, pool=pool)
-    warm = S.make_requests(seed + 1, 4, tri_lo=10_000, tri_hi=50_000, ray_lo=65_536, ray_hi=100_000, follow_up=0.0)
-    for q in warm:
-        q["key"] = 0
-    servers = {"coalesced": {"slots": 4, "max_group": 32}, "plain": {"slots": 1, "max_group": 1}}
+    of up to 32), plain (1 slot, one mesh per launch) and coalescing with the
+    nearest-first trace order; p50/p99 of done - arrival on the server clock.
+    Each measurement runs in a fresh process, as a deployed server does (a
+    server started in a process where another one has already served
+    measures slower)."""
+    import subprocess
 
-    def one(name, offsets):
-        with S.Server(algo=S.ALGO_LBVH, **servers[name]) as srv:
-            S.run_open_loop(srv, warm, np.zeros(len(warm)))
-            done, _, t0 = S.run_open_loop(srv, reqs, offsets)
-            st = S.latency_stats(done, t0)
-            ss = srv.stats()
-            st["groups"] = ss["groups"]
-            st["max_group"] = ss["max_group_seen"]
-            st["evictions"] = ss["evictions"]
-            st["peak_device_GB"] = round(ss["peak"] / 1e9, 2)
-            return st
+    def one(name, rate):
+        spec = {"server": name, "rate": rate, "count": count, "seed": seed, "pool": pool}
+        p = subprocess.run([sys.executable, os.path.abspath(__file__), "--serve-child", json.dumps(spec)],
+                           stdout=subprocess.PIPE, text=True, timeout=900)
+        if p.returncode != 0:
+            raise RuntimeError(f"serving measurement {spec} failed with status {p.returncode}")
+        return json.loads(p.stdout.strip().splitlines()[-1])
 
     out = {"requests": count, "distinct_meshes": pool, "seed": seed, "algo": "LBVH",
-           "triangles_total": int(sum(q["ntri"] for q in reqs)),
-           "rays_total": int(sum(q["rays"].shape[0] for q in reqs))}
-    for name in servers:
+           "process_per_measurement": True}
+    for name in SERVERS:
         log(f"[bench] serving: capacity ({name})")
-        out[f"capacity_{name}"] = one(name, np.zeros(count))
-    for name in servers:
+        r = one(name, None)
+        out["triangles_total"] = r.pop("triangles_total")
+        out["rays_total"] = r.pop("rays_total")
+        out[f"capacity_{name}"] = r
+    for name in SERVERS:
         cap = out[f"capacity_{name}"]["throughput_rps"]
         for load in (0.5, 0.9):
             log(f"[bench] serving: {int(load * 100)}% of its capacity ({name})")
-            r = one(name, S.poisson_arrivals(load * cap, count, seed))
-            r["offered_rps"] = round(load * cap, 2)
+            r = one(name, load * cap)
+            r.pop("triangles_total")
+            r.pop("rays_total")
             out[f"load{int(load * 100)}_{name}"] = r
     same = 0.9 * out["capacity_plain"]["throughput_rps"]
     log("[bench] serving: coalescing at the plain path's 90% load")
-    r = one("coalesced", S.poisson_arrivals(same, count, seed))
-    r["offered_rps"] = round(same, 2)
+    r = one("coalesced", same)
+    r.pop("triangles_total")
+    r.pop("rays_total")
     out["plain90_load_coalesced"] = r
     return out
 
@@ -410,10 +439,14 @@ def main():
     ap.add_argument("--no-large", action="store_true", help="skip the 32M-triangle strands workload")
     ap.add_argument("--calib", action="store_true",
                     help="only run the 256 MiB stream-copy kernel (reference point for HBM counters)")
+    ap.add_argument("--serve-child", default="", help=argparse.SUPPRESS)
     ap.add_argument("--sections", default="",
                     help="profiling: comma list of companions,batch,large,serve to run without the headline "
                          "(prints their JSON only)")
     args = ap.parse_args()
+    if args.serve_child:
+        serve_child(json.loads(args.serve_child))
+        return
     if args.calib:
         calib_copy()
         return

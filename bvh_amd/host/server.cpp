@@ -455,6 +455,7 @@ int bvh_server_create(const bvh_server_config* cfg, bvh_server_t* out) {
     if (cfg->algo != BVH_ALGO_LBVH && cfg->algo != BVH_ALGO_PLOC) return BVH_E_UNSUPPORTED;
     if (cfg->algo == BVH_ALGO_PLOC && (cfg->ploc_radius < 1 || cfg->ploc_radius > 1024)) return BVH_E_ARG;
     if (cfg->slots < 0 || cfg->slots > 16 || cfg->max_group < 0) return BVH_E_ARG;
+    if (cfg->trace_order != BVH_TRACE_REFERENCE && cfg->trace_order != BVH_TRACE_NEAREST) return BVH_E_ARG;
     auto s = std::make_unique<bvh_server_rec>();
     s->cfg = *cfg;
     if (s->cfg.slots == 0) s->cfg.slots = 4;
@@ -466,7 +467,9 @@ int bvh_server_create(const bvh_server_config* cfg, bvh_server_t* out) {
     for (int i = 0; i < s->cfg.slots; ++i) {
         bvh_ctx_t c = nullptr;
         int st = bvh_ctx_create(cfg->device, &c);
+        if (st == BVH_OK) st = bvh_ctx_set_trace_order(c, cfg->trace_order);
         if (st != BVH_OK) {
+            if (c) bvh_ctx_destroy(c);
             for (bvh_ctx_t x : s->ctxs) bvh_ctx_destroy(x);
             return st;
         }

@@ -27,7 +27,7 @@ ALGO_LBVH, ALGO_PLOC = N.ALGO_LBVH, N.ALGO_PLOC
 class ServerConfig(C.Structure):
     _fields_ = [("device", C.c_int32), ("slots", C.c_int32), ("max_group", C.c_int32),
                 ("max_group_tris", C.c_uint32), ("memory_budget", C.c_uint64), ("algo", C.c_int32),
-                ("ploc_radius", C.c_int32)]
+                ("ploc_radius", C.c_int32), ("trace_order", C.c_int32)]
 
 
 class Request(C.Structure):
@@ -98,9 +98,10 @@ def form_group(bytes_, tris, runnable, free_bytes, max_group, max_group_tris) ->
 class Server:
     """Native server: `slots` streams, groups of up to `max_group` requests."""
 
-    def __init__(self, algo=ALGO_LBVH, radius=16, slots=4, max_group=32, max_group_tris=0, budget=0, device=0):
+    def __init__(self, algo=ALGO_LBVH, radius=16, slots=4, max_group=32, max_group_tris=0, budget=0, device=0,
+                 trace_order=0):
         L = lib()
-        cfg = ServerConfig(device, slots, max_group, max_group_tris, budget, algo, radius)
+        cfg = ServerConfig(device, slots, max_group, max_group_tris, budget, algo, radius, trace_order)
         h = C.c_void_p()
         st = L.bvh_server_create(C.byref(cfg), C.byref(h))
         if st != 0:

@@ -39,6 +39,7 @@ typedef struct bvh_server_config {
     uint64_t memory_budget;   /* device bytes the server may hold (0 = 90% of the device's free memory) */
     int32_t algo;             /* BVH_ALGO_LBVH or BVH_ALGO_PLOC */
     int32_t ploc_radius;      /* PLOC window radius */
+    int32_t trace_order;      /* BVH_TRACE_REFERENCE (0) or BVH_TRACE_NEAREST (bvh_hip.h) for every trace */
 } bvh_server_config;
 
 typedef struct bvh_request {

@@ -41,6 +41,29 @@ def test_serving_same_alone_and_in_mix(ctx, algo):
     assert sum(c["built"] for c in done) == sum(1 for q in reqs if q["mesh"] is not None)
 
 
+def test_serving_nearest_order_same_alone_and_in_mix(ctx):
+    """A server configured with the nearest-first order: each request's hits
+    equal its own build traced alone in that order, and differ from the
+    reference order's only at equal t."""
+    reqs = S.make_requests(5, 12, tri_lo=2000, tri_hi=150_000, ray_lo=2000, ray_hi=40_000, follow_up=0.25)
+    with S.Server(algo=B.ALGO_LBVH, slots=3, max_group=6, trace_order=B.TRACE_NEAREST) as srv:
+        done, outs, _ = S.run_open_loop(srv, reqs, np.zeros(len(reqs)))
+    assert [c["status"] for c in done] == [0] * len(reqs), done
+    mesh_of = {q["key"]: q["mesh"] for q in reqs if q["mesh"] is not None}
+    for q in reqs:
+        ctx.set_trace_order(B.TRACE_NEAREST)
+        try:
+            t, prim = _alone(ctx, mesh_of[q["key"]], q["rays"], B.ALGO_LBVH, 16)
+        finally:
+            ctx.set_trace_order(B.TRACE_REFERENCE)
+        got_t, got_prim = outs[q["id"]]
+        np.testing.assert_array_equal(got_prim, prim, err_msg=f"request {q['id']}")
+        assert got_t.tobytes() == t.tobytes(), f"request {q['id']}"
+        rt, rprim = _alone(ctx, mesh_of[q["key"]], q["rays"], B.ALGO_LBVH, 16)
+        other = got_prim != rprim
+        assert np.array_equal(got_t[other], rt[other]), f"request {q['id']}"
+
+
 def test_serving_resident_followups_and_eviction(ctx):
     meshes = [B.Mesh.blobs(60 + i, 50_000) for i in range(4)]
     tr, rs = S.footprint(50_000, 1000, B.ALGO_LBVH)

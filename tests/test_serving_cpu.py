@@ -155,3 +155,11 @@ def test_footprint_covers_trace_buffers(rays):
     # rays 24 + t 4 + prim 4 + counters 12 + overflow list 4 (+ two 8-byte sort keys from 16384 rays on)
     per = 48 + (16 if rays >= 16384 else 0)
     assert f.trace_workspace >= per * rays
+
+
+def test_server_rejects_an_unknown_trace_order():
+    # Checked before any device is touched.
+    from bvh_amd import NativeError
+    with pytest.raises(NativeError) as ei:
+        S.Server(algo=S.ALGO_LBVH, trace_order=5)
+    assert ei.value.status == 1

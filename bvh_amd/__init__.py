@@ -175,6 +175,7 @@ class Context:
         if st != N.BVH_OK:
             raise NativeError(st, "bvh_ctx_create")
         self.h = h
+        self.device = device
 
     def _check(self, st, what=""):
         if st != N.BVH_OK:
@@ -263,6 +264,16 @@ class Context:
         n = rays.nbytes // 24
         self._check(self._lib.bvh_trace_async(self.h, tree.h, dmesh.h, rays.ptr, n, t.ptr, prim.ptr,
                                               counters.ptr if counters is not None else None), "trace_async")
+
+    def cache_info(self):
+        """(bytes, blocks) of released device blocks cached for this context's device."""
+        b, n = C.c_uint64(), C.c_uint32()
+        self._check(self._lib.bvh_device_cache_info(self.device, C.byref(b), C.byref(n)), "cache info")
+        return int(b.value), int(n.value)
+
+    def cache_trim(self):
+        """Frees the device's cached blocks (bvh_device_cache_trim)."""
+        self._check(self._lib.bvh_device_cache_trim(self.device), "cache trim")
 
     def set_trace_order(self, order: int):
         """Visit order of this context's traces (bvh_ctx_set_trace_order):
@@ -390,6 +401,13 @@ class DeviceMesh:
             self.ctx._lib.bvh_mesh_free(self.ctx.h, self.h)
             self.h = None
 
+    def release(self):
+        """Returns the mesh's device blocks to the device cache (bvh_mesh_release):
+        only when no queued work on any context uses the mesh."""
+        if self.h:
+            self.ctx._check(self.ctx._lib.bvh_mesh_release(self.ctx.h, self.h), "mesh release")
+            self.h = None
+
 
 class Tree:
     def __init__(self, ctx: Context, h):
@@ -427,6 +445,12 @@ class Tree:
     def free(self):
         if self.h:
             self.ctx._lib.bvh_tree_free(self.ctx.h, self.h)
+            self.h = None
+
+    def release(self):
+        """bvh_tree_release: like release() of a mesh."""
+        if self.h:
+            self.ctx._check(self.ctx._lib.bvh_tree_release(self.ctx.h, self.h), "tree release")
             self.h = None
 
 

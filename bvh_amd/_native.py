@@ -107,10 +107,15 @@ _HIP_SIGS = {
     "bvh_host_free_pinned": (_i32, [_vp]),
     "bvh_mesh_upload": (_i32, [_vp, C.POINTER(_vp), _u32, _pp]),
     "bvh_mesh_free": (_i32, [_vp, _vp]),
+    "bvh_mesh_release": (_i32, [_vp, _vp]),
+    "bvh_device_cache_info": (_i32, [_i32, C.POINTER(C.c_uint64), C.POINTER(C.c_uint32)]),
+    "bvh_device_cache_trim": (_i32, [_i32]),
+    "bvh_device_cache_set_limit": (_i32, [_i32, C.c_uint64]),
     "bvh_mesh_size": (_i32, [_vp, C.POINTER(_u32)]),
     "bvh_mesh_check": (_i32, [_vp]),
     "bvh_tree_create": (_i32, [_vp, _pp]),
     "bvh_tree_free": (_i32, [_vp, _vp]),
+    "bvh_tree_release": (_i32, [_vp, _vp]),
     "bvh_tree_info": (_i32, [_vp, C.POINTER(_u32), C.POINTER(_u32)]),
     "bvh_tree_download": (_i32, [_vp, _vp, _vp, _vp]),
     "bvh_tree_from_nodes": (_i32, [_vp, _vp, _u32, _vp]),

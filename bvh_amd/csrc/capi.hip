@@ -16,6 +16,7 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <map>
 #include <mutex>
 #include <string>
 #include <vector>
@@ -99,6 +100,8 @@ struct bvh_context {
 struct bvh_mesh_rec {
     float* base = nullptr;  // 9 x n floats, one allocation
     float4* tri = nullptr;  // 3 x float4 per triangle (traversal copy)
+    size_t base_cap = 0, tri_cap = 0;  // allocation sizes (blocks may come from the release cache)
+    int device = 0;
     uint32_t n = 0;
     bool nonfinite = false;  // a coordinate is NaN or infinite: every build is refused
     size_t stride = 0;      // floats between arrays
@@ -111,6 +114,7 @@ struct bvh_mesh_rec {
 };
 
 struct bvh_tree_rec {
+    int device = -1;         // device of its storage (set when first allocated)
     DevBuf nodes, indices;
     DevBuf rec;              // traversal records, built on first trace after a build
     bool rec_valid = false;
@@ -176,6 +180,95 @@ int set_err(bvh_ctx_t ctx, int code, const char* fmt, ...) {
         if (st_) return st_;       \
     } while (0)
 
+// ---- released device blocks ------------------------------------------------
+// Meshes and trees given back with bvh_mesh_release / bvh_tree_release (their
+// owner vouches that no queued work uses them) keep their device blocks here,
+// per device, instead of going to hipFree, which waits for the whole device;
+// a later mesh upload or new buffer takes the smallest cached block that is
+// large enough and at most 1/16 plus 64 KiB larger than asked for (a serving
+// loop admits work by the sizes it asks for, so the excess a reused block
+// brings must stay small against its memory headroom).  An
+// allocation that fails frees every cached block of its device and tries
+// once more.  The cache holds at most `limit` bytes (default an eighth of the
+// device's memory); a released block beyond it is freed at once.
+struct BlockCache {
+    std::mutex mu;
+    std::multimap<size_t, void*> blocks;  // capacity -> block
+    uint64_t bytes = 0;
+    uint64_t limit = 0;  // 0: not set yet
+};
+constexpr int kCacheDevices = 64;
+
+BlockCache& block_cache(int device) {
+    static BlockCache caches[kCacheDevices];
+    return caches[(unsigned)device % kCacheDevices];
+}
+
+void* cache_take(int device, size_t want, size_t* cap) {
+    BlockCache& c = block_cache(device);
+    std::lock_guard<std::mutex> lk(c.mu);
+    auto it = c.blocks.lower_bound(want);
+    if (it == c.blocks.end() || it->first > want + want / 16 + ((size_t)64 << 10)) return nullptr;
+    void* p = it->second;
+    *cap = it->first;
+    c.bytes -= it->first;
+    c.blocks.erase(it);
+    return p;
+}
+
+void cache_give(int device, void* p, size_t cap) {
+    if (!p) return;
+    BlockCache& c = block_cache(device);
+    {
+        std::lock_guard<std::mutex> lk(c.mu);
+        if (c.limit == 0) {
+            size_t fr = 0, tot = 0;
+            c.limit = hipMemGetInfo(&fr, &tot) == hipSuccess ? tot / 8 : ((uint64_t)8 << 30);
+        }
+        if (c.bytes + cap <= c.limit) {
+            c.blocks.emplace(cap, p);
+            c.bytes += cap;
+            return;
+        }
+    }
+    (void)hipFree(p);
+}
+
+// Frees every cached block of the device (each hipFree waits for the device).
+void cache_trim(int device) {
+    BlockCache& c = block_cache(device);
+    std::lock_guard<std::mutex> lk(c.mu);
+    for (auto& kv : c.blocks) (void)hipFree(kv.second);
+    c.blocks.clear();
+    c.bytes = 0;
+}
+
+// Device memory of at least `want` bytes (*cap = its size): a cached block or
+// a new allocation.
+hipError_t dev_alloc(int device, void** p, size_t want, size_t* cap) {
+    if ((*p = cache_take(device, want, cap)) != nullptr) return hipSuccess;
+    hipError_t e = hipMalloc(p, want);
+    if (e != hipSuccess) {
+        (void)hipGetLastError();
+        bool any;
+        {
+            BlockCache& c = block_cache(device);
+            std::lock_guard<std::mutex> lk(c.mu);
+            any = !c.blocks.empty();
+        }
+        if (any) {
+            cache_trim(device);
+            e = hipMalloc(p, want);
+        }
+    }
+    if (e != hipSuccess) {
+        *p = nullptr;
+        return e;
+    }
+    *cap = want;
+    return hipSuccess;
+}
+
 // Grow-only allocation; a new buffer is zeroed when `zero` is set (never
 // called while a stream is being captured).
 int ensure(bvh_ctx_t ctx, DevBuf& b, size_t bytes, bool zero = false) {
@@ -196,13 +289,14 @@ int ensure(bvh_ctx_t ctx, DevBuf& b, size_t bytes, bool zero = false) {
     }
     size_t want = bytes + bytes / 8;
     want = (want + 255) & ~(size_t)255;
-    hipError_t e = hipMalloc(&b.p, want);
+    size_t cap = 0;
+    hipError_t e = dev_alloc(ctx->device, &b.p, want, &cap);
     if (e != hipSuccess) {
         b.p = nullptr;
         return set_err(ctx, BVH_E_NOMEM, "hipMalloc(%zu) failed: %s", want, hipGetErrorString(e));
     }
-    b.cap = want;
-    if (zero) HIP_TRY(ctx, hipMemsetAsync(b.p, 0, want, ctx->stream));
+    b.cap = cap;
+    if (zero) HIP_TRY(ctx, hipMemsetAsync(b.p, 0, cap, ctx->stream));
     return BVH_OK;
 }
 
@@ -274,7 +368,16 @@ int pick_bits(int requested) {
     return 8;
 }
 
+// A tree's storage lives on the device of the context that first allocates it.
+int tree_on(bvh_ctx_t ctx, bvh_tree_t tree) {
+    if (tree->device >= 0 && tree->device != ctx->device)
+        return set_err(ctx, BVH_E_ARG, "tree belongs to device %d, context to device %d", tree->device, ctx->device);
+    tree->device = ctx->device;
+    return BVH_OK;
+}
+
 int ensure_tree(bvh_ctx_t ctx, bvh_tree_t tree, uint32_t n) {
+    TRY(tree_on(ctx, tree));
     TRY(ensure(ctx, tree->nodes, (size_t)(2 * (size_t)n - 1) * sizeof(DNode)));
     TRY(ensure(ctx, tree->indices, (size_t)n * 4));
     tree->num_prims = n;
@@ -1061,6 +1164,7 @@ int batch_tune(bvh_batch_t b) {
 // other contexts may trace the tree afterwards).
 int prepare_trace(bvh_ctx_t ctx, bvh_tree_t tree) {
     if (tree->rec_valid) return BVH_OK;
+    TRY(tree_on(ctx, tree));
     TRY(ensure(ctx, tree->rec, (size_t)tree->num_nodes * 64));
     launch_trace_layout(P<DNode>(tree->nodes), tree->num_nodes, P<float4>(tree->rec), ctx->stream);
     TRY(check_launch(ctx, "k_trace_layout"));
@@ -1424,7 +1528,9 @@ int bvh_mesh_upload(bvh_ctx_t ctx, const float* const* soa, uint32_t n, bvh_mesh
     bvh_mesh_rec* m = new bvh_mesh_rec();
     m->n = n;
     m->stride = ((size_t)n + 63) & ~(size_t)63;  // 256-byte aligned arrays
-    hipError_t e = hipMalloc(&m->base, m->stride * 9 * sizeof(float));
+    m->device = ctx->device;
+    hipError_t e = dev_alloc(ctx->device, reinterpret_cast<void**>(&m->base), m->stride * 9 * sizeof(float),
+                             &m->base_cap);
     if (e != hipSuccess) {
         delete m;
         return set_err(ctx, BVH_E_NOMEM, "mesh allocation failed: %s", hipGetErrorString(e));
@@ -1439,7 +1545,7 @@ int bvh_mesh_upload(bvh_ctx_t ctx, const float* const* soa, uint32_t n, bvh_mesh
     }
     // The traversal copy is followed by one word that the copy kernel sets
     // when any coordinate is NaN or infinite.
-    e = hipMalloc(&m->tri, (size_t)n * 48 + 64);
+    e = dev_alloc(ctx->device, reinterpret_cast<void**>(&m->tri), (size_t)n * 48 + 64, &m->tri_cap);
     if (e != hipSuccess) {
         (void)hipFree(m->base);
         delete m;
@@ -1458,6 +1564,51 @@ int bvh_mesh_upload(bvh_ctx_t ctx, const float* const* soa, uint32_t n, bvh_mesh
     }
     m->nonfinite = hbad != 0;
     *out = m;
+    return BVH_OK;
+}
+
+int bvh_mesh_release(bvh_ctx_t ctx, bvh_mesh_t mesh) {
+    if (!ctx || !mesh) return BVH_E_ARG;
+    if (mesh->device != ctx->device) return set_err(ctx, BVH_E_ARG, "mesh of another device");
+    HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+    cache_give(ctx->device, mesh->base, mesh->base_cap);
+    cache_give(ctx->device, mesh->tri, mesh->tri_cap);
+    delete mesh;
+    return BVH_OK;
+}
+
+int bvh_device_cache_info(int32_t device, uint64_t* bytes, uint32_t* blocks) {
+    if (device < 0 || device >= kCacheDevices) return BVH_E_ARG;
+    BlockCache& c = block_cache(device);
+    std::lock_guard<std::mutex> lk(c.mu);
+    if (bytes) *bytes = c.bytes;
+    if (blocks) *blocks = (uint32_t)c.blocks.size();
+    return BVH_OK;
+}
+
+int bvh_device_cache_set_limit(int32_t device, uint64_t bytes) {
+    if (device < 0 || device >= kCacheDevices) return BVH_E_ARG;
+    if (hipSetDevice(device) != hipSuccess) return BVH_E_HIP;
+    BlockCache& c = block_cache(device);
+    std::vector<void*> drop;
+    {
+        std::lock_guard<std::mutex> lk(c.mu);
+        c.limit = bytes ? bytes : 1;  // 0 would mean "not set": keep nothing instead
+        while (c.bytes > c.limit && !c.blocks.empty()) {  // largest blocks first
+            auto it = std::prev(c.blocks.end());
+            c.bytes -= it->first;
+            drop.push_back(it->second);
+            c.blocks.erase(it);
+        }
+    }
+    for (void* p : drop) (void)hipFree(p);
+    return BVH_OK;
+}
+
+int bvh_device_cache_trim(int32_t device) {
+    if (device < 0 || device >= kCacheDevices) return BVH_E_ARG;
+    if (hipSetDevice(device) != hipSuccess) return BVH_E_HIP;
+    cache_trim(device);
     return BVH_OK;
 }
 
@@ -1497,6 +1648,19 @@ int bvh_tree_free(bvh_ctx_t ctx, bvh_tree_t tree) {
     return BVH_OK;
 }
 
+int bvh_tree_release(bvh_ctx_t ctx, bvh_tree_t tree) {
+    if (!ctx || !tree) return BVH_E_ARG;
+    if (tree->device >= 0 && tree->device != ctx->device) return set_err(ctx, BVH_E_ARG, "tree of another device");
+    HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+    for (DevBuf* b : {&tree->nodes, &tree->indices, &tree->rec}) {
+        cache_give(ctx->device, b->p, b->cap);
+        b->p = nullptr;
+        b->cap = 0;
+    }
+    delete tree;
+    return BVH_OK;
+}
+
 int bvh_tree_info(bvh_tree_t tree, uint32_t* num_nodes, uint32_t* num_prims) {
     if (!tree) return BVH_E_ARG;
     if (num_nodes) *num_nodes = tree->num_nodes;
@@ -1520,6 +1684,7 @@ int bvh_tree_download(bvh_ctx_t ctx, bvh_tree_t tree, void* nodes, uint32_t* ind
 int bvh_tree_from_nodes(bvh_ctx_t ctx, const void* nodes, uint32_t num_nodes, bvh_tree_t tree) {
     if (!ctx || !nodes || !tree || num_nodes == 0) return BVH_E_ARG;
     int st;
+    if ((st = tree_on(ctx, tree))) return st;
     if ((st = ensure(ctx, tree->nodes, (size_t)num_nodes * sizeof(DNode)))) return st;
     HIP_TRY(ctx, hipMemcpyAsync(tree->nodes.p, nodes, (size_t)num_nodes * sizeof(DNode), hipMemcpyHostToDevice,
                                 ctx->stream));

@@ -88,6 +88,7 @@ struct bvh_server_rec {
     // what each slot's grow-only context workspace holds (or has reserved for
     // the group it runs).  Free memory = budget - in_use - held_total.
     uint64_t in_use = 0, peak = 0, resident_bytes = 0, lru = 0, held_total = 0;
+    uint64_t cache_limit = 0, admit_budget = 0;  // device-cache share and what admission may use
     std::vector<uint64_t> held_build, held_trace;
     uint32_t in_flight = 0, groups = 0, completed = 0, evictions = 0, rejected = 0, max_group_seen = 0;
     std::string err;
@@ -212,7 +213,7 @@ bool bvh_server_rec::form_group(int slot, std::vector<Req>& group, Evicted* ev) 
         if (ncand == 0) return false;  // everything queued waits for a mesh being built
         const std::vector<uint8_t> run(ncand, 1);
         const uint64_t used = in_use + held_total;
-        const uint64_t free_bytes = cfg.memory_budget > used ? cfg.memory_budget - used : 0;
+        const uint64_t free_bytes = admit_budget > used ? admit_budget - used : 0;
         // A slot answers its group's requests one after another, so it takes
         // at most its share of the queue (all slots' worth when the queue is
         // deep): the rest stays for the next slot that frees up.
@@ -312,9 +313,9 @@ bool bvh_server_rec::form_group(int slot, std::vector<Req>& group, Evicted* ev) 
 // and publish its completion.
 void bvh_server_rec::finish_one(int slot, Req& q, int status, uint32_t gsize) {
     bvh_ctx_t ctx = ctxs[slot];
-    if (q.build && !q.res) {  // unkeyed meshes end with their request
-        if (q.tree) bvh_tree_free(ctx, q.tree);
-        if (q.mesh) bvh_mesh_free(ctx, q.mesh);
+    if (q.build && !q.res) {  // unkeyed meshes end with their request (nothing else uses them)
+        if (q.tree) bvh_tree_release(ctx, q.tree);
+        if (q.mesh) bvh_mesh_release(ctx, q.mesh);
     }
     std::lock_guard<std::mutex> lk(mu);
     in_use -= q.transient;
@@ -344,8 +345,8 @@ void bvh_server_rec::publish_residents(int slot, std::vector<Req>& group, std::v
             q.res->tree = q.tree;
             q.res->ready = true;
         } else {
-            if (q.tree) bvh_tree_free(ctx, q.tree);
-            if (q.mesh) bvh_mesh_free(ctx, q.mesh);
+            if (q.tree) bvh_tree_release(ctx, q.tree);
+            if (q.mesh) bvh_mesh_release(ctx, q.mesh);
             q.mesh = nullptr;
             q.tree = nullptr;
             in_use -= q.res->bytes;
@@ -367,11 +368,14 @@ void bvh_server_rec::run_slot(int slot) {
                 Evicted ev;
                 if (form_group(slot, group, &ev)) break;
                 if (ev.mesh || ev.tree) {
-                    // Free the evicted mesh without holding the lock (hipFree
-                    // synchronises the device); its bytes stay counted until then.
+                    // Release the evicted mesh without holding the lock; its
+                    // bytes stay counted until then.  An idle resident has no
+                    // queued work on any slot (its build and every trace of it
+                    // have returned), so its blocks go to the device cache
+                    // without waiting for the other slots' work.
                     lk.unlock();
-                    if (ev.tree) bvh_tree_free(ctx, ev.tree);
-                    if (ev.mesh) bvh_mesh_free(ctx, ev.mesh);
+                    if (ev.tree) bvh_tree_release(ctx, ev.tree);
+                    if (ev.mesh) bvh_mesh_release(ctx, ev.mesh);
                     lk.lock();
                     in_use -= ev.bytes;
                     resident_bytes -= ev.bytes;
@@ -482,6 +486,16 @@ int bvh_server_create(const bvh_server_config* cfg, bvh_server_t* out) {
             return BVH_E_HIP;
         }
         s->cfg.memory_budget = fr / 10 * 9;
+    }
+    // Released meshes and trees keep their blocks in the library's device
+    // cache (at most a sixteenth of the budget here), and a reused block may
+    // be up to a sixteenth larger than the sizes admission counts: admission
+    // keeps both out of the budget.
+    s->cache_limit = s->cfg.memory_budget / 16;
+    s->admit_budget = s->cfg.memory_budget - 2 * s->cache_limit;
+    if (bvh_device_cache_set_limit(cfg->device, s->cache_limit) != BVH_OK) {
+        for (bvh_ctx_t x : s->ctxs) bvh_ctx_destroy(x);
+        return BVH_E_HIP;
     }
     bvh_server_rec* p = s.get();
     for (int i = 0; i < p->cfg.slots; ++i) p->threads.emplace_back([p, i] { p->run_slot(i); });

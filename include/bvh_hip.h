@@ -110,6 +110,20 @@ int bvh_host_free_pinned(void* ptr);
 /* soa[0..8] = v0x v0y v0z v1x v1y v1z v2x v2y v2z, each n floats (host memory). */
 int bvh_mesh_upload(bvh_ctx_t ctx, const float* const* soa, uint32_t n, bvh_mesh_t* out);
 int bvh_mesh_free(bvh_ctx_t ctx, bvh_mesh_t mesh);
+/* Like bvh_mesh_free for a caller that knows no queued work on any context
+ * uses the mesh any more (bvh_mesh_free waits for the whole device instead):
+ * waits for ctx's own stream only and keeps the mesh's device blocks in a
+ * per-device cache that later uploads and tree buffers of a similar size
+ * reuse (no reference counterpart: a serving loop evicting resident meshes
+ * is the caller it is for).  An allocation that finds no device memory frees
+ * the cache and retries; bvh_device_cache_trim frees it on demand. */
+int bvh_mesh_release(bvh_ctx_t ctx, bvh_mesh_t mesh);
+/* Cached bytes and blocks of a device, freeing them all, and the most the
+ * cache keeps (default an eighth of the device's memory; a release beyond it
+ * frees the blocks at once; 0 keeps nothing). */
+int bvh_device_cache_info(int32_t device, uint64_t* bytes, uint32_t* blocks);
+int bvh_device_cache_trim(int32_t device);
+int bvh_device_cache_set_limit(int32_t device, uint64_t bytes);
 int bvh_mesh_size(bvh_mesh_t mesh, uint32_t* n);
 /* BVH_E_NONFINITE when a vertex coordinate of the mesh is NaN or infinite
  * (found by the upload; every build of such a mesh is refused before any
@@ -119,6 +133,8 @@ int bvh_mesh_check(bvh_mesh_t mesh);
 /* ---- trees -------------------------------------------------------------- */
 int bvh_tree_create(bvh_ctx_t ctx, bvh_tree_t* out);
 int bvh_tree_free(bvh_ctx_t ctx, bvh_tree_t tree);
+/* bvh_mesh_release's counterpart for trees: no queued work may use it. */
+int bvh_tree_release(bvh_ctx_t ctx, bvh_tree_t tree);
 /* num_nodes = 2n-1 for every builder; num_prims = n. */
 int bvh_tree_info(bvh_tree_t tree, uint32_t* num_nodes, uint32_t* num_prims);
 /* nodes: num_nodes x 48-byte BVHNode records; indices: num_prims uint32 (either may be NULL). */

@@ -178,6 +178,67 @@ def same_hits(ref, near):
             "max_rel_t_difference": float(rel.max()) if rel.size else 0.0}
 
 
+def batched_vs_plain(ctx, steps, k=512, lo=1000, hi=200_000, seed=11):
+    """512 uneven meshes (log-uniform 1,000-200,000 triangles, seed 11, each a
+    uniform-cube mesh with its own seed, scale and offset): one captured
+    launch sequence per algorithm against 512 plain per-mesh builds, meshes/s
+    of each, and per-mesh bit-identity of the trees."""
+    import bvh_amd as B
+    counts = B.batch_counts(seed, k, lo, hi)
+    log(f"[bench] batched: generating {k} meshes, {int(counts.sum()):,} triangles")
+    meshes = [B.Mesh.batch_mesh(seed, i, int(c)) for i, c in enumerate(counts)]
+    dms = [ctx.upload(m) for m in meshes]
+    del meshes
+    out = {"meshes": k, "triangles": int(counts.sum()), "seed": seed}
+    for name, algo, radius in (("lbvh", B.ALGO_LBVH, 25), ("ploc_r16", B.ALGO_PLOC, 16)):
+        log(f"[bench] batched {name}: plain per-mesh builds")
+        plain_trees = [ctx.tree() for _ in dms]
+        for dm, t in zip(dms, plain_trees):  # warm (allocations)
+            ctx.build(dm, t, algo=algo, radius=radius)
+        reps = max(1, min(3, steps // 5))
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            for dm, t in zip(dms, plain_trees):
+                ctx.build(dm, t, algo=algo, radius=radius)
+        plain_s = (time.perf_counter() - t0) / reps
+        log(f"[bench] batched {name}: capture + tune")
+        trees = [ctx.tree() for _ in dms]
+        batch = ctx.batch(dms, trees, algo=algo, radius=radius, tune=True)
+        batch.run()
+        dev = []
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            dev.append(batch.run())
+        batch_s = (time.perf_counter() - t0) / steps
+        info = batch.info()
+        log(f"[bench] batched {name}: bit-identity check")
+        identical = True
+        for a, b in zip(trees, plain_trees):
+            na, ia = a.download()
+            nb, ib = b.download()
+            if na.tobytes() != nb.tobytes() or not np.array_equal(ia, ib):
+                identical = False
+                break
+        batch.free()
+        for t in trees + plain_trees:
+            t.free()
+        out[name] = {
+            "plain_ms": round(plain_s * 1e3, 2), "plain_meshes_s": round(k / plain_s, 1),
+            "batched_ms": round(batch_s * 1e3, 3), "batched_device_ms": round(float(np.median(dev)), 3),
+            "batched_meshes_s": round(k / batch_s, 1), "speedup": round(plain_s / batch_s, 2),
+            "batched_Mtri_s": round(int(counts.sum()) / batch_s / 1e6, 1),
+            "bit_identical_all_meshes": identical,
+            "graph_nodes": info["graph_nodes"], "radix_bits": info["radix_bits"], "sort_passes": info["sort_passes"],
+            "key_bits": info["key_bits"], "tune": info["tune"],
+        }
+        if algo == B.ALGO_PLOC:
+            out[name]["global_iterations"] = info["ploc_iterations"]
+            out[name]["nn_grid"] = info["ploc_nn_grid"]
+    for d in dms:
+        d.free()
+    return out
+
+
 def large_strands(ctx, steps=2):
     """32,000,000 triangles: 200,000 random-walk strands x 160 segments (step
     0.01, seed 99); PLOC r=100 and LBVH+ (3 rounds) builds, then 4,194,304

@@ -109,6 +109,8 @@ _HIP_SIGS = {
     "bvh_mesh_free": (_i32, [_vp, _vp]),
     "bvh_mesh_release": (_i32, [_vp, _vp]),
     "bvh_device_cache_info": (_i32, [_i32, C.POINTER(C.c_uint64), C.POINTER(C.c_uint32)]),
+    "bvh_mesh_bytes": (_i32, [_vp, C.POINTER(C.c_uint64)]),
+    "bvh_tree_bytes": (_i32, [_vp, C.POINTER(C.c_uint64)]),
     "bvh_device_cache_trim": (_i32, [_i32]),
     "bvh_device_cache_set_limit": (_i32, [_i32, C.c_uint64]),
     "bvh_mesh_size": (_i32, [_vp, C.POINTER(_u32)]),

@@ -185,9 +185,9 @@ int set_err(bvh_ctx_t ctx, int code, const char* fmt, ...) {
 // owner vouches that no queued work uses them) keep their device blocks here,
 // per device, instead of going to hipFree, which waits for the whole device;
 // a later mesh upload or new buffer takes the smallest cached block that is
-// large enough and at most 1/16 plus 64 KiB larger than asked for (a serving
-// loop admits work by the sizes it asks for, so the excess a reused block
-// brings must stay small against its memory headroom).  An
+// large enough and at most a quarter plus 2 MiB larger than asked for (a
+// wide window, so that most released blocks find a taker; bvh_mesh_bytes
+// and bvh_tree_bytes report what an object really holds).  An
 // allocation that fails frees every cached block of its device and tries
 // once more.  The cache holds at most `limit` bytes (default an eighth of the
 // device's memory); a released block beyond it is freed at once.
@@ -208,7 +208,7 @@ void* cache_take(int device, size_t want, size_t* cap) {
     BlockCache& c = block_cache(device);
     std::lock_guard<std::mutex> lk(c.mu);
     auto it = c.blocks.lower_bound(want);
-    if (it == c.blocks.end() || it->first > want + want / 16 + ((size_t)64 << 10)) return nullptr;
+    if (it == c.blocks.end() || it->first > want + want / 4 + ((size_t)2 << 20)) return nullptr;
     void* p = it->second;
     *cap = it->first;
     c.bytes -= it->first;
@@ -1574,6 +1574,18 @@ int bvh_mesh_release(bvh_ctx_t ctx, bvh_mesh_t mesh) {
     cache_give(ctx->device, mesh->base, mesh->base_cap);
     cache_give(ctx->device, mesh->tri, mesh->tri_cap);
     delete mesh;
+    return BVH_OK;
+}
+
+int bvh_mesh_bytes(bvh_mesh_t mesh, uint64_t* bytes) {
+    if (!mesh || !bytes) return BVH_E_ARG;
+    *bytes = mesh->base_cap + mesh->tri_cap;
+    return BVH_OK;
+}
+
+int bvh_tree_bytes(bvh_tree_t tree, uint64_t* bytes) {
+    if (!tree || !bytes) return BVH_E_ARG;
+    *bytes = tree->nodes.cap + tree->indices.cap + tree->rec.cap;
     return BVH_OK;
 }
 

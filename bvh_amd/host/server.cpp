@@ -344,6 +344,15 @@ void bvh_server_rec::publish_residents(int slot, std::vector<Req>& group, std::v
             q.res->mesh = q.mesh;
             q.res->tree = q.tree;
             q.res->ready = true;
+            // Count what the mesh and tree really hold (a block reused from
+            // the device cache may be larger than the estimate admitted).
+            uint64_t mb = 0, tb = 0;
+            if (bvh_mesh_bytes(q.mesh, &mb) == BVH_OK && bvh_tree_bytes(q.tree, &tb) == BVH_OK) {
+                in_use = in_use - q.res->bytes + (mb + tb);
+                resident_bytes = resident_bytes - q.res->bytes + (mb + tb);
+                q.res->bytes = mb + tb;
+                peak = std::max(peak, in_use + held_total);
+            }
         } else {
             if (q.tree) bvh_tree_release(ctx, q.tree);
             if (q.mesh) bvh_mesh_release(ctx, q.mesh);
@@ -488,11 +497,13 @@ int bvh_server_create(const bvh_server_config* cfg, bvh_server_t* out) {
         s->cfg.memory_budget = fr / 10 * 9;
     }
     // Released meshes and trees keep their blocks in the library's device
-    // cache (at most a sixteenth of the budget here), and a reused block may
-    // be up to a sixteenth larger than the sizes admission counts: admission
-    // keeps both out of the budget.
-    s->cache_limit = s->cfg.memory_budget / 16;
-    s->admit_budget = s->cfg.memory_budget - 2 * s->cache_limit;
+    // cache (at most an eighth of the budget here: a smaller cache fills with
+    // blocks no request fits and frees the rest with device-wide waits), and
+    // a group's reused blocks may hold more than admission estimated until its
+    // residents are counted at their real size (a sixteenth): admission keeps
+    // both out of the budget.
+    s->cache_limit = s->cfg.memory_budget / 8;
+    s->admit_budget = s->cfg.memory_budget - s->cache_limit - s->cfg.memory_budget / 16;
     if (bvh_device_cache_set_limit(cfg->device, s->cache_limit) != BVH_OK) {
         for (bvh_ctx_t x : s->ctxs) bvh_ctx_destroy(x);
         return BVH_E_HIP;

@@ -114,10 +114,16 @@ int bvh_mesh_free(bvh_ctx_t ctx, bvh_mesh_t mesh);
  * uses the mesh any more (bvh_mesh_free waits for the whole device instead):
  * waits for ctx's own stream only and keeps the mesh's device blocks in a
  * per-device cache that later uploads and tree buffers of a similar size
- * reuse (no reference counterpart: a serving loop evicting resident meshes
- * is the caller it is for).  An allocation that finds no device memory frees
+ * reuse when a block is at most a quarter plus 2 MiB larger than asked for
+ * (no reference counterpart: a serving loop evicting resident meshes is the
+ * caller it is for).  An allocation that finds no device memory frees
  * the cache and retries; bvh_device_cache_trim frees it on demand. */
 int bvh_mesh_release(bvh_ctx_t ctx, bvh_mesh_t mesh);
+/* Device bytes a mesh or a tree holds (a reused block may be larger than its
+ * contents need).  No reference counterpart: for callers that budget device
+ * memory, such as the serving loop. */
+int bvh_mesh_bytes(bvh_mesh_t mesh, uint64_t* bytes);
+int bvh_tree_bytes(bvh_tree_t tree, uint64_t* bytes);
 /* Cached bytes and blocks of a device, freeing them all, and the most the
  * cache keeps (default an eighth of the device's memory; a release beyond it
  * frees the blocks at once; 0 keeps nothing). */

@@ -37,9 +37,10 @@ typedef struct bvh_server_config {
     int32_t max_group;        /* requests coalesced into one group, >= 1 (1 = plain one-mesh-per-launch) */
     uint32_t max_group_tris;  /* triangles built by one group (0 = 16M); a larger single mesh runs alone */
     uint64_t memory_budget;   /* device bytes the server may hold (0 = 90% of the device's free memory);
-                                 a sixteenth is the device cache of evicted meshes' blocks
+                                 an eighth is the device cache of evicted meshes' blocks
                                  (bvh_device_cache_set_limit) and a sixteenth headroom for
-                                 reused blocks larger than asked for: admission uses 7/8 */
+                                 reused blocks larger than the estimates admitted (residents
+                                 are counted at their real size once built): admission uses 13/16 */
     int32_t algo;             /* BVH_ALGO_LBVH or BVH_ALGO_PLOC */
     int32_t ploc_radius;      /* PLOC window radius */
     int32_t trace_order;      /* BVH_TRACE_REFERENCE (0) or BVH_TRACE_NEAREST (bvh_hip.h) for every trace */

@@ -2,6 +2,8 @@
 device blocks go to the per-device cache, later uploads and trees of a
 similar size reuse them, and what is built in reused memory is bit-identical
 to the oracle and to a build in fresh memory."""
+import ctypes as C
+
 import numpy as np
 import pytest
 
@@ -35,6 +37,10 @@ def test_released_blocks_are_reused_bit_identically(ctx):
     rb = B.rays_sphere_to_box(402, b.bounds(), 30_000)
     dm2, tree2, got = _build_trace(ctx, b, rb)
     assert ctx.cache_info()[1] == 0, ctx.cache_info()
+    # The new objects report the released blocks' sizes.
+    mb, tb = C.c_uint64(), C.c_uint64()
+    assert ctx._lib.bvh_mesh_bytes(dm2.h, C.byref(mb)) == 0 and ctx._lib.bvh_tree_bytes(tree2.h, C.byref(tb)) == 0
+    assert mb.value + tb.value == cached
     # The same mesh built and traced in fresh memory on another context.
     other = B.Context(0)
     try:

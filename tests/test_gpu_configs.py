@@ -16,6 +16,9 @@
 * 512 meshes of 1,000-200,000 triangles (seed 11), LBVH and PLOC r = 16 in
   one tuned captured launch sequence: every tree bit-identical to its own
   standalone build, the smallest ones also to the oracle.
+* The nearest-first trace order on the same 4M and 32M cases against the
+  oracle's nearest-first restatement (ids and t bits), and against the
+  reference order: a different triangle only at an equal t.
 * Serving requests drawn from the serving workload's ranges (10,000 to
   2,000,000 triangles, 65,536 to 1,048,576 rays, seed 23): each result in the
   mix equal to the request alone.
@@ -51,6 +54,27 @@ def _compare_trace(ctx, tree, dm, nodes, mesh, rays, sample=None):
     assert np.all(np.abs(t[hit] - ot[hit]) <= T_REL * np.abs(ot[hit]))
     np.testing.assert_array_equal(t.view(np.uint32), ot.view(np.uint32))
     return hit.mean()
+
+
+def _compare_nearest(ctx, tree, dm, nodes, mesh, rays, sample=None):
+    """The nearest-first order (BVH_TRACE_NEAREST) against the oracle's, and
+    against the reference order's hits: the same triangle, or one at the
+    same t (an equal-t tie, which the two orders break differently)."""
+    rt, rp, _ = ctx.trace(tree, dm, rays)
+    ctx.set_trace_order(B.TRACE_NEAREST)
+    try:
+        t, prim, _ = ctx.trace(tree, dm, rays)
+    finally:
+        ctx.set_trace_order(B.TRACE_REFERENCE)
+    if sample is not None:
+        rays, t, prim, rt, rp = rays[sample], t[sample], prim[sample], rt[sample], rp[sample]
+    ot, op, _ = O.trace(nodes, mesh.soa, rays, nearest=True)
+    np.testing.assert_array_equal(prim, op)
+    np.testing.assert_array_equal(t.view(np.uint32), ot.view(np.uint32))
+    other = prim != rp
+    np.testing.assert_array_equal(t[other].view(np.uint32), rt[other].view(np.uint32))
+    assert other.mean() < 1e-3
+    return other.sum()
 
 
 # ---------------------------------------------------------------- 4M blobs --
@@ -90,6 +114,7 @@ def test_blobs_4m_ploc_r25(ctx, blobs4m):
         assert O.nodes_equal(nodes, onodes) is None
         rays = B.rays_sphere_camera(mesh.bounds(), 1024, 768)
         assert _compare_trace(ctx, tree, dm, onodes, mesh, rays) > 0.05
+        _compare_nearest(ctx, tree, dm, onodes, mesh, rays)
     finally:
         tree.free()
 
@@ -103,6 +128,7 @@ def test_blobs_4m_camera_rays_lbvh(ctx, blobs4m):
         assert O.nodes_equal(nodes, onodes) is None
         rays = B.rays_sphere_camera(mesh.bounds(), 1024, 768)
         assert _compare_trace(ctx, tree, dm, onodes, mesh, rays) > 0.05
+        _compare_nearest(ctx, tree, dm, onodes, mesh, rays)
     finally:
         tree.free()
 
@@ -151,6 +177,7 @@ def test_strands_32m_ploc_r100(ctx, strands32m):
         nodes, idx = tree.download()
         check_tree_properties(nodes, idx, tb)
         _compare_trace(ctx, tree, dm, nodes, mesh, rays, sample=sample)
+        _compare_nearest(ctx, tree, dm, nodes, mesh, rays, sample=sample)
     finally:
         tree.free()
 
@@ -167,6 +194,7 @@ def test_strands_32m_lbvh_plus_three_rounds(ctx, strands32m):
             costs.append(sah64(nodes))
             if rounds == 3:
                 _compare_trace(ctx, tree, dm, nodes, mesh, rays, sample=sample)
+                _compare_nearest(ctx, tree, dm, nodes, mesh, rays, sample=sample)
         finally:
             tree.free()
         del nodes

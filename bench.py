@@ -239,6 +239,34 @@ def batched_vs_plain(ctx, steps, k=512, lo=1000, hi=200_000, seed=11):
     return out
 
 
+def batched_eager(ctx, steps, k=512, lo=1000, hi=200_000, seed=11):
+    """The same 512 meshes built by the eager segmented path
+    (bvh_build_batched: the batch's kernels launched directly, no captured
+    graph), for kernel traces and counters of the segmented kernels."""
+    import bvh_amd as B
+    counts = B.batch_counts(seed, k, lo, hi)
+    log(f"[bench] batched (eager): generating {k} meshes, {int(counts.sum()):,} triangles")
+    meshes = [B.Mesh.batch_mesh(seed, i, int(c)) for i, c in enumerate(counts)]
+    dms = [ctx.upload(m) for m in meshes]
+    del meshes
+    out = {"meshes": k, "triangles": int(counts.sum()), "seed": seed}
+    for name, algo, radius in (("lbvh", B.ALGO_LBVH, 25), ("ploc_r16", B.ALGO_PLOC, 16)):
+        log(f"[bench] batched (eager) {name}")
+        trees = [ctx.tree() for _ in dms]
+        ctx.build_batched(dms, trees, algo=algo, radius=radius)
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            ctx.build_batched(dms, trees, algo=algo, radius=radius)
+        ms = (time.perf_counter() - t0) / steps * 1e3
+        for t in trees:
+            t.free()
+        out[name] = {"ms": round(ms, 3), "meshes_s": round(k / ms * 1e3, 1),
+                     "Mtri_s": round(int(counts.sum()) / ms / 1e3, 1)}
+    for d in dms:
+        d.free()
+    return out
+
+
 def large_strands(ctx, steps=2):
     """32,000,000 triangles: 200,000 random-walk strands x 160 segments (step
     0.01, seed 99); PLOC r=100 and LBVH+ (3 rounds) builds, then 4,194,304
@@ -464,7 +492,7 @@ def run_sections(args):
     """Selected workloads only, for profiling (kernel traces, counters)."""
     import bvh_amd as B
     want = {x.strip() for x in args.sections.split(",") if x.strip()}
-    unknown = want - {"companions", "batch", "large", "serve"}
+    unknown = want - {"companions", "batch", "batch_eager", "large", "serve"}
     if unknown:
         raise SystemExit(f"unknown sections: {sorted(unknown)}")
     out = {}
@@ -479,6 +507,8 @@ def run_sections(args):
         dm.free()
     if "batch" in want:
         out["batched_vs_plain"] = batched_vs_plain(ctx, args.steps)
+    if "batch_eager" in want:
+        out["batched_eager"] = batched_eager(ctx, args.steps)
     if "large" in want:
         out["large_32m_strands"] = large_strands(ctx, max(1, args.steps // 4))
     ctx.close()
@@ -502,7 +532,7 @@ def main():
                     help="only run the 256 MiB stream-copy kernel (reference point for HBM counters)")
     ap.add_argument("--serve-child", default="", help=argparse.SUPPRESS)
     ap.add_argument("--sections", default="",
-                    help="profiling: comma list of companions,batch,large,serve to run without the headline "
+                    help="profiling: comma list of companions,batch,batch_eager,large,serve to run without the headline "
                          "(prints their JSON only)")
     args = ap.parse_args()
     if args.serve_child:

@@ -3,11 +3,15 @@
 // Replaces traverseBVH / intersectAABB / intersectTriangle of the reference
 // renderer (src/render_engine.cu:194-356) for caller-supplied rays.
 //
-// Semantics are the reference's, exactly: depth-first, left child before
-// right, a node is entered only if its slab test passes with
-// tenter < (current closest t), Moller-Trumbore with the reference's
-// tolerances, closest hit by strict '<'.  Per-ray counters follow the
-// reference's definitions (nodes visited, AABB tests, triangle tests).
+// Two visit orders (a template switch, chosen per context):
+//   * the reference's, exactly: depth-first, left child before right, a node
+//     is entered only if its slab test passes with tenter < (current closest
+//     t), Moller-Trumbore with the reference's tolerances, closest hit by
+//     strict '<'.  Per-ray counters follow the reference's definitions
+//     (nodes visited, AABB tests, triangle tests);
+//   * nearest-first (no reference counterpart): the nearer of two children
+//     first, a node entered while tenter <= closest t, the closest hit the
+//     least (t, triangle index) pair; same predicates and counter rules.
 //
 // MI355X mapping:
 //   * children's slab tests run when their parent is expanded and only

@@ -243,6 +243,15 @@ void cache_trim(int device) {
     c.bytes = 0;
 }
 
+// Runs f with `device` current, then restores the caller's device.
+template <class F>
+int on_device(int32_t device, F f) {
+    int prev = 0;
+    if (hipGetDevice(&prev) != hipSuccess || hipSetDevice(device) != hipSuccess) return BVH_E_HIP;
+    f();
+    return hipSetDevice(prev) == hipSuccess ? BVH_OK : BVH_E_HIP;
+}
+
 // Device memory of at least `want` bytes (*cap = its size): a cached block or
 // a new allocation.
 hipError_t dev_alloc(int device, void** p, size_t want, size_t* cap) {
@@ -1525,6 +1534,7 @@ int bvh_mesh_upload(bvh_ctx_t ctx, const float* const* soa, uint32_t n, bvh_mesh
     if (n > (1u << 30)) return set_err(ctx, BVH_E_ARG, "mesh too large (%u > 2^30 triangles)", n);
     for (int k = 0; k < 9; ++k)
         if (!soa[k]) return set_err(ctx, BVH_E_ARG, "soa[%d] is null", k);
+    HIP_TRY(ctx, hipSetDevice(ctx->device));  // the mesh lives on the context's device
     bvh_mesh_rec* m = new bvh_mesh_rec();
     m->n = n;
     m->stride = ((size_t)n + 63) & ~(size_t)63;  // 256-byte aligned arrays
@@ -1600,7 +1610,6 @@ int bvh_device_cache_info(int32_t device, uint64_t* bytes, uint32_t* blocks) {
 
 int bvh_device_cache_set_limit(int32_t device, uint64_t bytes) {
     if (device < 0 || device >= kCacheDevices) return BVH_E_ARG;
-    if (hipSetDevice(device) != hipSuccess) return BVH_E_HIP;
     BlockCache& c = block_cache(device);
     std::vector<void*> drop;
     {
@@ -1613,15 +1622,14 @@ int bvh_device_cache_set_limit(int32_t device, uint64_t bytes) {
             c.blocks.erase(it);
         }
     }
-    for (void* p : drop) (void)hipFree(p);
-    return BVH_OK;
+    return on_device(device, [&] {
+        for (void* p : drop) (void)hipFree(p);
+    });
 }
 
 int bvh_device_cache_trim(int32_t device) {
     if (device < 0 || device >= kCacheDevices) return BVH_E_ARG;
-    if (hipSetDevice(device) != hipSuccess) return BVH_E_HIP;
-    cache_trim(device);
-    return BVH_OK;
+    return on_device(device, [&] { cache_trim(device); });
 }
 
 int bvh_mesh_free(bvh_ctx_t ctx, bvh_mesh_t mesh) {

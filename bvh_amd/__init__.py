@@ -275,6 +275,12 @@ class Context:
         """Frees the device's cached blocks (bvh_device_cache_trim)."""
         self._check(self._lib.bvh_device_cache_trim(self.device), "cache trim")
 
+    def cache_set_limit(self, nbytes: int) -> int:
+        """Most bytes the device's cache keeps (bvh_device_cache_set_limit); returns the previous limit."""
+        prev = C.c_uint64()
+        self._check(self._lib.bvh_device_cache_set_limit(self.device, int(nbytes), C.byref(prev)), "cache limit")
+        return int(prev.value)
+
     def set_trace_order(self, order: int):
         """Visit order of this context's traces (bvh_ctx_set_trace_order):
         TRACE_REFERENCE (default: the reference renderer's order, ids, t and

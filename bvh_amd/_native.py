@@ -112,7 +112,7 @@ _HIP_SIGS = {
     "bvh_mesh_bytes": (_i32, [_vp, C.POINTER(C.c_uint64)]),
     "bvh_tree_bytes": (_i32, [_vp, C.POINTER(C.c_uint64)]),
     "bvh_device_cache_trim": (_i32, [_i32]),
-    "bvh_device_cache_set_limit": (_i32, [_i32, C.c_uint64]),
+    "bvh_device_cache_set_limit": (_i32, [_i32, C.c_uint64, C.POINTER(C.c_uint64)]),
     "bvh_mesh_size": (_i32, [_vp, C.POINTER(_u32)]),
     "bvh_mesh_check": (_i32, [_vp]),
     "bvh_tree_create": (_i32, [_vp, _pp]),

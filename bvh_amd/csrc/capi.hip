@@ -90,6 +90,9 @@ struct bvh_context {
     int32_t trace_order = BVH_TRACE_REFERENCE;  // visit order of this context's traces
     uint32_t* host_flags = nullptr;          // pinned: a single build's 8 flag words land here
     std::vector<uint32_t*> pin_chunks;       // pinned status slots of asynchronous calls
+    char* stage[2] = {nullptr, nullptr};     // pinned staging buffers of host copies (kStageBytes each)
+    uint32_t* pin_small = nullptr;           // pinned words for small synchronous readbacks
+    hipEvent_t stage_ev[2] = {nullptr, nullptr};  // DMA from/to stage[i] done
     std::vector<Pending> pending;            // asynchronous calls since the last bvh_sync, in order
     // last build's sorted keys (for bvh_debug_sorted_keys)
     uint64_t* last_keys = nullptr;
@@ -1169,6 +1172,81 @@ int batch_tune(bvh_batch_t b) {
     return batch_apply(b, best);
 }
 
+// ---- host copies through pinned staging ---------------------------------------
+// Copies between pageable host memory and the device go through two pinned
+// buffers of the context: the calling thread copies one chunk into (or out
+// of) a pinned buffer while the DMA engine moves the other, so a copy runs
+// at the pinned DMA rate and does not depend on the runtime's pageable-copy
+// path (whose copies measured twice as slow once another server had run in
+// the same process).  Small copies go straight to hipMemcpyAsync.
+constexpr size_t kStageBytes = (size_t)8 << 20;
+constexpr size_t kStageMin = (size_t)256 << 10;
+
+int stage_ready(bvh_ctx_t ctx) {
+    for (int i = 0; i < 2; ++i) {
+        if (!ctx->stage[i]) HIP_TRY(ctx, hipHostMalloc((void**)&ctx->stage[i], kStageBytes, hipHostMallocDefault));
+        if (!ctx->stage_ev[i]) HIP_TRY(ctx, hipEventCreateWithFlags(&ctx->stage_ev[i], hipEventDisableTiming));
+    }
+    return BVH_OK;
+}
+
+// Reads `words` (at most 16) 32-bit words from the device after the work
+// queued on the context stream, through pinned memory.
+int read_words(bvh_ctx_t ctx, uint32_t* dst, const void* src, int words) {
+    if (!ctx->pin_small) HIP_TRY(ctx, hipHostMalloc((void**)&ctx->pin_small, 64, hipHostMallocDefault));
+    HIP_TRY(ctx, hipMemcpyAsync(ctx->pin_small, src, (size_t)words * 4, hipMemcpyDeviceToHost, ctx->stream));
+    HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+    std::memcpy(dst, ctx->pin_small, (size_t)words * 4);
+    return BVH_OK;
+}
+
+// Enqueues dst <- src (host) on the context stream; src may be reused as
+// soon as this returns.
+int stage_h2d(bvh_ctx_t ctx, void* dst, const void* src, size_t bytes) {
+    if (bytes < kStageMin) {
+        HIP_TRY(ctx, hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, ctx->stream));
+        return BVH_OK;
+    }
+    TRY(stage_ready(ctx));
+    int b = 0;
+    for (size_t off = 0; off < bytes; off += kStageBytes, b ^= 1) {
+        const size_t len = std::min(kStageBytes, bytes - off);
+        HIP_TRY(ctx, hipEventSynchronize(ctx->stage_ev[b]));  // the previous DMA out of this buffer is done
+        std::memcpy(ctx->stage[b], static_cast<const char*>(src) + off, len);
+        HIP_TRY(ctx, hipMemcpyAsync(static_cast<char*>(dst) + off, ctx->stage[b], len, hipMemcpyHostToDevice,
+                                    ctx->stream));
+        HIP_TRY(ctx, hipEventRecord(ctx->stage_ev[b], ctx->stream));
+    }
+    return BVH_OK;
+}
+
+// dst (host) <- src after the work already queued on the context stream;
+// returns when dst holds the data.
+int stage_d2h(bvh_ctx_t ctx, void* dst, const void* src, size_t bytes) {
+    if (bytes < kStageMin) {
+        HIP_TRY(ctx, hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, ctx->stream));
+        HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+        return BVH_OK;
+    }
+    TRY(stage_ready(ctx));
+    const size_t chunks = (bytes + kStageBytes - 1) / kStageBytes;
+    auto issue = [&](size_t c) -> int {
+        const size_t off = c * kStageBytes, len = std::min(kStageBytes, bytes - off);
+        HIP_TRY(ctx, hipMemcpyAsync(ctx->stage[c & 1], static_cast<const char*>(src) + off, len,
+                                    hipMemcpyDeviceToHost, ctx->stream));
+        HIP_TRY(ctx, hipEventRecord(ctx->stage_ev[c & 1], ctx->stream));
+        return BVH_OK;
+    };
+    TRY(issue(0));
+    for (size_t c = 0; c < chunks; ++c) {
+        if (c + 1 < chunks) TRY(issue(c + 1));  // its buffer was emptied by chunk c - 1's copy below
+        HIP_TRY(ctx, hipEventSynchronize(ctx->stage_ev[c & 1]));
+        const size_t off = c * kStageBytes, len = std::min(kStageBytes, bytes - off);
+        std::memcpy(static_cast<char*>(dst) + off, ctx->stage[c & 1], len);
+    }
+    return BVH_OK;
+}
+
 // Builds the tree's traversal records once per build (complete on return, so
 // other contexts may trace the tree afterwards).
 int prepare_trace(bvh_ctx_t ctx, bvh_tree_t tree) {
@@ -1480,6 +1558,11 @@ int bvh_ctx_destroy(bvh_ctx_t ctx) {
     free_buf(ctx->trace_io);
     if (ctx->host_flags) (void)hipHostFree(ctx->host_flags);
     for (uint32_t* p : ctx->pin_chunks) (void)hipHostFree(p);
+    if (ctx->pin_small) (void)hipHostFree(ctx->pin_small);
+    for (int i = 0; i < 2; ++i) {
+        if (ctx->stage[i]) (void)hipHostFree(ctx->stage[i]);
+        if (ctx->stage_ev[i]) (void)hipEventDestroy(ctx->stage_ev[i]);
+    }
     for (auto& e : ctx->ev)
         if (e) (void)hipEventDestroy(e);
     (void)hipStreamDestroy(ctx->stream);
@@ -1546,11 +1629,12 @@ int bvh_mesh_upload(bvh_ctx_t ctx, const float* const* soa, uint32_t n, bvh_mesh
         return set_err(ctx, BVH_E_NOMEM, "mesh allocation failed: %s", hipGetErrorString(e));
     }
     for (int k = 0; k < 9; ++k) {
-        e = hipMemcpyAsync(m->base + k * m->stride, soa[k], (size_t)n * 4, hipMemcpyHostToDevice, ctx->stream);
-        if (e != hipSuccess) {
+        const int st = stage_h2d(ctx, m->base + k * m->stride, soa[k], (size_t)n * 4);
+        if (st != BVH_OK) {
+            (void)hipStreamSynchronize(ctx->stream);
             (void)hipFree(m->base);
             delete m;
-            return set_err(ctx, BVH_E_HIP, "mesh upload failed: %s", hipGetErrorString(e));
+            return st;
         }
     }
     // The traversal copy is followed by one word that the copy kernel sets
@@ -1565,8 +1649,8 @@ int bvh_mesh_upload(bvh_ctx_t ctx, const float* const* soa, uint32_t n, bvh_mesh
     launch_fill_zero(bad, 1, ctx->stream);
     launch_tri_aos(m->view(), m->tri, bad, ctx->stream);
     uint32_t hbad = 0;
-    if (hipMemcpyAsync(&hbad, bad, 4, hipMemcpyDeviceToHost, ctx->stream) != hipSuccess ||
-        hipStreamSynchronize(ctx->stream) != hipSuccess) {
+    if (read_words(ctx, &hbad, bad, 1) != BVH_OK) {
+        (void)hipStreamSynchronize(ctx->stream);
         (void)hipFree(m->base);
         (void)hipFree(m->tri);
         delete m;
@@ -1608,12 +1692,19 @@ int bvh_device_cache_info(int32_t device, uint64_t* bytes, uint32_t* blocks) {
     return BVH_OK;
 }
 
-int bvh_device_cache_set_limit(int32_t device, uint64_t bytes) {
+int bvh_device_cache_set_limit(int32_t device, uint64_t bytes, uint64_t* previous) {
     if (device < 0 || device >= kCacheDevices) return BVH_E_ARG;
     BlockCache& c = block_cache(device);
     std::vector<void*> drop;
     {
         std::lock_guard<std::mutex> lk(c.mu);
+        if (previous) {
+            if (c.limit == 0) {
+                size_t fr = 0, tot = 0;
+                c.limit = hipMemGetInfo(&fr, &tot) == hipSuccess ? tot / 8 : ((uint64_t)8 << 30);
+            }
+            *previous = c.limit;
+        }
         c.limit = bytes ? bytes : 1;  // 0 would mean "not set": keep nothing instead
         while (c.bytes > c.limit && !c.blocks.empty()) {  // largest blocks first
             auto it = std::prev(c.blocks.end());
@@ -2005,8 +2096,7 @@ int bvh_trace(bvh_ctx_t ctx, bvh_tree_t tree, bvh_mesh_t mesh, const float* d_ra
     TRY(trace_enqueue(ctx, tree, mesh, d_rays, nrays, d_t, d_prim, d_counters, &ovf_dev));
     HIP_TRY(ctx, hipEventRecord(ctx->ev[1], ctx->stream));
     uint32_t ovf[2] = {0, 0};
-    HIP_TRY(ctx, hipMemcpyAsync(ovf, ovf_dev, 8, hipMemcpyDeviceToHost, ctx->stream));
-    HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+    TRY(read_words(ctx, ovf, ovf_dev, 2));
     if (ms) HIP_TRY(ctx, hipEventElapsedTime(ms, ctx->ev[0], ctx->ev[1]));
     if (ovf[1]) return set_err(ctx, BVH_E_STATE, "traversal needed more than 256 stack entries (or the nodes are not a tree)");
     return BVH_OK;
@@ -2040,11 +2130,11 @@ int bvh_trace_host(bvh_ctx_t ctx, bvh_tree_t tree, bvh_mesh_t mesh, const float*
     int32_t* d_prim = reinterpret_cast<int32_t*>(reinterpret_cast<char*>(d_t) + ((tb + 63) & ~(size_t)63));
     int32_t* d_cnt = counters ? reinterpret_cast<int32_t*>(reinterpret_cast<char*>(d_prim) + ((tb + 63) & ~(size_t)63))
                               : nullptr;
-    HIP_TRY(ctx, hipMemcpyAsync(d_rays, rays, rb, hipMemcpyHostToDevice, ctx->stream));
+    if ((st = stage_h2d(ctx, d_rays, rays, rb))) return st;
     if ((st = bvh_trace(ctx, tree, mesh, d_rays, nrays, d_t, d_prim, d_cnt, ms))) return st;
-    HIP_TRY(ctx, hipMemcpyAsync(t, d_t, tb, hipMemcpyDeviceToHost, ctx->stream));
-    HIP_TRY(ctx, hipMemcpyAsync(prim, d_prim, tb, hipMemcpyDeviceToHost, ctx->stream));
-    if (counters) HIP_TRY(ctx, hipMemcpyAsync(counters, d_cnt, cb, hipMemcpyDeviceToHost, ctx->stream));
+    if ((st = stage_d2h(ctx, t, d_t, tb))) return st;
+    if ((st = stage_d2h(ctx, prim, d_prim, tb))) return st;
+    if (counters && (st = stage_d2h(ctx, counters, d_cnt, cb))) return st;
     HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
     return BVH_OK;
 }

@@ -89,6 +89,7 @@ struct bvh_server_rec {
     // the group it runs).  Free memory = budget - in_use - held_total.
     uint64_t in_use = 0, peak = 0, resident_bytes = 0, lru = 0, held_total = 0;
     uint64_t cache_limit = 0, admit_budget = 0;  // device-cache share and what admission may use
+    uint64_t cache_limit_before = 0;             // the device cache's limit before this server (restored at destroy)
     std::vector<uint64_t> held_build, held_trace;
     uint32_t in_flight = 0, groups = 0, completed = 0, evictions = 0, rejected = 0, max_group_seen = 0;
     std::string err;
@@ -504,7 +505,7 @@ int bvh_server_create(const bvh_server_config* cfg, bvh_server_t* out) {
     // both out of the budget.
     s->cache_limit = s->cfg.memory_budget / 8;
     s->admit_budget = s->cfg.memory_budget - s->cache_limit - s->cfg.memory_budget / 16;
-    if (bvh_device_cache_set_limit(cfg->device, s->cache_limit) != BVH_OK) {
+    if (bvh_device_cache_set_limit(cfg->device, s->cache_limit, &s->cache_limit_before) != BVH_OK) {
         for (bvh_ctx_t x : s->ctxs) bvh_ctx_destroy(x);
         return BVH_E_HIP;
     }
@@ -599,6 +600,7 @@ int bvh_server_destroy(bvh_server_t s) {
     }
     s->residents.clear();
     for (bvh_ctx_t x : s->ctxs) bvh_ctx_destroy(x);
+    if (s->cache_limit_before) (void)bvh_device_cache_set_limit(s->cfg.device, s->cache_limit_before, nullptr);
     delete s;
     return BVH_OK;
 }

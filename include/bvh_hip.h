@@ -126,10 +126,11 @@ int bvh_mesh_bytes(bvh_mesh_t mesh, uint64_t* bytes);
 int bvh_tree_bytes(bvh_tree_t tree, uint64_t* bytes);
 /* Cached bytes and blocks of a device, freeing them all, and the most the
  * cache keeps (default an eighth of the device's memory; a release beyond it
- * frees the blocks at once; 0 keeps nothing). */
+ * frees the blocks at once; 0 keeps nothing; *previous, if given, receives
+ * the limit it replaces). */
 int bvh_device_cache_info(int32_t device, uint64_t* bytes, uint32_t* blocks);
 int bvh_device_cache_trim(int32_t device);
-int bvh_device_cache_set_limit(int32_t device, uint64_t bytes);
+int bvh_device_cache_set_limit(int32_t device, uint64_t bytes, uint64_t* previous);
 int bvh_mesh_size(bvh_mesh_t mesh, uint32_t* n);
 /* BVH_E_NONFINITE when a vertex coordinate of the mesh is NaN or infinite
  * (found by the upload; every build of such a mesh is refused before any

@@ -13,6 +13,15 @@ import oracle as O
 pytestmark = pytest.mark.gpu
 
 
+@pytest.fixture(autouse=True)
+def roomy_cache(ctx):
+    """A cache limit these tests fit in, whatever ran before; restored after."""
+    prev = ctx.cache_set_limit(4 << 30)
+    yield
+    ctx.cache_trim()
+    ctx.cache_set_limit(prev)
+
+
 def _build_trace(ctx, mesh, rays, algo=B.ALGO_LBVH, **kw):
     dm = ctx.upload(mesh)
     tree = ctx.tree()

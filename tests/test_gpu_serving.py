@@ -87,6 +87,16 @@ def test_serving_resident_followups_and_eviction(ctx):
         np.testing.assert_array_equal(prim, prim_ref)
 
 
+def test_server_restores_the_cache_limit(ctx):
+    before = ctx.cache_set_limit(3 << 30)
+    try:
+        with S.Server(algo=B.ALGO_LBVH, slots=1, budget=1 << 20):
+            pass
+        assert ctx.cache_set_limit(3 << 30) == 3 << 30   # the server's 1/8 of 1 MiB is gone again
+    finally:
+        ctx.cache_set_limit(before)
+
+
 def test_serving_rejections(ctx):
     with S.Server(algo=B.ALGO_LBVH, slots=1, budget=1 << 20) as srv:
         m = B.Mesh.uniform(3, 100_000)

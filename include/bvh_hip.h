@@ -92,7 +92,9 @@ int bvh_device_count(int* count);
 int bvh_ctx_create(int device, bvh_ctx_t* out);
 int bvh_ctx_destroy(bvh_ctx_t ctx);
 const char* bvh_last_error(bvh_ctx_t ctx);
-/* Free and total device memory of the context's device (hipMemGetInfo). */
+/* Free and total device memory of the context's device (hipMemGetInfo; free
+ * includes what the library's device pool holds unused, see
+ * bvh_device_cache_info). */
 int bvh_device_memory(bvh_ctx_t ctx, uint64_t* free_bytes, uint64_t* total_bytes);
 const char* bvh_status_string(int status);
 
@@ -127,7 +129,10 @@ int bvh_tree_bytes(bvh_tree_t tree, uint64_t* bytes);
 /* Cached bytes and blocks of a device, freeing them all, and the most the
  * cache keeps (default an eighth of the device's memory; a release beyond it
  * frees the blocks at once; 0 keeps nothing; *previous, if given, receives
- * the limit it replaces). */
+ * the limit it replaces).  Freed blocks go back to a per-device memory pool
+ * that keeps the memory for later allocations (no driver call per block);
+ * bvh_device_cache_trim also hands the pool's unused memory back to the
+ * device. */
 int bvh_device_cache_info(int32_t device, uint64_t* bytes, uint32_t* blocks);
 int bvh_device_cache_trim(int32_t device);
 int bvh_device_cache_set_limit(int32_t device, uint64_t bytes, uint64_t* previous);

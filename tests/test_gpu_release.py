@@ -1,7 +1,8 @@
 """Released meshes and trees (bvh_mesh_release, bvh_tree_release): their
 device blocks go to the per-device cache, later uploads and trees of a
 similar size reuse them, and what is built in reused memory is bit-identical
-to the oracle and to a build in fresh memory."""
+to the oracle and to a build in fresh memory.  Freed blocks return to the
+device pool, whose unused memory bvh_device_memory counts as free."""
 import ctypes as C
 
 import numpy as np
@@ -101,3 +102,32 @@ def test_ploc_tree_in_reused_memory_matches_oracle(ctx):
     assert O.nodes_equal(nodes, onodes) is None
     np.testing.assert_array_equal(idx, oidx)
     ctx.cache_trim()
+
+
+def _free_bytes(ctx):
+    f, t = C.c_uint64(), C.c_uint64()
+    assert ctx._lib.bvh_device_memory(ctx.h, C.byref(f), C.byref(t)) == 0
+    return f.value, t.value
+
+
+def test_freed_blocks_stay_counted_as_free(ctx):
+    # Freed device blocks go back to the library's device pool, which keeps
+    # the memory; bvh_device_memory counts it as free, before and after the
+    # trim hands it back to the device.  Other processes on the device move
+    # the figure too, so the tolerance is generous against a 168 MB mesh.
+    ctx.cache_trim()
+    before, total = _free_bytes(ctx)
+    m = B.Mesh.uniform(406, 2_000_000)        # 72 MB of coordinates + 96 MB copy
+    dm = ctx.upload(m)
+    during, _ = _free_bytes(ctx)
+    assert during <= before - 150_000_000
+    dm.free()
+    after_free, _ = _free_bytes(ctx)
+    assert after_free >= before - 32_000_000
+    ctx.cache_trim()
+    after_trim, _ = _free_bytes(ctx)
+    assert after_trim >= before - 32_000_000 and after_trim <= total
+    # The pool's memory is reused: the same upload again fits where it was.
+    dm = ctx.upload(m)
+    assert _free_bytes(ctx)[0] <= before - 150_000_000
+    dm.free()
